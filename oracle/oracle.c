@@ -1,0 +1,417 @@
+/* CPU oracle for the BLS12-381 Fr prover operators -- TEST INFRASTRUCTURE.
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg load
+ * this file's library; the product path never does.  See oracle.h for what
+ * each function restates and how tests pin it.
+ *
+ * Arithmetic is 4 x 64-bit limbs with unsigned __int128 (CIOS Montgomery),
+ * deliberately a different limb size and algorithm from the GPU's 8 x 32-bit
+ * FIPS product so that the two cannot share a mistake.
+ */
+#include "oracle.h"
+#include <stdlib.h>
+#include <string.h>
+
+typedef unsigned __int128 u128;
+
+/* r, PLONK/utils/mont/cpu/ff/bls12-381.hpp:7-11 */
+static const uint64_t P[4] = {0xffffffff00000001ull, 0x53bda402fffe5bfeull,
+                              0x3339d80809a1d805ull, 0x73eda753299d7d48ull};
+/* -r^-1 mod 2^64, cpu/ff/bls12-381.hpp:24 */
+static const uint64_t N0 = 0xfffffffeffffffffull;
+/* R^2 mod r, cpu/ff/bls12-381.hpp:12-16 */
+static const uint64_t RR[4] = {0xc999e990f3f29c6dull, 0x2b6cedcb87925c23ull,
+                               0x05d314967254398full, 0x0748d9d99f59ff11ull};
+/* R mod r, cpu/ff/bls12-381.hpp:17-21 */
+static const uint64_t ONE[4] = {0x00000001fffffffeull, 0x5884b7fa00034802ull,
+                                0x998c4fefecbc4ff5ull, 0x1824b159acc5056full};
+
+static int geq_p(const uint64_t* a) {
+  for (int i = 3; i >= 0; i--) {
+    if (a[i] > P[i]) return 1;
+    if (a[i] < P[i]) return 0;
+  }
+  return 1;
+}
+
+static void sub_p(uint64_t* a) {
+  uint64_t br = 0;
+  for (int i = 0; i < 4; i++) {
+    u128 d = (u128)a[i] - P[i] - br;
+    a[i] = (uint64_t)d;
+    br = (uint64_t)(d >> 64) & 1;
+  }
+}
+
+int or_is_canonical(const uint64_t* a) { return !geq_p(a); }
+
+void or_add(uint64_t* o, const uint64_t* a, const uint64_t* b) {
+  uint64_t t[4];
+  uint64_t c = 0;
+  for (int i = 0; i < 4; i++) {
+    u128 s = (u128)a[i] + b[i] + c;
+    t[i] = (uint64_t)s;
+    c = (uint64_t)(s >> 64);
+  }
+  if (c || geq_p(t)) sub_p(t);
+  memcpy(o, t, 32);
+}
+
+void or_sub(uint64_t* o, const uint64_t* a, const uint64_t* b) {
+  uint64_t t[4];
+  uint64_t br = 0;
+  for (int i = 0; i < 4; i++) {
+    u128 d = (u128)a[i] - b[i] - br;
+    t[i] = (uint64_t)d;
+    br = (uint64_t)(d >> 64) & 1;
+  }
+  if (br) {
+    uint64_t c = 0;
+    for (int i = 0; i < 4; i++) {
+      u128 s = (u128)t[i] + P[i] + c;
+      t[i] = (uint64_t)s;
+      c = (uint64_t)(s >> 64);
+    }
+  }
+  memcpy(o, t, 32);
+}
+
+void or_neg(uint64_t* o, const uint64_t* a) {
+  static const uint64_t Z[4] = {0, 0, 0, 0};
+  or_sub(o, Z, a);
+}
+
+/* CIOS Montgomery product a*b*2^-256 mod r */
+void or_mul(uint64_t* o, const uint64_t* a, const uint64_t* b) {
+  uint64_t t[6] = {0, 0, 0, 0, 0, 0};
+  for (int i = 0; i < 4; i++) {
+    uint64_t C = 0;
+    for (int j = 0; j < 4; j++) {
+      u128 s = (u128)a[j] * b[i] + t[j] + C;
+      t[j] = (uint64_t)s;
+      C = (uint64_t)(s >> 64);
+    }
+    u128 s = (u128)t[4] + C;
+    t[4] = (uint64_t)s;
+    t[5] = (uint64_t)(s >> 64);
+    uint64_t m = t[0] * N0;
+    s = (u128)m * P[0] + t[0];
+    C = (uint64_t)(s >> 64);
+    for (int j = 1; j < 4; j++) {
+      s = (u128)m * P[j] + t[j] + C;
+      t[j - 1] = (uint64_t)s;
+      C = (uint64_t)(s >> 64);
+    }
+    s = (u128)t[4] + C;
+    t[3] = (uint64_t)s;
+    t[4] = t[5] + (uint64_t)(s >> 64);
+  }
+  if (t[4] || geq_p(t)) sub_p(t);
+  memcpy(o, t, 32);
+}
+
+void or_one(uint64_t* o) { memcpy(o, ONE, 32); }
+
+void or_to_mont(uint64_t* o, const uint64_t* a) { or_mul(o, a, RR); }
+
+void or_from_mont(uint64_t* o, const uint64_t* a) {
+  static const uint64_t U1[4] = {1, 0, 0, 0};
+  or_mul(o, a, U1);
+}
+
+void or_from_u64(uint64_t* o, uint64_t v) {
+  uint64_t t[4] = {v, 0, 0, 0};
+  or_to_mont(o, t);
+}
+
+void or_pow(uint64_t* o, const uint64_t* a, const uint64_t* e) {
+  uint64_t acc[4], base[4];
+  memcpy(acc, ONE, 32);
+  memcpy(base, a, 32);
+  for (int i = 0; i < 256; i++) {
+    if ((e[i >> 6] >> (i & 63)) & 1) or_mul(acc, acc, base);
+    or_mul(base, base, base);
+  }
+  memcpy(o, acc, 32);
+}
+
+void or_inv(uint64_t* o, const uint64_t* a) {
+  /* Fermat: a^(r-2); 0^(r-2) = 0 gives the reference's x/0 = 0
+   * (mont_t.cuh:1117-1126) */
+  uint64_t e[4] = {P[0] - 2, P[1], P[2], P[3]};
+  or_pow(o, a, e);
+}
+
+void or_vec_add(uint64_t* o, const uint64_t* a, const uint64_t* b, uint64_t n) {
+  for (uint64_t i = 0; i < n; i++) or_add(o + 4 * i, a + 4 * i, b + 4 * i);
+}
+void or_vec_sub(uint64_t* o, const uint64_t* a, const uint64_t* b, uint64_t n) {
+  for (uint64_t i = 0; i < n; i++) or_sub(o + 4 * i, a + 4 * i, b + 4 * i);
+}
+void or_vec_mul(uint64_t* o, const uint64_t* a, const uint64_t* b, uint64_t n) {
+  for (uint64_t i = 0; i < n; i++) or_mul(o + 4 * i, a + 4 * i, b + 4 * i);
+}
+void or_vec_inv(uint64_t* o, const uint64_t* a, uint64_t n) {
+  for (uint64_t i = 0; i < n; i++) or_inv(o + 4 * i, a + 4 * i);
+}
+
+void or_group_gen(uint64_t* o) { or_from_u64(o, 7); }
+
+/* omega_{2^32} = 7^((r-1)/2^32); omega_{2^lg} = omega_{2^32}^(2^(32-lg)).
+ * Same closed form as forward_roots_of_unity in parameters/bls12_381.h. */
+void or_root_of_unity(uint64_t* o, int lg) {
+  uint64_t g[4];
+  or_group_gen(g);
+  /* (r-1) >> 32 */
+  uint64_t rm1[4] = {P[0] - 1, P[1], P[2], P[3]};
+  uint64_t e[4];
+  for (int i = 0; i < 4; i++) {
+    e[i] = rm1[i] >> 32;
+    if (i < 3) e[i] |= rm1[i + 1] << 32;
+  }
+  uint64_t w[4];
+  or_pow(w, g, e);
+  for (int i = lg; i < 32; i++) or_mul(w, w, w);
+  memcpy(o, w, 32);
+}
+
+void or_domain_size_inverse(uint64_t* o, int lg) {
+  uint64_t two_k[4];
+  uint64_t t[4] = {0, 0, 0, 0};
+  t[lg >> 6] = 1ull << (lg & 63);
+  or_to_mont(two_k, t);
+  or_inv(o, two_k);
+}
+
+uint64_t or_bitrev(uint64_t v, int bits) {
+  uint64_t r = 0;
+  for (int i = 0; i < bits; i++) {
+    r = (r << 1) | (v & 1);
+    v >>= 1;
+  }
+  return r;
+}
+
+static uint64_t splitmix(uint64_t* s) {
+  uint64_t z = (*s += 0x9E3779B97F4A7C15ull);
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+void or_fill_uniform(uint64_t* out, uint64_t n, uint64_t seed) {
+  uint64_t s = seed;
+  for (uint64_t i = 0; i < n; i++) {
+    uint64_t l[4];
+    do {
+      for (int j = 0; j < 4; j++) l[j] = splitmix(&s);
+      l[3] &= 0x7fffffffffffffffull;
+    } while (geq_p(l));
+    memcpy(out + 4 * i, l, 32);
+  }
+}
+
+uint64_t or_seed(int op, int lg, int aux) {
+  return 0x504C4E4B00000000ull ^ ((uint64_t)op << 40) ^ ((uint64_t)lg << 8) ^ (uint64_t)aux;
+}
+
+/* ---- NTT: bit-reverse, then radix-2 DIT (natural in, natural out) ------ */
+static void bitrev_permute(uint64_t* d, int lg) {
+  uint64_t n = 1ull << lg;
+  for (uint64_t i = 0; i < n; i++) {
+    uint64_t j = or_bitrev(i, lg);
+    if (j > i) {
+      uint64_t t[4];
+      memcpy(t, d + 4 * i, 32);
+      memcpy(d + 4 * i, d + 4 * j, 32);
+      memcpy(d + 4 * j, t, 32);
+    }
+  }
+}
+
+static void ntt_with_root(uint64_t* d, int lg, const uint64_t* w) {
+  uint64_t n = 1ull << lg;
+  if (lg == 0) return;
+  bitrev_permute(d, lg);
+  /* tw[j] = w^j, j < n/2 */
+  uint64_t half = n >> 1;
+  uint64_t* tw = (uint64_t*)malloc(32 * half);
+  memcpy(tw, ONE, 32);
+  for (uint64_t j = 1; j < half; j++) or_mul(tw + 4 * j, tw + 4 * (j - 1), w);
+  for (int s = 0; s < lg; s++) {
+    uint64_t h = 1ull << s;           /* half block */
+    uint64_t stride = half >> s;      /* w_{2h}^j = w^(j*stride) */
+    for (uint64_t blk = 0; blk < n; blk += 2 * h) {
+      for (uint64_t j = 0; j < h; j++) {
+        uint64_t* a = d + 4 * (blk + j);
+        uint64_t* b = d + 4 * (blk + j + h);
+        uint64_t t[4];
+        or_mul(t, b, tw + 4 * (j * stride));
+        or_sub(b, a, t);
+        or_add(a, a, t);
+      }
+    }
+  }
+  free(tw);
+}
+
+void or_ntt(uint64_t* data, int lg, int inverse) {
+  uint64_t w[4];
+  or_root_of_unity(w, lg);
+  if (inverse) or_inv(w, w);
+  ntt_with_root(data, lg, w);
+  if (inverse) {
+    uint64_t ninv[4];
+    or_domain_size_inverse(ninv, lg);
+    uint64_t n = 1ull << lg;
+    for (uint64_t i = 0; i < n; i++) or_mul(data + 4 * i, data + 4 * i, ninv);
+  }
+}
+
+void or_coset_ntt(uint64_t* data, int lg, const uint64_t* shift) {
+  uint64_t n = 1ull << lg;
+  uint64_t p[4];
+  memcpy(p, ONE, 32);
+  for (uint64_t i = 0; i < n; i++) {
+    or_mul(data + 4 * i, data + 4 * i, p);
+    or_mul(p, p, shift);
+  }
+  or_ntt(data, lg, 0);
+}
+
+/* out[j*B + t] = coset_NTT_N(c; shift*w_M^t)[j], B = 2^lg_blowup */
+void or_lde(const uint64_t* coeffs, uint64_t* out, int lg_n, int lg_blowup, const uint64_t* shift) {
+  uint64_t n = 1ull << lg_n;
+  uint64_t B = 1ull << lg_blowup;
+  uint64_t wM[4];
+  or_root_of_unity(wM, lg_n + lg_blowup);
+  uint64_t* tmp = (uint64_t*)malloc(32 * n);
+  uint64_t st[4];
+  memcpy(st, shift, 32);
+  for (uint64_t t = 0; t < B; t++) {
+    memcpy(tmp, coeffs, 32 * n);
+    or_coset_ntt(tmp, lg_n, st);
+    for (uint64_t j = 0; j < n; j++) memcpy(out + 4 * (j * B + t), tmp + 4 * j, 32);
+    or_mul(st, st, wM);
+  }
+  free(tmp);
+}
+
+void or_naive_lde(const uint64_t* coeffs, uint64_t* out, int lg_n, int lg_blowup) {
+  uint64_t n = 1ull << lg_n;
+  uint64_t M = n << lg_blowup;
+  uint64_t* tmp = (uint64_t*)malloc(32 * n);
+  memcpy(tmp, coeffs, 32 * n);
+  or_ntt(tmp, lg_n, 0);
+  for (uint64_t k = 0; k < M; k++) memcpy(out + 4 * k, tmp + 4 * (k & (n - 1)), 32);
+  free(tmp);
+}
+
+void or_grand_product(uint64_t* data, uint64_t n) {
+  uint64_t z[4];
+  memcpy(z, ONE, 32);
+  for (uint64_t i = 0; i < n; i++) {
+    uint64_t p[4];
+    memcpy(p, data + 4 * i, 32);
+    memcpy(data + 4 * i, z, 32);
+    or_mul(z, z, p);
+  }
+}
+
+void or_naive_gp(const uint64_t* in, uint64_t* out, uint64_t n) {
+  if (n == 0) return;
+  memset(out, 0, 32 * n);
+  uint64_t st[4];
+  memcpy(st, ONE, 32);
+  memcpy(out, st, 32);
+  for (uint64_t i = 0; i + 1 < n; i++) {
+    or_mul(st, st, in + 4 * i);
+    memcpy(out + 4 * i, st, 32);
+  }
+}
+
+void or_eval(const uint64_t* coeffs, uint64_t n, const uint64_t* x, uint64_t* y) {
+  uint64_t acc[4] = {0, 0, 0, 0};
+  for (uint64_t i = n; i-- > 0;) {
+    or_mul(acc, acc, x);
+    or_add(acc, acc, coeffs + 4 * i);
+  }
+  memcpy(y, acc, 32);
+}
+
+void or_div(uint64_t* data, uint64_t n, const uint64_t* c, uint64_t* rem) {
+  if (n == 0) return;
+  uint64_t z[4];
+  or_neg(z, c);
+  uint64_t s[4];
+  memcpy(s, data + 4 * (n - 1), 32);
+  memset(data + 4 * (n - 1), 0, 32);
+  for (uint64_t i = n - 1; i-- > 0;) {
+    uint64_t a[4];
+    memcpy(a, data + 4 * i, 32);
+    memcpy(data + 4 * i, s, 32);
+    or_mul(s, s, z);
+    or_add(s, s, a);
+  }
+  if (rem) memcpy(rem, s, 32);
+}
+
+static int is_zero4(const uint64_t* a) { return (a[0] | a[1] | a[2] | a[3]) == 0; }
+
+void or_naive_div(const uint64_t* in, uint64_t n, const uint64_t* c, const uint64_t* d1, uint64_t* q) {
+  if (n < 2) return;
+  uint64_t* rem = (uint64_t*)malloc(32 * n);
+  memcpy(rem, in, 32 * n);
+  memset(q, 0, 32 * (n - 1));
+  uint64_t inv_lead[4];
+  or_inv(inv_lead, d1);
+  uint64_t len = n;
+  while (len >= 2) {
+    uint64_t cur[4], t[4];
+    or_mul(cur, rem + 4 * (len - 1), inv_lead);
+    memcpy(q + 4 * (len - 2), cur, 32);
+    or_mul(t, cur, c);
+    or_sub(rem + 4 * (len - 2), rem + 4 * (len - 2), t);
+    or_mul(t, cur, d1);
+    or_sub(rem + 4 * (len - 1), rem + 4 * (len - 1), t);
+    /* the reference reads rem[-1] once every coefficient is zero
+     * (miss_ops.cu:82-84); stop at an empty remainder instead */
+    while (len > 0 && is_zero4(rem + 4 * (len - 1))) len--;
+  }
+  free(rem);
+}
+
+void or_perm_ratio(const uint64_t* a, const uint64_t* b, const uint64_t* c,
+                   const uint64_t* s1, const uint64_t* s2, const uint64_t* s3,
+                   const uint64_t* beta, const uint64_t* gamma,
+                   const uint64_t* k1, const uint64_t* k2, const uint64_t* k3,
+                   int lg_n, uint64_t* ratio) {
+  uint64_t n = 1ull << lg_n;
+  uint64_t w[4], x[4];
+  or_root_of_unity(w, lg_n);
+  memcpy(x, ONE, 32);
+  const uint64_t* W[3] = {a, b, c};
+  const uint64_t* S[3] = {s1, s2, s3};
+  const uint64_t* K[3] = {k1, k2, k3};
+  for (uint64_t i = 0; i < n; i++) {
+    uint64_t num[4], den[4], t[4];
+    memcpy(num, ONE, 32);
+    memcpy(den, ONE, 32);
+    for (int j = 0; j < 3; j++) {
+      /* w_j + beta*k_j*omega^i + gamma */
+      or_mul(t, beta, K[j]);
+      or_mul(t, t, x);
+      or_add(t, t, W[j] + 4 * i);
+      or_add(t, t, gamma);
+      or_mul(num, num, t);
+      /* w_j + beta*s_j + gamma */
+      or_mul(t, beta, S[j] + 4 * i);
+      or_add(t, t, W[j] + 4 * i);
+      or_add(t, t, gamma);
+      or_mul(den, den, t);
+    }
+    or_inv(den, den);
+    or_mul(ratio + 4 * i, num, den);
+    or_mul(x, x, w);
+  }
+}

@@ -1,0 +1,281 @@
+// BLS12-381 scalar field Fr in Montgomery form, for gfx950 (CDNA4) VALU.
+//
+// An element is 8 little-endian 32-bit limbs (32 bytes), always canonical:
+// value in [0, r), stored as x*R mod r with R = 2^256.  The byte layout is the
+// one of the reference's `cuda::BLS12_381_Fr_G1` (8 x u32) and
+// `cpu::BLS12_381_Fr_G1` (4 x u64): PLONK/utils/mont/cuda/ff/bls12-381.hpp:7-45,
+// PLONK/utils/mont/cpu/ff/bls12-381.hpp:6-32.
+//
+// Arithmetic (all results canonical, bit-identical to any other exact
+// implementation):
+//   add  : 8-limb add chain, trial subtraction of r, select      (reference mont_t.cuh:215-222)
+//   sub  : 8-limb sub chain, masked add-back of r                (reference mont_t.cuh:267-290)
+//   mul  : Montgomery product a*b*R^-1 mod r, finely integrated
+//          product scanning (FIPS) over 32-bit limbs.  Each limb product
+//          is one v_mad_u64_u32 (64-bit accumulate, carry-out to an SGPR
+//          lane mask) plus one v_addc_co_u32 folding that carry into a
+//          third accumulator word.  r = 1 (mod 2^32), so the Montgomery
+//          digit is m = -t (mod 2^32) and m*r0 = m needs no multiply.
+//                                                                (reference mont_t.cuh:396-419)
+// v_mad_u64_u32 runs at the full VALU rate on gfx950, so the carry handling
+// is as expensive as the multiplies; FIPS keeps it to one op per product.
+#pragma once
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace pl {
+
+struct __align__(16) fr {
+  uint32_t l[8];
+};
+
+// r = 0x73eda753299d7d483339d80809a1d80553bda402fffe5bfeffffffff00000001
+#define PL_R0 0x00000001u
+#define PL_R1 0xffffffffu
+#define PL_R2 0xfffe5bfeu
+#define PL_R3 0x53bda402u
+#define PL_R4 0x09a1d805u
+#define PL_R5 0x3339d808u
+#define PL_R6 0x299d7d48u
+#define PL_R7 0x73eda753u
+
+__host__ __device__ constexpr uint32_t r_limb(int i) {
+  return i == 0 ? PL_R0 : i == 1 ? PL_R1 : i == 2 ? PL_R2 : i == 3 ? PL_R3
+       : i == 4 ? PL_R4 : i == 5 ? PL_R5 : i == 6 ? PL_R6 : PL_R7;
+}
+
+// R mod r (Montgomery one)
+__host__ __device__ constexpr uint32_t one_limb(int i) {
+  return i == 0 ? 0xfffffffeu : i == 1 ? 0x00000001u : i == 2 ? 0x00034802u
+       : i == 3 ? 0x5884b7fau : i == 4 ? 0xecbc4ff5u : i == 5 ? 0x998c4fefu
+       : i == 6 ? 0xacc5056fu : 0x1824b159u;
+}
+
+__device__ __forceinline__ fr fr_zero() {
+  fr z;
+#pragma unroll
+  for (int i = 0; i < 8; i++) z.l[i] = 0;
+  return z;
+}
+
+__device__ __forceinline__ fr fr_one() {
+  fr z;
+#pragma unroll
+  for (int i = 0; i < 8; i++) z.l[i] = one_limb(i);
+  return z;
+}
+
+__device__ __forceinline__ bool fr_is_zero(const fr& a) {
+  uint32_t o = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) o |= a.l[i];
+  return o == 0;
+}
+
+__device__ __forceinline__ bool fr_eq(const fr& a, const fr& b) {
+  uint32_t o = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) o |= a.l[i] ^ b.l[i];
+  return o == 0;
+}
+
+// ---- carry-chain primitives (one VALU op each) ---------------------------
+// A carry is a wave64 lane mask in an SGPR pair.
+typedef uint64_t lanemask;
+
+__device__ __forceinline__ uint32_t add_co(uint32_t a, uint32_t b, lanemask& co) {
+  uint32_t d;
+  asm("v_add_co_u32_e64 %0, %1, %2, %3" : "=v"(d), "=s"(co) : "v"(a), "v"(b));
+  return d;
+}
+__device__ __forceinline__ uint32_t addc_co(uint32_t a, uint32_t b, lanemask ci, lanemask& co) {
+  uint32_t d;
+  asm("v_addc_co_u32_e64 %0, %1, %2, %3, %4" : "=v"(d), "=s"(co) : "v"(a), "v"(b), "s"(ci));
+  return d;
+}
+__device__ __forceinline__ uint32_t addc_co_s(uint32_t a, uint32_t b_uniform, lanemask ci, lanemask& co) {
+  uint32_t d;
+  asm("v_addc_co_u32_e64 %0, %1, %2, %3, %4" : "=v"(d), "=s"(co) : "v"(a), "s"(b_uniform), "s"(ci));
+  return d;
+}
+__device__ __forceinline__ uint32_t sub_co(uint32_t a, uint32_t b, lanemask& bo) {
+  uint32_t d;
+  asm("v_sub_co_u32_e64 %0, %1, %2, %3" : "=v"(d), "=s"(bo) : "v"(a), "v"(b));
+  return d;
+}
+__device__ __forceinline__ uint32_t subb_co(uint32_t a, uint32_t b, lanemask bi, lanemask& bo) {
+  uint32_t d;
+  asm("v_subb_co_u32_e64 %0, %1, %2, %3, %4" : "=v"(d), "=s"(bo) : "v"(a), "v"(b), "s"(bi));
+  return d;
+}
+__device__ __forceinline__ uint32_t sub_co_s(uint32_t a, uint32_t b_uniform, lanemask& bo) {
+  uint32_t d;
+  asm("v_sub_co_u32_e64 %0, %1, %2, %3" : "=v"(d), "=s"(bo) : "v"(a), "s"(b_uniform));
+  return d;
+}
+__device__ __forceinline__ uint32_t subb_co_s(uint32_t a, uint32_t b_uniform, lanemask bi, lanemask& bo) {
+  uint32_t d;
+  asm("v_subb_co_u32_e64 %0, %1, %2, %3, %4" : "=v"(d), "=s"(bo) : "v"(a), "s"(b_uniform), "s"(bi));
+  return d;
+}
+// select: mask bit set -> t, else f
+__device__ __forceinline__ uint32_t sel(uint32_t f, uint32_t t, lanemask m) {
+  uint32_t d;
+  asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(d) : "v"(f), "v"(t), "s"(m));
+  return d;
+}
+
+// acc(64) += x*y with carry-out of the 64-bit sum
+__device__ __forceinline__ uint64_t mad_co(uint32_t x, uint32_t y, uint64_t acc, lanemask& co) {
+  uint64_t d;
+  asm("v_mad_u64_u32 %0, %1, %2, %3, %4" : "=v"(d), "=s"(co) : "v"(x), "v"(y), "v"(acc));
+  return d;
+}
+__device__ __forceinline__ uint64_t mad_co_s(uint32_t x, uint32_t y_uniform, uint64_t acc, lanemask& co) {
+  uint64_t d;
+  asm("v_mad_u64_u32 %0, %1, %2, %3, %4" : "=v"(d), "=s"(co) : "v"(x), "s"(y_uniform), "v"(acc));
+  return d;
+}
+// ext + carry
+__device__ __forceinline__ uint32_t inc_c(uint32_t ext, lanemask c) {
+  uint32_t d;
+  lanemask dummy;
+  asm("v_addc_co_u32_e64 %0, %1, %2, 0, %3" : "=v"(d), "=s"(dummy) : "v"(ext), "s"(c));
+  return d;
+}
+// 0/1 from a carry mask
+__device__ __forceinline__ uint32_t bit_c(lanemask c) {
+  uint32_t d;
+  asm("v_cndmask_b32_e64 %0, 0, 1, %1" : "=v"(d) : "s"(c));
+  return d;
+}
+
+// ---- field ops -----------------------------------------------------------
+
+// a - r if a >= r (a < 2r, a fits 256 bits)
+__device__ __forceinline__ fr fr_reduce_once(const fr& a) {
+  fr d;
+  lanemask b;
+  d.l[0] = sub_co_s(a.l[0], PL_R0, b);
+#pragma unroll
+  for (int i = 1; i < 8; i++) d.l[i] = subb_co_s(a.l[i], r_limb(i), b, b);
+  fr o;
+#pragma unroll
+  for (int i = 0; i < 8; i++) o.l[i] = sel(d.l[i], a.l[i], b);  // borrow -> a < r -> keep a
+  return o;
+}
+
+__device__ __forceinline__ fr fr_add(const fr& a, const fr& b) {
+  fr s;
+  lanemask c;
+  s.l[0] = add_co(a.l[0], b.l[0], c);
+#pragma unroll
+  for (int i = 1; i < 8; i++) s.l[i] = addc_co(a.l[i], b.l[i], c, c);
+  // a + b < 2r < 2^256: no carry out of limb 7
+  return fr_reduce_once(s);
+}
+
+__device__ __forceinline__ fr fr_sub(const fr& a, const fr& b) {
+  fr d;
+  lanemask bo;
+  d.l[0] = sub_co(a.l[0], b.l[0], bo);
+#pragma unroll
+  for (int i = 1; i < 8; i++) d.l[i] = subb_co(a.l[i], b.l[i], bo, bo);
+  // borrow -> add r back
+  fr o;
+  lanemask c;
+  uint32_t z = 0;
+  o.l[0] = add_co(d.l[0], sel(z, r_limb(0), bo), c);
+#pragma unroll
+  for (int i = 1; i < 8; i++) o.l[i] = addc_co(d.l[i], sel(z, r_limb(i), bo), c, c);
+  return o;
+}
+
+__device__ __forceinline__ fr fr_neg(const fr& a) {
+  return fr_sub(fr_zero(), a);
+}
+
+// Montgomery multiplication, FIPS.  acc is a 96-bit column accumulator
+// (lo64 = acc, top word = ext).
+__device__ __forceinline__ fr fr_mul(const fr& a, const fr& b) {
+  uint32_t m[8];
+  uint64_t acc = 0;
+  uint32_t ext = 0;
+  bool ext_zero = true;   // compile-time after unrolling
+  lanemask c;
+#define PL_MAC(ACC_EXPR)                                   \
+  do {                                                     \
+    acc = (ACC_EXPR);                                      \
+    ext = ext_zero ? bit_c(c) : inc_c(ext, c);             \
+    ext_zero = false;                                      \
+  } while (0)
+#pragma unroll
+  for (int k = 0; k < 8; k++) {
+#pragma unroll
+    for (int i = 0; i < k; i++)            // m_i * r_{k-i}, r index >= 1
+      PL_MAC(mad_co_s(m[i], r_limb(k - i), acc, c));
+#pragma unroll
+    for (int i = 0; i <= k; i++)           // a_i * b_{k-i}
+      PL_MAC(mad_co(a.l[i], b.l[k - i], acc, c));
+    // Montgomery digit: m_k = -lo (mod 2^32) since -r^-1 = -1 (mod 2^32).
+    // lo + m_k*r0 = lo + m_k = 0 (mod 2^32) with carry (lo != 0).
+    uint32_t lo = (uint32_t)acc;
+    uint32_t hi = (uint32_t)(acc >> 32);
+    m[k] = 0u - lo;
+    lanemask cc;
+    (void)add_co(lo, m[k], cc);
+    uint32_t nhi = addc_co(hi, 0u, cc, cc);
+    uint32_t next = ext_zero ? bit_c(cc) : inc_c(ext, cc);
+    acc = ((uint64_t)next << 32) | nhi;
+    ext_zero = true;
+  }
+  fr t;
+#pragma unroll
+  for (int k = 8; k < 15; k++) {
+#pragma unroll
+    for (int i = k - 7; i < 8; i++)
+      PL_MAC(mad_co_s(m[i], r_limb(k - i), acc, c));
+#pragma unroll
+    for (int i = k - 7; i < 8; i++)
+      PL_MAC(mad_co(a.l[i], b.l[k - i], acc, c));
+    t.l[k - 8] = (uint32_t)acc;
+    acc = ((uint64_t)ext << 32) | (uint32_t)(acc >> 32);
+    ext_zero = true;
+  }
+#undef PL_MAC
+  t.l[7] = (uint32_t)acc;
+  // result < 2r < 2^256, so the bits above limb 7 are zero here
+  return fr_reduce_once(t);
+}
+
+__device__ __forceinline__ fr fr_sqr(const fr& a) { return fr_mul(a, a); }
+
+// a^e, e a 64-bit exponent (square and multiply, MSB first)
+__device__ __forceinline__ fr fr_pow_u64(const fr& a, uint64_t e) {
+  fr r = fr_one();
+  if (e == 0) return r;
+  int top = 63 - __builtin_clzll(e);
+  r = a;
+  for (int i = top - 1; i >= 0; i--) {
+    r = fr_sqr(r);
+    if ((e >> i) & 1) r = fr_mul(r, a);
+  }
+  return r;
+}
+
+// ---- memory helpers: one element = two 16-byte vector accesses -----------
+__device__ __forceinline__ fr fr_load(const fr* p) {
+  const uint4* q = reinterpret_cast<const uint4*>(p);
+  uint4 x = q[0], y = q[1];
+  fr r;
+  r.l[0] = x.x; r.l[1] = x.y; r.l[2] = x.z; r.l[3] = x.w;
+  r.l[4] = y.x; r.l[5] = y.y; r.l[6] = y.z; r.l[7] = y.w;
+  return r;
+}
+__device__ __forceinline__ void fr_store(fr* p, const fr& a) {
+  uint4* q = reinterpret_cast<uint4*>(p);
+  q[0] = make_uint4(a.l[0], a.l[1], a.l[2], a.l[3]);
+  q[1] = make_uint4(a.l[4], a.l[5], a.l[6], a.l[7]);
+}
+
+}  // namespace pl
