@@ -1,0 +1,73 @@
+/* libpl: BLS12-381 Fr prover operators for AMD Instinct MI355X (gfx950).
+ *
+ * C-ABI boundary of the library.  Every data buffer is device memory owned by
+ * the caller, holding canonical Montgomery-form Fr elements of 32 bytes
+ * (8 x u32 little-endian limbs == 4 x u64 limbs), 16-byte aligned.  The
+ * library owns only domain tables (immutable after creation, shareable across
+ * streams) and, when the caller passes no workspace, stream-ordered scratch.
+ *
+ * Calls are asynchronous on the given stream (NULL = the default stream) and
+ * thread-safe for distinct streams.  Every entry point returns PL_OK (0) or a
+ * negative pl_status; nothing in the library exits the process.
+ *
+ * The names in brackets are the reference interfaces each entry point
+ * replaces (file:line in zlyber/ppopp26-artifact).
+ */
+#ifndef PL_H
+#define PL_H
+#include <stddef.h>
+#include <stdint.h>
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+/* ABI-identical to hipStream_t */
+typedef struct ihipStream_t* pl_stream_t;
+
+typedef struct pl_domain pl_domain;
+typedef struct pl_lde_domain pl_lde_domain;
+
+enum pl_status {
+  PL_OK = 0,
+  PL_ERR_ARG = -1,         /* bad size, alignment or null pointer */
+  PL_ERR_HIP = -2,         /* a HIP runtime call failed */
+  PL_ERR_UNSUPPORTED = -3, /* size outside the supported range */
+  PL_ERR_NOMEM = -4,       /* device allocation failed */
+  PL_ERR_NO_DEVICE = -5    /* no HIP device visible */
+};
+
+const char* pl_status_string(int status);
+/* library version, major*10000 + minor*100 + patch */
+int pl_version(void);
+/* the HIP error behind the last PL_ERR_HIP / PL_ERR_NOMEM on this thread */
+const char* pl_last_hip_error(void);
+
+/* ---- NTT domains ------------------------------------------------------
+ * [Ntt::Ntt / Intt::Intt, PLONK/utils/function.cu:278,328;
+ *  params_zkp_cuda, PLONK/utils/zkp/cuda/zksnark_ntt.cu:17-39;
+ *  NTTParameters, zksnark_ntt/parameters/parameters.cuh:142-222]
+ * Twiddle tables for forward and inverse NTTs of size 2^lg, 1 <= lg <= 28,
+ * resident in HBM on the current device. */
+int pl_domain_create(int lg, pl_domain** out);
+int pl_domain_destroy(pl_domain* d);
+int pl_domain_lg(const pl_domain* d);
+size_t pl_domain_bytes(const pl_domain* d);
+
+/* ---- NTT / iNTT -------------------------------------------------------
+ * [Ntt::forward / Ntt::forward_, function.cu:280-286;
+ *  Intt::forward / Intt::forward_, function.cu:328-336;
+ *  ntt_zkp_cuda, zksnark_ntt.cu:342-381]
+ * X[k] = sum_j x[j] w^(jk), w = 7^((r-1)/2^lg), natural order in and out.
+ * inverse != 0: w^-1 and a final factor 2^-lg.  d_in may equal d_out.
+ * d_work: NULL, or pl_ntt_work_bytes(lg, 1) bytes of device scratch. */
+int pl_ntt(const void* d_in, void* d_out, int lg, int inverse, const pl_domain* d,
+           void* d_work, pl_stream_t stream);
+/* npoly contiguous polynomials of size 2^lg in one launch sequence */
+int pl_ntt_batch(const void* d_in, void* d_out, int lg, uint32_t npoly, int inverse,
+                 const pl_domain* d, void* d_work, pl_stream_t stream);
+size_t pl_ntt_work_bytes(int lg, uint32_t npoly);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

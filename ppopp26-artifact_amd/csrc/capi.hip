@@ -1,0 +1,109 @@
+// C-ABI entry points of libpl (declared in include/pl.h).
+#include <string.h>
+
+#include <new>
+
+#include "pl_internal.hpp"
+
+struct pl_domain {
+  pl::Domain d;
+};
+
+namespace {
+
+thread_local char g_last_error[256] = "";
+
+int hip_fail(hipError_t e) {
+  snprintf(g_last_error, sizeof(g_last_error), "%s", hipGetErrorString(e));
+  return e == hipErrorOutOfMemory ? PL_ERR_NOMEM : PL_ERR_HIP;
+}
+
+bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
+
+}  // namespace
+
+extern "C" {
+
+const char* pl_status_string(int status) {
+  switch (status) {
+    case PL_OK: return "ok";
+    case PL_ERR_ARG: return "invalid argument";
+    case PL_ERR_HIP: return "HIP runtime error";
+    case PL_ERR_UNSUPPORTED: return "unsupported size";
+    case PL_ERR_NOMEM: return "device out of memory";
+    case PL_ERR_NO_DEVICE: return "no HIP device";
+    default: return "unknown status";
+  }
+}
+
+int pl_version(void) { return 100; }
+
+const char* pl_last_hip_error(void) { return g_last_error; }
+
+int pl_domain_create(int lg, pl_domain** out) {
+  if (!out || lg < 1 || lg > 28) return lg > 28 ? PL_ERR_UNSUPPORTED : PL_ERR_ARG;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return PL_ERR_NO_DEVICE;
+  pl_domain* d = new (std::nothrow) pl_domain();
+  if (!d) return PL_ERR_NOMEM;
+  hipGetDevice(&d->d.device);
+  hipError_t e = pl::domain_build(d->d, lg, nullptr);
+  if (e != hipSuccess) {
+    pl::domain_free(d->d);
+    delete d;
+    return hip_fail(e);
+  }
+  *out = d;
+  return PL_OK;
+}
+
+int pl_domain_destroy(pl_domain* d) {
+  if (!d) return PL_ERR_ARG;
+  pl::domain_free(d->d);
+  delete d;
+  return PL_OK;
+}
+
+int pl_domain_lg(const pl_domain* d) { return d ? d->d.lg : PL_ERR_ARG; }
+
+size_t pl_domain_bytes(const pl_domain* d) { return d ? d->d.bytes : 0; }
+
+size_t pl_ntt_work_bytes(int lg, uint32_t npoly) {
+  if (lg < 1 || lg > 28) return 0;
+  pl::NttPlan p = pl::make_plan(lg);
+  return p.P >= 2 ? ((size_t)npoly << lg) * sizeof(pl::fr) : 0;
+}
+
+int pl_ntt_batch(const void* d_in, void* d_out, int lg, uint32_t npoly, int inverse, const pl_domain* d,
+                 void* d_work, pl_stream_t stream) {
+  if (!d || !d_in || !d_out || npoly == 0) return PL_ERR_ARG;
+  if (d->d.lg != lg) return PL_ERR_ARG;
+  if (!aligned16(d_in) || !aligned16(d_out) || !aligned16(d_work)) return PL_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  size_t wb = pl_ntt_work_bytes(lg, npoly);
+  pl::fr* tmp = (pl::fr*)d_work;
+  bool own = false;
+  if (wb && !tmp) {
+    hipError_t e = hipMallocAsync((void**)&tmp, wb, s);
+    if (e != hipSuccess) return hip_fail(e);
+    own = true;
+  }
+  pl::NttCall c;
+  c.in = (const pl::fr*)d_in;
+  c.out = (pl::fr*)d_out;
+  c.tmp = tmp;
+  c.in_pstride = 1ull << lg;
+  c.npoly = npoly;
+  c.inverse = inverse != 0;
+  hipError_t e = pl::ntt_launch(d->d, c, s);
+  if (own) hipFreeAsync(tmp, s);
+  return e == hipSuccess ? PL_OK : hip_fail(e);
+}
+
+int pl_ntt(const void* d_in, void* d_out, int lg, int inverse, const pl_domain* d, void* d_work,
+           pl_stream_t stream) {
+  return pl_ntt_batch(d_in, d_out, lg, 1, inverse, d, d_work, stream);
+}
+
+}  // extern "C"

@@ -23,7 +23,20 @@
 #include <hip/hip_runtime.h>
 #include <stdint.h>
 
+#include <type_traits>
+
 namespace pl {
+
+// Compile-time loop: f(std::integral_constant<int, I>) for I in [B, E).
+// Used wherever a loop indexes register arrays with field ops in its body;
+// the unroller's size threshold is too small for those bodies.
+template <int B, int E, typename F>
+__device__ __forceinline__ void static_for(F&& f) {
+  if constexpr (B < E) {
+    f(std::integral_constant<int, B>{});
+    static_for<B + 1, E>(f);
+  }
+}
 
 struct __align__(16) fr {
   uint32_t l[8];
@@ -79,51 +92,13 @@ __device__ __forceinline__ bool fr_eq(const fr& a, const fr& b) {
   return o == 0;
 }
 
-// ---- carry-chain primitives (one VALU op each) ---------------------------
-// A carry is a wave64 lane mask in an SGPR pair.
+// ---- carry-chain primitives ----------------------------------------------
+// Add/sub chains use the compiler's carry builtins (it picks VOP2/VOP3
+// encodings within the constant-bus limit and keeps r's limbs in VGPRs).
+// Only the multiply-accumulate needs inline asm: v_mad_u64_u32's carry-out
+// of the 64-bit sum is not reachable from C.  A carry there is a wave64 lane
+// mask in an SGPR pair.
 typedef uint64_t lanemask;
-
-__device__ __forceinline__ uint32_t add_co(uint32_t a, uint32_t b, lanemask& co) {
-  uint32_t d;
-  asm("v_add_co_u32_e64 %0, %1, %2, %3" : "=v"(d), "=s"(co) : "v"(a), "v"(b));
-  return d;
-}
-__device__ __forceinline__ uint32_t addc_co(uint32_t a, uint32_t b, lanemask ci, lanemask& co) {
-  uint32_t d;
-  asm("v_addc_co_u32_e64 %0, %1, %2, %3, %4" : "=v"(d), "=s"(co) : "v"(a), "v"(b), "s"(ci));
-  return d;
-}
-__device__ __forceinline__ uint32_t addc_co_s(uint32_t a, uint32_t b_uniform, lanemask ci, lanemask& co) {
-  uint32_t d;
-  asm("v_addc_co_u32_e64 %0, %1, %2, %3, %4" : "=v"(d), "=s"(co) : "v"(a), "s"(b_uniform), "s"(ci));
-  return d;
-}
-__device__ __forceinline__ uint32_t sub_co(uint32_t a, uint32_t b, lanemask& bo) {
-  uint32_t d;
-  asm("v_sub_co_u32_e64 %0, %1, %2, %3" : "=v"(d), "=s"(bo) : "v"(a), "v"(b));
-  return d;
-}
-__device__ __forceinline__ uint32_t subb_co(uint32_t a, uint32_t b, lanemask bi, lanemask& bo) {
-  uint32_t d;
-  asm("v_subb_co_u32_e64 %0, %1, %2, %3, %4" : "=v"(d), "=s"(bo) : "v"(a), "v"(b), "s"(bi));
-  return d;
-}
-__device__ __forceinline__ uint32_t sub_co_s(uint32_t a, uint32_t b_uniform, lanemask& bo) {
-  uint32_t d;
-  asm("v_sub_co_u32_e64 %0, %1, %2, %3" : "=v"(d), "=s"(bo) : "v"(a), "s"(b_uniform));
-  return d;
-}
-__device__ __forceinline__ uint32_t subb_co_s(uint32_t a, uint32_t b_uniform, lanemask bi, lanemask& bo) {
-  uint32_t d;
-  asm("v_subb_co_u32_e64 %0, %1, %2, %3, %4" : "=v"(d), "=s"(bo) : "v"(a), "s"(b_uniform), "s"(bi));
-  return d;
-}
-// select: mask bit set -> t, else f
-__device__ __forceinline__ uint32_t sel(uint32_t f, uint32_t t, lanemask m) {
-  uint32_t d;
-  asm("v_cndmask_b32_e64 %0, %1, %2, %3" : "=v"(d) : "v"(f), "v"(t), "s"(m));
-  return d;
-}
 
 // acc(64) += x*y with carry-out of the 64-bit sum
 __device__ __forceinline__ uint64_t mad_co(uint32_t x, uint32_t y, uint64_t acc, lanemask& co) {
@@ -155,39 +130,35 @@ __device__ __forceinline__ uint32_t bit_c(lanemask c) {
 // a - r if a >= r (a < 2r, a fits 256 bits)
 __device__ __forceinline__ fr fr_reduce_once(const fr& a) {
   fr d;
-  lanemask b;
-  d.l[0] = sub_co_s(a.l[0], PL_R0, b);
+  uint32_t b = 0;
 #pragma unroll
-  for (int i = 1; i < 8; i++) d.l[i] = subb_co_s(a.l[i], r_limb(i), b, b);
+  for (int i = 0; i < 8; i++) d.l[i] = __builtin_subc(a.l[i], r_limb(i), b, &b);
   fr o;
 #pragma unroll
-  for (int i = 0; i < 8; i++) o.l[i] = sel(d.l[i], a.l[i], b);  // borrow -> a < r -> keep a
+  for (int i = 0; i < 8; i++) o.l[i] = b ? a.l[i] : d.l[i];  // borrow: a < r, keep a
   return o;
 }
 
 __device__ __forceinline__ fr fr_add(const fr& a, const fr& b) {
   fr s;
-  lanemask c;
-  s.l[0] = add_co(a.l[0], b.l[0], c);
+  uint32_t c = 0;
 #pragma unroll
-  for (int i = 1; i < 8; i++) s.l[i] = addc_co(a.l[i], b.l[i], c, c);
+  for (int i = 0; i < 8; i++) s.l[i] = __builtin_addc(a.l[i], b.l[i], c, &c);
   // a + b < 2r < 2^256: no carry out of limb 7
   return fr_reduce_once(s);
 }
 
 __device__ __forceinline__ fr fr_sub(const fr& a, const fr& b) {
   fr d;
-  lanemask bo;
-  d.l[0] = sub_co(a.l[0], b.l[0], bo);
+  uint32_t bo = 0;
 #pragma unroll
-  for (int i = 1; i < 8; i++) d.l[i] = subb_co(a.l[i], b.l[i], bo, bo);
-  // borrow -> add r back
+  for (int i = 0; i < 8; i++) d.l[i] = __builtin_subc(a.l[i], b.l[i], bo, &bo);
+  // borrow: add r back under a mask
+  const uint32_t mask = 0u - bo;
   fr o;
-  lanemask c;
-  uint32_t z = 0;
-  o.l[0] = add_co(d.l[0], sel(z, r_limb(0), bo), c);
+  uint32_t c = 0;
 #pragma unroll
-  for (int i = 1; i < 8; i++) o.l[i] = addc_co(d.l[i], sel(z, r_limb(i), bo), c, c);
+  for (int i = 0; i < 8; i++) o.l[i] = __builtin_addc(d.l[i], r_limb(i) & mask, c, &c);
   return o;
 }
 
@@ -197,7 +168,10 @@ __device__ __forceinline__ fr fr_neg(const fr& a) {
 
 // Montgomery multiplication, FIPS.  acc is a 96-bit column accumulator
 // (lo64 = acc, top word = ext).
-__device__ __forceinline__ fr fr_mul(const fr& a, const fr& b) {
+// UB = true: b is wave-uniform (compile-time constant or kernel argument) and
+// is read from SGPRs, saving VGPRs and v_mov traffic.
+template <bool UB>
+__device__ __forceinline__ fr fr_mul_t(const fr& a, const fr& b) {
   uint32_t m[8];
   uint64_t acc = 0;
   uint32_t ext = 0;
@@ -216,16 +190,16 @@ __device__ __forceinline__ fr fr_mul(const fr& a, const fr& b) {
       PL_MAC(mad_co_s(m[i], r_limb(k - i), acc, c));
 #pragma unroll
     for (int i = 0; i <= k; i++)           // a_i * b_{k-i}
-      PL_MAC(mad_co(a.l[i], b.l[k - i], acc, c));
+      PL_MAC(UB ? mad_co_s(a.l[i], b.l[k - i], acc, c) : mad_co(a.l[i], b.l[k - i], acc, c));
     // Montgomery digit: m_k = -lo (mod 2^32) since -r^-1 = -1 (mod 2^32).
     // lo + m_k*r0 = lo + m_k = 0 (mod 2^32) with carry (lo != 0).
     uint32_t lo = (uint32_t)acc;
     uint32_t hi = (uint32_t)(acc >> 32);
     m[k] = 0u - lo;
-    lanemask cc;
-    (void)add_co(lo, m[k], cc);
-    uint32_t nhi = addc_co(hi, 0u, cc, cc);
-    uint32_t next = ext_zero ? bit_c(cc) : inc_c(ext, cc);
+    uint32_t c1, c2;
+    (void)__builtin_addc(lo, m[k], 0u, &c1);
+    uint32_t nhi = __builtin_addc(hi, 0u, c1, &c2);
+    uint32_t next = ext_zero ? c2 : ext + c2;
     acc = ((uint64_t)next << 32) | nhi;
     ext_zero = true;
   }
@@ -237,7 +211,7 @@ __device__ __forceinline__ fr fr_mul(const fr& a, const fr& b) {
       PL_MAC(mad_co_s(m[i], r_limb(k - i), acc, c));
 #pragma unroll
     for (int i = k - 7; i < 8; i++)
-      PL_MAC(mad_co(a.l[i], b.l[k - i], acc, c));
+      PL_MAC(UB ? mad_co_s(a.l[i], b.l[k - i], acc, c) : mad_co(a.l[i], b.l[k - i], acc, c));
     t.l[k - 8] = (uint32_t)acc;
     acc = ((uint64_t)ext << 32) | (uint32_t)(acc >> 32);
     ext_zero = true;
@@ -248,6 +222,8 @@ __device__ __forceinline__ fr fr_mul(const fr& a, const fr& b) {
   return fr_reduce_once(t);
 }
 
+__device__ __forceinline__ fr fr_mul(const fr& a, const fr& b) { return fr_mul_t<false>(a, b); }
+__device__ __forceinline__ fr fr_mul_u(const fr& a, const fr& b) { return fr_mul_t<true>(a, b); }
 __device__ __forceinline__ fr fr_sqr(const fr& a) { return fr_mul(a, a); }
 
 // a^e, e a 64-bit exponent (square and multiply, MSB first)

@@ -1,0 +1,203 @@
+// NTT host side: plans, domain tables, pass sequencing; small-size kernel
+// and table builders.  The pass kernel itself is in ntt_pass.hpp.
+#include "ntt_pass.hpp"
+
+namespace pl {
+
+// Sizes 2^1 .. 2^3: one thread per output, direct sum.  Blocks hold whole
+// polynomials (blockDim is a multiple of 8), all reads precede the barrier,
+// so in-place calls are safe.
+__global__ void __launch_bounds__(256) ntt_tiny_kernel(PassArgs a, uint32_t npoly) {
+  const uint64_t tid = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint32_t lg = a.lgN;
+  const uint32_t n = 1u << lg;
+  const bool valid = tid < ((uint64_t)npoly << lg);
+  const uint64_t poly = tid >> lg;
+  const uint32_t k = (uint32_t)(tid & (n - 1));
+  fr acc = fr_zero();
+  if (valid) {
+    const fr* src = a.in + poly * a.in_pstride;
+    const fr* pm = a.premul ? a.premul + ((uint64_t)(poly & a.pm_mask) << lg) : nullptr;
+    for (uint32_t j = 0; j < n; j++) {
+      fr v = fr_load(src + j);
+      if (pm) v = fr_mul(v, fr_load(pm + j));
+      const fr w = fr_load(a.inner + (((j * k) & (n - 1)) << (8 - lg)));
+      acc = fr_add(acc, fr_mul(v, w));
+    }
+    if (a.do_scale) acc = fr_mul_u(acc, a.scale);
+  }
+  __syncthreads();
+  if (valid) {
+    const uint32_t lgI = a.lgI;
+    fr* dst = a.out + ((poly >> lgI) << (lg + lgI)) + (poly & ((1ull << lgI) - 1));
+    fr_store(dst + ((uint64_t)k << lgI), acc);
+  }
+}
+
+// ---- table construction (domain creation; not on the hot path) ----------
+
+// out[i] = base^(i << sh) * scale, i < n  (square and multiply)
+__global__ void pow_direct_kernel(fr* out, uint64_t n, fr base, uint32_t sh, fr scale) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  fr_store(out + i, fr_mul(fr_pow_u64(base, i << sh), scale));
+}
+
+// out[i] = w^(e(i)) * scale with w^e = W0[e & 16383] * W1[e >> 14];
+// e(i) = (i & (S-1)) * (i >> lgS) when lgS > 0 (pass twiddles), else i.
+__global__ void pow_window_kernel(fr* out, uint64_t n, const fr* w0, const fr* w1, uint32_t lgS,
+                                  fr scale, uint32_t do_scale) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  uint64_t e = lgS ? (i & ((1ull << lgS) - 1)) * (i >> lgS) : i;
+  fr v = fr_mul(fr_load(w0 + (e & 16383)), fr_load(w1 + (e >> 14)));
+  if (do_scale) v = fr_mul(v, scale);
+  fr_store(out + i, v);
+}
+
+static inline fr to_dev(const host::fr64& h) {
+  fr f;
+  memcpy(f.l, h.l, 32);
+  return f;
+}
+
+static inline unsigned grid_for(uint64_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }
+
+NttPlan make_plan(int lg) {
+  NttPlan p;
+  p.lg = lg;
+  if (lg < 4) {
+    p.P = 0;
+    return p;
+  }
+  int P = (lg + 7) / 8;
+  int base = lg / P, rem = lg % P;
+  int lgM = lg;
+  for (int i = 0; i < P; i++) {
+    p.m[i] = base + (i < rem ? 1 : 0);
+    p.lgM[i] = lgM;
+    p.lgS[i] = lgM - p.m[i];
+    lgM -= p.m[i];
+  }
+  p.P = P;
+  return p;
+}
+
+// out[i] = base^e(i) * scale for the n entries of a table (see pow_window_kernel)
+static hipError_t fill_pow(fr* out, uint64_t n, uint32_t lgS, const host::fr64& base, const host::fr64* scale,
+                           fr* win, hipStream_t s) {
+  fr one = to_dev(host::one());
+  fr b = to_dev(base);
+  hipLaunchKernelGGL(pow_direct_kernel, dim3(grid_for(16384, 256)), dim3(256), 0, s, win, (uint64_t)16384, b, 0u, one);
+  hipLaunchKernelGGL(pow_direct_kernel, dim3(grid_for(16384, 256)), dim3(256), 0, s, win + 16384, (uint64_t)16384, b,
+                     14u, one);
+  fr sc = scale ? to_dev(*scale) : one;
+  hipLaunchKernelGGL(pow_window_kernel, dim3(grid_for(n, 256)), dim3(256), 0, s, out, n, (const fr*)win,
+                     (const fr*)(win + 16384), lgS, sc, scale ? 1u : 0u);
+  return hipGetLastError();
+}
+
+hipError_t fill_pow_table(fr* out, uint64_t n, const host::fr64& base, const host::fr64& scale, hipStream_t s) {
+  fr* win = nullptr;
+  hipError_t e = hipMalloc((void**)&win, 2 * 16384 * sizeof(fr));
+  if (e != hipSuccess) return e;
+  e = fill_pow(out, n, 0, base, &scale, win, s);
+  hipError_t e2 = hipStreamSynchronize(s);
+  hipFree(win);
+  return e != hipSuccess ? e : e2;
+}
+
+hipError_t domain_build(Domain& d, int lg, hipStream_t s) {
+  d.lg = lg;
+  d.plan = make_plan(lg);
+  const NttPlan& P = d.plan;
+  size_t count = 2 * 256;  // inner tables
+  for (int p = 0; p + 1 < P.P; p++) count += 2 * (1ull << P.lgM[p]);
+  d.bytes = count * sizeof(fr);
+  hipError_t e = hipMalloc((void**)&d.block, d.bytes);
+  if (e != hipSuccess) return e;
+  fr* cur = d.block;
+  d.inner_fwd = cur; cur += 256;
+  d.inner_inv = cur; cur += 256;
+  for (int p = 0; p + 1 < P.P; p++) {
+    d.tw_fwd[p] = cur; cur += 1ull << P.lgM[p];
+    d.tw_inv[p] = cur; cur += 1ull << P.lgM[p];
+  }
+  fr* win = nullptr;
+  e = hipMalloc((void**)&win, 2 * 16384 * sizeof(fr));
+  if (e != hipSuccess) return e;
+  host::fr64 w256 = host::root_of_unity(8);
+  host::fr64 w256i = host::inv(w256);
+  d.n_inv = host::domain_size_inverse(lg);
+  if ((e = fill_pow(d.inner_fwd, 256, 0, w256, nullptr, win, s)) != hipSuccess) goto done;
+  if ((e = fill_pow(d.inner_inv, 256, 0, w256i, nullptr, win, s)) != hipSuccess) goto done;
+  for (int p = 0; p + 1 < P.P; p++) {
+    host::fr64 w = host::root_of_unity(P.lgM[p]);
+    host::fr64 wi = host::inv(w);
+    uint64_t n = 1ull << P.lgM[p];
+    if ((e = fill_pow(d.tw_fwd[p], n, P.lgS[p], w, nullptr, win, s)) != hipSuccess) goto done;
+    // the inverse transform's 2^-lg rides on the first pass's twiddles
+    if ((e = fill_pow(d.tw_inv[p], n, P.lgS[p], wi, p == 0 ? &d.n_inv : nullptr, win, s)) != hipSuccess) goto done;
+  }
+  e = hipStreamSynchronize(s);
+done:
+  hipFree(win);
+  return e;
+}
+
+void domain_free(Domain& d) {
+  if (d.block) hipFree(d.block);
+  d.block = nullptr;
+}
+
+static void launch_pass_la(int la, bool inv, bool last, const PassArgs& a, hipStream_t s) {
+  switch (la) {
+    case 0: launch_pass_la0(inv, last, a, s); break;
+    case 1: launch_pass_la1(inv, last, a, s); break;
+    case 2: launch_pass_la2(inv, last, a, s); break;
+    case 3: launch_pass_la3(inv, last, a, s); break;
+    default: launch_pass_la4(inv, last, a, s); break;
+  }
+}
+
+hipError_t ntt_launch(const Domain& d, const NttCall& c, hipStream_t s) {
+  const NttPlan& P = d.plan;
+  const int lg = d.lg;
+  PassArgs a;
+  memset(&a, 0, sizeof(a));
+  a.inner = c.inverse ? d.inner_inv : d.inner_fwd;
+  a.lgN = lg;
+  a.lgI = c.lgI;
+  a.pm_mask = c.pm_mask;
+  a.scale = to_dev(d.n_inv);
+  if (P.P == 0) {
+    a.in = c.in;
+    a.out = c.out;
+    a.in_pstride = c.in_pstride;
+    a.premul = c.premul;
+    a.do_scale = c.inverse ? 1 : 0;
+    uint64_t total = (uint64_t)c.npoly << lg;
+    hipLaunchKernelGGL(ntt_tiny_kernel, dim3(grid_for(total, 256)), dim3(256), 0, s, a, c.npoly);
+    return hipGetLastError();
+  }
+  for (int p = 0; p < P.P; p++) {
+    const bool last = p == P.P - 1;
+    a.in = p == 0 ? c.in : c.tmp;
+    a.in_pstride = p == 0 ? c.in_pstride : (1ull << lg);
+    a.premul = p == 0 ? c.premul : nullptr;
+    a.out = last ? c.out : c.tmp;
+    a.tw = last ? nullptr : (c.inverse ? d.tw_inv[p] : d.tw_fwd[p]);
+    a.lgMp = P.lgM[p];
+    a.lgS = P.lgS[p];
+    a.nlines = (uint64_t)c.npoly << (lg - P.m[p]);
+    a.do_scale = (c.inverse && P.P == 1) ? 1 : 0;
+    if (last) {
+      a.ndig = p;
+      for (int q = 0; q < p; q++) a.lgrad[q] = P.m[q];
+    }
+    launch_pass_la(P.m[p] - 4, c.inverse, last, a, s);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace pl

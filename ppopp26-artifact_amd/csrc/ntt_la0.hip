@@ -1,0 +1,6 @@
+// Pass kernels for columns of 2^4 points (forward/inverse, intermediate/last).
+#include "ntt_pass.hpp"
+
+namespace pl {
+PL_DEFINE_PASS_LAUNCHER(0)
+}  // namespace pl

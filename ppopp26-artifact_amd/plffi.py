@@ -1,0 +1,139 @@
+"""ctypes binding of libpl's C-ABI (include/pl.h) for tests and bench.py.
+
+Device memory and streams come from PyTorch (plumbing only): a field vector
+is a torch int64 tensor of shape (n, 4) on the GPU, i.e. n canonical
+Montgomery elements of 32 bytes.  All compute goes through lib/libpl.so; if
+that library is missing the import fails loudly (there is no fallback).
+"""
+import ctypes
+import os
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+LIB_PATH = os.path.join(HERE, "lib", "libpl.so")
+
+PL_OK = 0
+STATUS = {0: "ok", -1: "invalid argument", -2: "HIP runtime error", -3: "unsupported size",
+          -4: "device out of memory", -5: "no HIP device"}
+
+_lib = None
+_vp = ctypes.c_void_p
+_u64 = ctypes.c_uint64
+_u32 = ctypes.c_uint32
+_i = ctypes.c_int
+
+
+class PlError(RuntimeError):
+    def __init__(self, fn, status):
+        detail = ""
+        if status in (-2, -4) and _lib is not None:
+            detail = ": " + _lib.pl_last_hip_error().decode()
+        super().__init__("%s failed: %s (%d)%s" % (fn, STATUS.get(status, "?"), status, detail))
+        self.status = status
+
+
+# name -> (restype, argtypes); the symbol list a loader test checks
+SIGNATURES = {
+    "pl_status_string": (ctypes.c_char_p, [_i]),
+    "pl_version": (_i, []),
+    "pl_last_hip_error": (ctypes.c_char_p, []),
+    "pl_domain_create": (_i, [_i, ctypes.POINTER(_vp)]),
+    "pl_domain_destroy": (_i, [_vp]),
+    "pl_domain_lg": (_i, [_vp]),
+    "pl_domain_bytes": (ctypes.c_size_t, [_vp]),
+    "pl_ntt": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp]),
+    "pl_ntt_batch": (_i, [_vp, _vp, _i, _u32, _i, _vp, _vp, _vp]),
+    "pl_ntt_work_bytes": (ctypes.c_size_t, [_i, _u32]),
+}
+
+
+def lib():
+    global _lib
+    if _lib is None:
+        if not os.path.exists(LIB_PATH):
+            raise ImportError("libpl.so is not built (%s); run __graft_entry__.build()" % LIB_PATH)
+        L = ctypes.CDLL(LIB_PATH)
+        for name, (res, args) in SIGNATURES.items():
+            f = getattr(L, name)
+            f.restype = res
+            f.argtypes = args
+        _lib = L
+    return _lib
+
+
+def check(fn, status):
+    if status != PL_OK:
+        raise PlError(fn, status)
+
+
+def _ptr(t):
+    return None if t is None else ctypes.c_void_p(t.data_ptr())
+
+
+def _stream(stream):
+    import torch
+    s = stream if stream is not None else torch.cuda.current_stream()
+    return ctypes.c_void_p(s.cuda_stream)
+
+
+class Domain:
+    """Twiddle tables for NTT/iNTT of size 2^lg (the reference's Ntt/Intt Params)."""
+
+    def __init__(self, lg):
+        self.lg = lg
+        h = ctypes.c_void_p()
+        check("pl_domain_create", lib().pl_domain_create(lg, ctypes.byref(h)))
+        self.h = h
+
+    @property
+    def nbytes(self):
+        return lib().pl_domain_bytes(self.h)
+
+    def close(self):
+        if self.h:
+            lib().pl_domain_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def ntt(x, dom, inverse=False, out=None, work=None, stream=None):
+    """NTT (or iNTT) of x, shape (2^lg, 4) int64 on the GPU.  out may be x."""
+    import torch
+    n = 1 << dom.lg
+    assert x.shape == (n, 4) and x.dtype == torch.int64 and x.is_cuda
+    if out is None:
+        out = torch.empty_like(x)
+    check("pl_ntt", lib().pl_ntt(_ptr(x), _ptr(out), dom.lg, 1 if inverse else 0, dom.h, _ptr(work),
+                                 _stream(stream)))
+    return out
+
+
+def ntt_batch(x, dom, npoly, inverse=False, out=None, work=None, stream=None):
+    import torch
+    n = 1 << dom.lg
+    assert x.shape == (npoly * n, 4) and x.dtype == torch.int64 and x.is_cuda
+    if out is None:
+        out = torch.empty_like(x)
+    check("pl_ntt_batch", lib().pl_ntt_batch(_ptr(x), _ptr(out), dom.lg, npoly, 1 if inverse else 0, dom.h,
+                                             _ptr(work), _stream(stream)))
+    return out
+
+
+def ntt_work_bytes(lg, npoly=1):
+    return lib().pl_ntt_work_bytes(lg, npoly)
+
+
+def to_dev(a):
+    """numpy uint64 (n, 4) -> torch int64 (n, 4) on cuda:0"""
+    import numpy as np
+    import torch
+    return torch.from_numpy(np.ascontiguousarray(a).view(np.int64)).to("cuda")
+
+
+def to_host(t):
+    import numpy as np
+    return t.cpu().numpy().view(np.uint64)

@@ -1,0 +1,89 @@
+"""GPU parity of the NTT / iNTT against the oracle and the golden vectors,
+through the C-ABI.  Bit-exact (integer arithmetic)."""
+import numpy as np
+import pytest
+
+import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pl():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    import plffi
+    plffi.lib()
+    return plffi
+
+
+def test_ntt_golden_small(pl, golden):
+    k = golden("ntt_small.npz")
+    for lg in range(1, 9):
+        dom = pl.Domain(lg)
+        x = pl.to_dev(k["in_%d" % lg])
+        y = pl.to_host(pl.ntt(x, dom))
+        np.testing.assert_array_equal(y, k["ntt_%d" % lg], err_msg="ntt lg=%d" % lg)
+        z = pl.to_host(pl.ntt(x, dom, inverse=True))
+        np.testing.assert_array_equal(z, k["intt_%d" % lg], err_msg="intt lg=%d" % lg)
+
+
+@pytest.mark.parametrize("lg", [9, 10, 12, 13, 16, 17, 20])
+def test_ntt_vs_oracle(pl, lg):
+    dom = pl.Domain(lg)
+    x = po.uniform(1 << lg, po.seed(po.OP_NTT, lg))
+    print("ntt lg=%d" % lg, flush=True)
+    y = pl.to_host(pl.ntt(pl.to_dev(x), dom))
+    np.testing.assert_array_equal(y, po.ntt(x, lg))
+    z = pl.to_host(pl.ntt(pl.to_dev(x), dom, inverse=True))
+    np.testing.assert_array_equal(z, po.ntt(x, lg, inverse=True))
+
+
+@pytest.mark.parametrize("lg", [22, 24])
+def test_ntt_roundtrip_large(pl, lg):
+    # size-independent properties at the benchmark sizes:
+    # iNTT(NTT(x)) == x, and NTT at the first 64 outputs vs direct evaluation
+    import torch
+    dom = pl.Domain(lg)
+    n = 1 << lg
+    x = po.uniform(n, po.seed(po.OP_NTT, lg))
+    xd = pl.to_dev(x)
+    print("ntt roundtrip lg=%d" % lg, flush=True)
+    y = pl.ntt(xd, dom)
+    z = pl.ntt(y, dom, inverse=True)
+    torch.cuda.synchronize()
+    assert torch.equal(z, xd)
+    yh = pl.to_host(y)
+    # X[k] = x(w^k): check k = 0, 1, n-1 by Horner on the CPU
+    w = po.root_of_unity(lg)
+    for k in (0, 1, n - 1):
+        pt = po.pow_(w, k)
+        np.testing.assert_array_equal(yh[k:k + 1], po.eval_(x, pt))
+
+
+def test_ntt_in_place(pl):
+    lg = 14
+    dom = pl.Domain(lg)
+    x = po.uniform(1 << lg, 3)
+    xd = pl.to_dev(x)
+    pl.ntt(xd, dom, out=xd)
+    np.testing.assert_array_equal(pl.to_host(xd), po.ntt(x, lg))
+
+
+def test_ntt_batch(pl):
+    lg, npoly = 11, 5
+    dom = pl.Domain(lg)
+    x = po.uniform(npoly << lg, 4)
+    y = pl.to_host(pl.ntt_batch(pl.to_dev(x), dom, npoly))
+    for p in range(npoly):
+        np.testing.assert_array_equal(y[p << lg:(p + 1) << lg], po.ntt(x[p << lg:(p + 1) << lg], lg))
+
+
+def test_reference_input_prefix(pl, golden):
+    # the reference's own input data (first 4096 elements of input.bin)
+    import os
+    from conftest import GOLDEN
+    raw = np.fromfile(os.path.join(GOLDEN, "input_prefix.bin"), dtype=np.uint64).reshape(-1, 4)
+    dom = pl.Domain(12)
+    y = pl.to_host(pl.ntt(pl.to_dev(raw), dom))
+    np.testing.assert_array_equal(y, po.ntt(raw, 12))
