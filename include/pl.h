@@ -67,6 +67,16 @@ int pl_ntt_batch(const void* d_in, void* d_out, int lg, uint32_t npoly, int inve
                  const pl_domain* d, void* d_work, pl_stream_t stream);
 size_t pl_ntt_work_bytes(int lg, uint32_t npoly);
 
+/* Pass-level access to the same transform (counterpart of the reference's
+ * per-stage entry Ntt::stage_no_rotate, function.cu:312-314; the
+ * intermediate layout is this library's own).  Running passes
+ * 0 .. pl_ntt_num_passes(lg)-1 in order, each reading the previous pass's
+ * output, equals pl_ntt_batch.  Intermediate passes may run in place; the
+ * last pass writes natural order and must not (d_in != d_out). */
+int pl_ntt_num_passes(int lg);
+int pl_ntt_pass(const void* d_in, void* d_out, int lg, uint32_t npoly, int pass, int inverse,
+                const pl_domain* d, pl_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -101,6 +101,28 @@ int pl_ntt_batch(const void* d_in, void* d_out, int lg, uint32_t npoly, int inve
   return e == hipSuccess ? PL_OK : hip_fail(e);
 }
 
+int pl_ntt_num_passes(int lg) {
+  if (lg < 1 || lg > 28) return PL_ERR_UNSUPPORTED;
+  pl::NttPlan p = pl::make_plan(lg);
+  return p.P == 0 ? 1 : p.P;
+}
+
+int pl_ntt_pass(const void* d_in, void* d_out, int lg, uint32_t npoly, int pass, int inverse, const pl_domain* d,
+                pl_stream_t stream) {
+  if (!d || !d_in || !d_out || npoly == 0 || d->d.lg != lg) return PL_ERR_ARG;
+  if (!aligned16(d_in) || !aligned16(d_out)) return PL_ERR_ARG;
+  const int np = pl_ntt_num_passes(lg);
+  if (pass < 0 || pass >= np) return PL_ERR_ARG;
+  // the last pass scatters across the whole polynomial: not in place
+  if (pass == np - 1 && np > 1 && d_in == d_out) return PL_ERR_ARG;
+  pl::NttCall c;
+  c.npoly = npoly;
+  c.inverse = inverse != 0;
+  hipError_t e = pl::ntt_launch_one(d->d, c, pass, (const pl::fr*)d_in, 1ull << lg, nullptr, (pl::fr*)d_out,
+                                    (hipStream_t)stream);
+  return e == hipSuccess ? PL_OK : hip_fail(e);
+}
+
 int pl_ntt(const void* d_in, void* d_out, int lg, int inverse, const pl_domain* d, void* d_work,
            pl_stream_t stream) {
   return pl_ntt_batch(d_in, d_out, lg, 1, inverse, d, d_work, stream);

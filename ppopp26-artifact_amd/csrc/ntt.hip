@@ -34,6 +34,17 @@ __global__ void __launch_bounds__(256) ntt_tiny_kernel(PassArgs a, uint32_t npol
   }
 }
 
+// Single-pass sizes with a per-element factor table (small coset NTTs):
+// out[poly][i] = in[poly * in_pstride + i] * premul[(poly & pm_mask) << lg | i]
+__global__ void premul_kernel(const fr* in, fr* out, uint64_t in_pstride, const fr* premul, uint32_t pm_mask,
+                              uint32_t lg, uint64_t total) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= total) return;
+  uint64_t poly = i >> lg, pos = i & ((1ull << lg) - 1);
+  fr v = fr_mul(fr_load(in + poly * in_pstride + pos), fr_load(premul + ((uint64_t)(poly & pm_mask) << lg) + pos));
+  fr_store(out + i, v);
+}
+
 // ---- table construction (domain creation; not on the hot path) ----------
 
 // out[i] = base^(i << sh) * scale, i < n  (square and multiply)
@@ -160,7 +171,11 @@ static void launch_pass_la(int la, bool inv, bool last, const PassArgs& a, hipSt
   }
 }
 
-hipError_t ntt_launch(const Domain& d, const NttCall& c, hipStream_t s) {
+// Pass p of the plan, reading `in` (poly stride in_pstride, optional premul
+// table) and writing `out`.  Intermediate passes keep the in-place layout, so
+// in may equal out for them; the last pass writes natural order.
+hipError_t ntt_launch_one(const Domain& d, const NttCall& c, int p, const fr* in, uint64_t in_pstride,
+                          const fr* premul, fr* out, hipStream_t s) {
   const NttPlan& P = d.plan;
   const int lg = d.lg;
   PassArgs a;
@@ -170,34 +185,51 @@ hipError_t ntt_launch(const Domain& d, const NttCall& c, hipStream_t s) {
   a.lgI = c.lgI;
   a.pm_mask = c.pm_mask;
   a.scale = to_dev(d.n_inv);
+  a.in = in;
+  a.out = out;
+  a.in_pstride = in_pstride;
+  a.premul = premul;
   if (P.P == 0) {
-    a.in = c.in;
-    a.out = c.out;
-    a.in_pstride = c.in_pstride;
-    a.premul = c.premul;
     a.do_scale = c.inverse ? 1 : 0;
     uint64_t total = (uint64_t)c.npoly << lg;
     hipLaunchKernelGGL(ntt_tiny_kernel, dim3(grid_for(total, 256)), dim3(256), 0, s, a, c.npoly);
     return hipGetLastError();
   }
-  for (int p = 0; p < P.P; p++) {
-    const bool last = p == P.P - 1;
-    a.in = p == 0 ? c.in : c.tmp;
-    a.in_pstride = p == 0 ? c.in_pstride : (1ull << lg);
-    a.premul = p == 0 ? c.premul : nullptr;
-    a.out = last ? c.out : c.tmp;
-    a.tw = last ? nullptr : (c.inverse ? d.tw_inv[p] : d.tw_fwd[p]);
-    a.lgMp = P.lgM[p];
-    a.lgS = P.lgS[p];
-    a.nlines = (uint64_t)c.npoly << (lg - P.m[p]);
-    a.do_scale = (c.inverse && P.P == 1) ? 1 : 0;
-    if (last) {
-      a.ndig = p;
-      for (int q = 0; q < p; q++) a.lgrad[q] = P.m[q];
-    }
-    launch_pass_la(P.m[p] - 4, c.inverse, last, a, s);
+  const bool last = p == P.P - 1;
+  if (last && premul) {
+    // the last-pass kernel carries no factor table (register budget): scale
+    // the input into the scratch buffer first
+    if (!c.tmp) return hipErrorInvalidValue;
+    uint64_t total = (uint64_t)c.npoly << lg;
+    hipLaunchKernelGGL(premul_kernel, dim3(grid_for(total, 256)), dim3(256), 0, s, in, c.tmp, in_pstride, premul,
+                       c.pm_mask, (uint32_t)lg, total);
+    a.in = c.tmp;
+    a.in_pstride = 1ull << lg;
+    a.premul = nullptr;
   }
+  a.tw = last ? nullptr : (c.inverse ? d.tw_inv[p] : d.tw_fwd[p]);
+  a.lgMp = P.lgM[p];
+  a.lgS = P.lgS[p];
+  a.nlines = (uint64_t)c.npoly << (lg - P.m[p]);
+  a.do_scale = (c.inverse && P.P == 1) ? 1 : 0;
+  if (last) {
+    a.ndig = p;
+    for (int q = 0; q < p; q++) a.lgrad[q] = P.m[q];
+  }
+  launch_pass_la(P.m[p] - 4, c.inverse, last, a, s);
   return hipGetLastError();
+}
+
+hipError_t ntt_launch(const Domain& d, const NttCall& c, hipStream_t s) {
+  const int np = d.plan.P == 0 ? 1 : d.plan.P;
+  const uint64_t N = 1ull << d.lg;
+  for (int p = 0; p < np; p++) {
+    const bool last = p == np - 1;
+    hipError_t e = ntt_launch_one(d, c, p, p == 0 ? c.in : c.tmp, p == 0 ? c.in_pstride : N,
+                                  p == 0 ? c.premul : nullptr, last ? c.out : c.tmp, s);
+    if (e != hipSuccess) return e;
+  }
+  return hipSuccess;
 }
 
 }  // namespace pl

@@ -88,7 +88,7 @@ __device__ __forceinline__ void dft_dif(fr (&x)[NX]) {
 struct PassArgs {
   const fr* in;
   fr* out;
-  const fr* premul;     // optional factor per input element [(poly & pm_mask) << lgN | pos]
+  const fr* premul;     // !LAST: optional factor per input element [(poly & pm_mask) << lgN | pos]
   const fr* tw;         // !LAST: pass twiddles, index j + S*k within the sub-problem
   const fr* inner;      // w_256^e (or w_256^-e), e < 256
   uint64_t in_pstride;  // element stride between input polys (0: shared input)
@@ -109,13 +109,13 @@ constexpr int kTile = 2048;
 
 // One pass over columns of M' = 16*A points, A = 2^LA.
 template <int LA, bool INV, bool LAST>
-__global__ void __launch_bounds__(kThreads) ntt_pass_kernel(PassArgs a) {
+__global__ void __launch_bounds__(kThreads, 2) ntt_pass_kernel(PassArgs a) {
   constexpr int A = 1 << LA;
   constexpr int m = LA + 4;
   constexpr int LGC = 11 - m;  // log2 lines per tile
   constexpr int C = 1 << LGC;
   constexpr int F2 = 16 / A;   // DFT_A groups per thread in sub-step 2
-  __shared__ uint4 lds[2][kTile];
+  __shared__ uint4 lds[kTile];  // one 16-byte half of every tile element
 
   const int t = threadIdx.x;
   const uint64_t line0 = (uint64_t)blockIdx.x << LGC;
@@ -169,7 +169,7 @@ __global__ void __launch_bounds__(kThreads) ntt_pass_kernel(PassArgs a) {
       constexpr int r2 = decltype(r2_)::value;
       x[r2] = fr_load(src + pos0 + (uint64_t)(rho1 + A * r2) * estride);
     });
-    if (a.premul) {
+    if (!LAST && a.premul) {
       const fr* pm = a.premul + ((uint64_t)(poly & a.pm_mask) << a.lgN);
       static_for<0, 16>([&](auto r2_) {
         constexpr int r2 = decltype(r2_)::value;
@@ -187,31 +187,44 @@ __global__ void __launch_bounds__(kThreads) ntt_pass_kernel(PassArgs a) {
       x[brev(k2, 4)] = fr_mul(x[brev(k2, 4)], fr_load(a.inner + (e << (8 - m))));
     });
   }
-  // LDS slot of W[line][rho1][k2]; consecutive lanes hit consecutive slots
-  static_for<0, 16>([&](auto k2_) {
-    constexpr int k2 = decltype(k2_)::value;
-    const int slot = ((k2 * A + rho1) << LGC) + lc;
-    const fr& v = x[brev(k2, 4)];
-    lds[0][slot] = make_uint4(v.l[0], v.l[1], v.l[2], v.l[3]);
-    lds[1][slot] = make_uint4(v.l[4], v.l[5], v.l[6], v.l[7]);
-  });
-  __syncthreads();
-
-  // ---- sub-step 2: DFT_A over rho1 for F2 values of k2 of one line ---------
+  // Exchange through LDS in two 16-byte halves (32 KiB per workgroup, so
+  // four workgroups = two waves per SIMD fit a CU).  LDS slot of
+  // W[line][rho1][k2]; consecutive lanes hit consecutive slots.
   const int lc2 = t & (C - 1);
   const int g = t >> LGC;
-  fr y[16];
-  static_for<0, F2>([&](auto f_) {
-    constexpr int f = decltype(f_)::value;
-    const int k2 = g * F2 + f;
-    static_for<0, A>([&](auto r1_) {
-      constexpr int r1 = decltype(r1_)::value;
-      const int slot = ((k2 * A + r1) << LGC) + lc2;
-      const uint4 lo = lds[0][slot], hi = lds[1][slot];
-      fr v;
-      v.l[0] = lo.x; v.l[1] = lo.y; v.l[2] = lo.z; v.l[3] = lo.w;
-      v.l[4] = hi.x; v.l[5] = hi.y; v.l[6] = hi.z; v.l[7] = hi.w;
-      y[f * A + r1] = v;
+  // Each half of W goes out of x and the gathered half of y comes back into
+  // the registers it vacated: y lives in x (x[f*A + r1] after the exchange).
+  fr (&y)[16] = x;
+  static_for<0, 2>([&](auto h_) {
+    constexpr int h = decltype(h_)::value;
+    uint32_t hv[16][4];
+    static_for<0, 16>([&](auto k2_) {
+      constexpr int k2 = decltype(k2_)::value;
+      hv[k2][0] = x[brev(k2, 4)].l[4 * h + 0];
+      hv[k2][1] = x[brev(k2, 4)].l[4 * h + 1];
+      hv[k2][2] = x[brev(k2, 4)].l[4 * h + 2];
+      hv[k2][3] = x[brev(k2, 4)].l[4 * h + 3];
+    });
+    if constexpr (h == 1) __syncthreads();  // phase-0 reads done before overwrite
+    static_for<0, 16>([&](auto k2_) {
+      constexpr int k2 = decltype(k2_)::value;
+      const int slot = ((k2 * A + rho1) << LGC) + lc;
+      lds[slot] = make_uint4(hv[k2][0], hv[k2][1], hv[k2][2], hv[k2][3]);
+    });
+    __syncthreads();
+    // sub-step 2 gathers: DFT_A over rho1 for F2 values of k2 of one line
+    static_for<0, F2>([&](auto f_) {
+      constexpr int f = decltype(f_)::value;
+      const int k2 = g * F2 + f;
+      static_for<0, A>([&](auto r1_) {
+        constexpr int r1 = decltype(r1_)::value;
+        const int slot = ((k2 * A + r1) << LGC) + lc2;
+        const uint4 q = lds[slot];
+        y[f * A + r1].l[4 * h + 0] = q.x;
+        y[f * A + r1].l[4 * h + 1] = q.y;
+        y[f * A + r1].l[4 * h + 2] = q.z;
+        y[f * A + r1].l[4 * h + 3] = q.w;
+      });
     });
   });
   if constexpr (LA > 0) {

@@ -55,6 +55,8 @@ struct NttCall {
 };
 
 hipError_t ntt_launch(const Domain& d, const NttCall& c, hipStream_t s);
+hipError_t ntt_launch_one(const Domain& d, const NttCall& c, int p, const fr* in, uint64_t in_pstride,
+                          const fr* premul, fr* out, hipStream_t s);
 hipError_t domain_build(Domain& d, int lg, hipStream_t s);
 void domain_free(Domain& d);
 

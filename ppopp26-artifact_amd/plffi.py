@@ -43,6 +43,8 @@ SIGNATURES = {
     "pl_ntt": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp]),
     "pl_ntt_batch": (_i, [_vp, _vp, _i, _u32, _i, _vp, _vp, _vp]),
     "pl_ntt_work_bytes": (ctypes.c_size_t, [_i, _u32]),
+    "pl_ntt_num_passes": (_i, [_i]),
+    "pl_ntt_pass": (_i, [_vp, _vp, _i, _u32, _i, _i, _vp, _vp]),
 }
 
 
@@ -121,6 +123,16 @@ def ntt_batch(x, dom, npoly, inverse=False, out=None, work=None, stream=None):
     check("pl_ntt_batch", lib().pl_ntt_batch(_ptr(x), _ptr(out), dom.lg, npoly, 1 if inverse else 0, dom.h,
                                              _ptr(work), _stream(stream)))
     return out
+
+
+def ntt_pass(x, out, dom, p, npoly=1, inverse=False, stream=None):
+    check("pl_ntt_pass", lib().pl_ntt_pass(_ptr(x), _ptr(out), dom.lg, npoly, p, 1 if inverse else 0, dom.h,
+                                           _stream(stream)))
+    return out
+
+
+def ntt_num_passes(lg):
+    return lib().pl_ntt_num_passes(lg)
 
 
 def ntt_work_bytes(lg, npoly=1):
