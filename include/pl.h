@@ -77,6 +77,47 @@ int pl_ntt_num_passes(int lg);
 int pl_ntt_pass(const void* d_in, void* d_out, int lg, uint32_t npoly, int pass, int inverse,
                 const pl_domain* d, pl_stream_t stream);
 
+/* ---- coset low-degree extension ---------------------------------------
+ * [our_lde, lde.cu:1267-1365 (lde_spread_powers_kernel lde.cu:623-648,
+ *  make_lde_plan lde.cu:288-302)]
+ * N = 2^lg_n coefficients c (natural order) -> M = N << lg_blowup values
+ *   out[k] = sum_{i<N} c_i (shift * w_M^k)^i,  k < M, natural order.
+ * h_shift: host pointer to one Montgomery element, or NULL for the
+ * reference's shift 7^(2^lg_blowup).  Computed as 2^lg_blowup coset NTTs of
+ * size N; the coset powers ride on the first pass, the interleave on the
+ * last.  1 <= lg_blowup <= 4, lg_n + lg_blowup <= 28.
+ * d_out may equal d_coeffs when that buffer holds M elements.
+ * d_work: NULL or pl_lde_work_bytes(lg_n, lg_blowup) bytes. */
+int pl_lde_domain_create(int lg_n, int lg_blowup, const void* h_shift, pl_lde_domain** out);
+int pl_lde_domain_destroy(pl_lde_domain* d);
+size_t pl_lde_domain_bytes(const pl_lde_domain* d);
+size_t pl_lde_work_bytes(int lg_n, int lg_blowup);
+int pl_lde(const void* d_coeffs, void* d_out, const pl_lde_domain* d, void* d_work, pl_stream_t stream);
+
+/* ---- scans -------------------------------------------------------------
+ * All three are work-efficient block scans (~3 Montgomery products per
+ * element).  d_work: NULL or pl_scan_work_bytes(n) bytes of device scratch.
+ *
+ * Grand product [accumulate_mul_poly / accumulate_mul_poly_,
+ * PLONK/utils/function.cu:256-262; mont_arithmetic.cu:181-201, 333-361, 544-556]:
+ * in place, d_inout[i] <- prod_{j<i} p[j] (exclusive; element 0 becomes
+ * Mont(1)).  d_incl_out: NULL, or receives the inclusive products
+ * prod_{j<=i} p[j] (what accumulate_mul_poly_ leaves in its output). */
+size_t pl_scan_work_bytes(uint64_t n);
+int pl_grand_product(void* d_inout, uint64_t n, void* d_incl_out, void* d_work, pl_stream_t stream);
+
+/* Evaluation [evaluate, function.cu:170-181; mont_arithmetic.cu:104-163]:
+ * *d_y = sum_{i<n} poly[i] * x^i, x a device element (the reference passes
+ * the point as a device buffer).  n == 0 gives 0. */
+int pl_eval(const void* d_poly, uint64_t n, const void* d_x, void* d_y, void* d_work, pl_stream_t stream);
+
+/* Division by (X + c) [poly_div_poly, function.cu:183-185;
+ * mont_arithmetic.cu:165-179, 203-212, 304-331, 531-542]:
+ * in place, d_inout[i] <- q[i] = sum_{j>i} a_j z^(j-i-1), z = -c, and
+ * q[n-1] = 0.  d_rem: NULL, or receives the remainder a(z) (the reference
+ * discards it).  c is a device element and is not modified. */
+int pl_div_linear(void* d_inout, uint64_t n, const void* d_c, void* d_rem, void* d_work, pl_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

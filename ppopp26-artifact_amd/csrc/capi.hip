@@ -9,6 +9,10 @@ struct pl_domain {
   pl::Domain d;
 };
 
+struct pl_lde_domain {
+  pl::LdeDomain d;
+};
+
 namespace {
 
 thread_local char g_last_error[256] = "";
@@ -126,6 +130,133 @@ int pl_ntt_pass(const void* d_in, void* d_out, int lg, uint32_t npoly, int pass,
 int pl_ntt(const void* d_in, void* d_out, int lg, int inverse, const pl_domain* d, void* d_work,
            pl_stream_t stream) {
   return pl_ntt_batch(d_in, d_out, lg, 1, inverse, d, d_work, stream);
+}
+
+int pl_lde_domain_create(int lg_n, int lg_blowup, const void* h_shift, pl_lde_domain** out) {
+  if (!out || lg_n < 1 || lg_blowup < 1) return PL_ERR_ARG;
+  if (lg_blowup > 4 || lg_n + lg_blowup > 28) return PL_ERR_UNSUPPORTED;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return PL_ERR_NO_DEVICE;
+  pl::host::fr64 shift;
+  if (h_shift) {
+    memcpy(shift.l, h_shift, 32);
+    if (pl::host::geq_p(shift.l)) return PL_ERR_ARG;
+  } else {
+    // the reference's coset factor 7^(2^lambda) (lde.cu:642-645, 659)
+    shift = pl::host::pow_u64(pl::host::from_u64(7), 1ull << lg_blowup);
+  }
+  pl_lde_domain* d = new (std::nothrow) pl_lde_domain();
+  if (!d) return PL_ERR_NOMEM;
+  hipError_t e = pl::lde_domain_build(d->d, lg_n, lg_blowup, shift, nullptr);
+  if (e != hipSuccess) {
+    pl::lde_domain_free(d->d);
+    delete d;
+    return hip_fail(e);
+  }
+  *out = d;
+  return PL_OK;
+}
+
+int pl_lde_domain_destroy(pl_lde_domain* d) {
+  if (!d) return PL_ERR_ARG;
+  pl::lde_domain_free(d->d);
+  delete d;
+  return PL_OK;
+}
+
+size_t pl_lde_domain_bytes(const pl_lde_domain* d) { return d ? d->d.bytes + d->d.ntt.bytes : 0; }
+
+size_t pl_lde_work_bytes(int lg_n, int lg_blowup) {
+  if (lg_n < 1 || lg_blowup < 1 || lg_n + lg_blowup > 28) return 0;
+  return (size_t)sizeof(pl::fr) << (lg_n + lg_blowup);
+}
+
+int pl_lde(const void* d_coeffs, void* d_out, const pl_lde_domain* d, void* d_work, pl_stream_t stream) {
+  if (!d || !d_coeffs || !d_out) return PL_ERR_ARG;
+  if (!aligned16(d_coeffs) || !aligned16(d_out) || !aligned16(d_work)) return PL_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  const int lg_n = d->d.lg_n, lb = d->d.lg_blowup;
+  size_t wb = pl_lde_work_bytes(lg_n, lb);
+  pl::fr* tmp = (pl::fr*)d_work;
+  bool own = false;
+  if (!tmp) {
+    hipError_t e = hipMallocAsync((void**)&tmp, wb, s);
+    if (e != hipSuccess) return hip_fail(e);
+    own = true;
+  }
+  pl::NttCall c;
+  c.in = (const pl::fr*)d_coeffs;
+  c.out = (pl::fr*)d_out;
+  c.tmp = tmp;
+  c.in_pstride = 0;  // every coset reads the same coefficients
+  c.npoly = 1u << lb;
+  c.premul = d->d.cp;
+  c.pm_mask = (1u << lb) - 1;
+  c.lgI = lb;
+  c.inverse = false;
+  hipError_t e = pl::ntt_launch(d->d.ntt, c, s);
+  if (own) (void)hipFreeAsync(tmp, s);
+  return e == hipSuccess ? PL_OK : hip_fail(e);
+}
+
+size_t pl_scan_work_bytes(uint64_t n) { return (size_t)pl::scan_scratch_elems(n) * sizeof(pl::fr); }
+
+namespace {
+// stream-ordered scratch when the caller passes none
+struct Scratch {
+  pl::fr* p = nullptr;
+  bool own = false;
+  hipStream_t s = nullptr;
+  hipError_t get(void* user, size_t bytes, hipStream_t st) {
+    s = st;
+    if (user) {
+      p = (pl::fr*)user;
+      return hipSuccess;
+    }
+    own = true;
+    return hipMallocAsync((void**)&p, bytes, st);
+  }
+  ~Scratch() {
+    if (own && p) (void)hipFreeAsync(p, s);
+  }
+};
+}  // namespace
+
+int pl_grand_product(void* d_inout, uint64_t n, void* d_incl_out, void* d_work, pl_stream_t stream) {
+  if (!d_inout || n == 0) return PL_ERR_ARG;
+  if (!aligned16(d_inout) || !aligned16(d_incl_out) || !aligned16(d_work)) return PL_ERR_ARG;
+  Scratch sc;
+  hipError_t e = sc.get(d_work, pl_scan_work_bytes(n), (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e);
+  e = pl::grand_product((pl::fr*)d_inout, (pl::fr*)d_incl_out, n, sc.p, (hipStream_t)stream);
+  return e == hipSuccess ? PL_OK : hip_fail(e);
+}
+
+int pl_eval(const void* d_poly, uint64_t n, const void* d_x, void* d_y, void* d_work, pl_stream_t stream) {
+  if (!d_x || !d_y || (n && !d_poly)) return PL_ERR_ARG;
+  if (!aligned16(d_poly) || !aligned16(d_x) || !aligned16(d_y) || !aligned16(d_work)) return PL_ERR_ARG;
+  hipStream_t s = (hipStream_t)stream;
+  if (n == 0) {
+    // evaluate() of an empty polynomial is zero (function.cu:171-176)
+    hipError_t e = pl::fill_zero((pl::fr*)d_y, 1, s);
+    return e == hipSuccess ? PL_OK : hip_fail(e);
+  }
+  Scratch sc;
+  hipError_t e = sc.get(d_work, pl_scan_work_bytes(n), s);
+  if (e != hipSuccess) return hip_fail(e);
+  e = pl::evaluate((const pl::fr*)d_poly, n, (const pl::fr*)d_x, (pl::fr*)d_y, sc.p, s);
+  return e == hipSuccess ? PL_OK : hip_fail(e);
+}
+
+int pl_div_linear(void* d_inout, uint64_t n, const void* d_c, void* d_rem, void* d_work, pl_stream_t stream) {
+  if (!d_inout || !d_c || n == 0) return PL_ERR_ARG;
+  if (!aligned16(d_inout) || !aligned16(d_c) || !aligned16(d_rem) || !aligned16(d_work)) return PL_ERR_ARG;
+  Scratch sc;
+  hipError_t e = sc.get(d_work, pl_scan_work_bytes(n), (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e);
+  e = pl::div_linear((pl::fr*)d_inout, n, (const pl::fr*)d_c, (pl::fr*)d_rem, sc.p, (hipStream_t)stream);
+  return e == hipSuccess ? PL_OK : hip_fail(e);
 }
 
 }  // extern "C"

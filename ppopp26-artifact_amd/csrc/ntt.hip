@@ -156,6 +156,35 @@ done:
   return e;
 }
 
+hipError_t lde_domain_build(LdeDomain& d, int lg_n, int lg_blowup, const host::fr64& shift, hipStream_t s) {
+  d.lg_n = lg_n;
+  d.lg_blowup = lg_blowup;
+  d.shift = shift;
+  hipError_t e = domain_build(d.ntt, lg_n, s);
+  if (e != hipSuccess) return e;
+  const uint64_t n = 1ull << lg_n;
+  const uint64_t B = 1ull << lg_blowup;
+  d.bytes = (n * B) * sizeof(fr);
+  if ((e = hipMalloc((void**)&d.cp, d.bytes)) != hipSuccess) return e;
+  fr* win = nullptr;
+  if ((e = hipMalloc((void**)&win, 2 * 16384 * sizeof(fr))) != hipSuccess) return e;
+  const host::fr64 wM = host::root_of_unity(lg_n + lg_blowup);
+  host::fr64 st = shift;
+  for (uint64_t t = 0; t < B; t++) {
+    if ((e = fill_pow(d.cp + t * n, n, 0, st, nullptr, win, s)) != hipSuccess) break;
+    st = host::mul(st, wM);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  hipFree(win);
+  return e;
+}
+
+void lde_domain_free(LdeDomain& d) {
+  domain_free(d.ntt);
+  if (d.cp) hipFree(d.cp);
+  d.cp = nullptr;
+}
+
 void domain_free(Domain& d) {
   if (d.block) hipFree(d.block);
   d.block = nullptr;
@@ -189,16 +218,10 @@ hipError_t ntt_launch_one(const Domain& d, const NttCall& c, int p, const fr* in
   a.out = out;
   a.in_pstride = in_pstride;
   a.premul = premul;
-  if (P.P == 0) {
-    a.do_scale = c.inverse ? 1 : 0;
-    uint64_t total = (uint64_t)c.npoly << lg;
-    hipLaunchKernelGGL(ntt_tiny_kernel, dim3(grid_for(total, 256)), dim3(256), 0, s, a, c.npoly);
-    return hipGetLastError();
-  }
-  const bool last = p == P.P - 1;
-  if (last && premul) {
-    // the last-pass kernel carries no factor table (register budget): scale
-    // the input into the scratch buffer first
+  if (premul && (P.P == 0 || p == P.P - 1)) {
+    // the last-pass and tiny kernels carry no factor table (register budget):
+    // scale the input into the scratch buffer first.  The scratch copy also
+    // makes a shared-input call (in_pstride 0) safe when out aliases in.
     if (!c.tmp) return hipErrorInvalidValue;
     uint64_t total = (uint64_t)c.npoly << lg;
     hipLaunchKernelGGL(premul_kernel, dim3(grid_for(total, 256)), dim3(256), 0, s, in, c.tmp, in_pstride, premul,
@@ -207,6 +230,13 @@ hipError_t ntt_launch_one(const Domain& d, const NttCall& c, int p, const fr* in
     a.in_pstride = 1ull << lg;
     a.premul = nullptr;
   }
+  if (P.P == 0) {
+    a.do_scale = c.inverse ? 1 : 0;
+    uint64_t total = (uint64_t)c.npoly << lg;
+    hipLaunchKernelGGL(ntt_tiny_kernel, dim3(grid_for(total, 256)), dim3(256), 0, s, a, c.npoly);
+    return hipGetLastError();
+  }
+  const bool last = p == P.P - 1;
   a.tw = last ? nullptr : (c.inverse ? d.tw_inv[p] : d.tw_fwd[p]);
   a.lgMp = P.lgM[p];
   a.lgS = P.lgS[p];

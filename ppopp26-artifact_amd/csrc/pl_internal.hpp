@@ -41,6 +41,18 @@ struct Domain {
   int device = 0;
 };
 
+// Coset tables for the LDE: cp[t << lg_n | i] = (shift * w_M^t)^i.
+struct LdeDomain {
+  int lg_n = 0, lg_blowup = 0;
+  Domain ntt;               // size-2^lg_n transform
+  fr* cp = nullptr;         // 2^(lg_n + lg_blowup) entries
+  size_t bytes = 0;
+  host::fr64 shift;
+};
+
+hipError_t lde_domain_build(LdeDomain& d, int lg_n, int lg_blowup, const host::fr64& shift, hipStream_t s);
+void lde_domain_free(LdeDomain& d);
+
 // Launch geometry for one call: a batch of `npoly` polynomials of size 2^lg.
 struct NttCall {
   const fr* in = nullptr;      // first pass input
@@ -60,7 +72,15 @@ hipError_t ntt_launch_one(const Domain& d, const NttCall& c, int p, const fr* in
 hipError_t domain_build(Domain& d, int lg, hipStream_t s);
 void domain_free(Domain& d);
 
-// element-wise helper kernels (ops.hip)
+// scans (scan.hip)
+uint64_t scan_scratch_elems(uint64_t n);
+// zero-fill by kernel (no memset nodes: callers may capture in hipGraphs)
+hipError_t fill_zero(fr* p, uint64_t n, hipStream_t s);
+hipError_t grand_product(fr* inout, fr* incl_out, uint64_t n, fr* scratch, hipStream_t s);
+hipError_t div_linear(fr* inout, uint64_t n, const fr* c, fr* rem, fr* scratch, hipStream_t s);
+hipError_t evaluate(const fr* coeffs, uint64_t n, const fr* x, fr* y, fr* scratch, hipStream_t s);
+
+// element-wise helper kernels
 hipError_t fill_pow_table(fr* out, uint64_t n, const host::fr64& base, const host::fr64& scale, hipStream_t s);
 
 }  // namespace pl

@@ -1,0 +1,320 @@
+// Scans over Fr: grand product, division by (X + c), evaluation.
+//
+//  grand product  z[0] = 1, z[i+1] = z[i] * p[i]     (exclusive prefix product)
+//                 [accumulate_mul_poly, mont_arithmetic.cu:181-201, 333-361]
+//  division       s_i = a_i + z * s_{i+1}, s_n = 0, z = -c; q[i] = s_{i+1},
+//                 q[n-1] = 0, remainder a(z) = s_0     (affine suffix scan)
+//                 [poly_div_poly, mont_arithmetic.cu:165-179, 203-212, 304-331]
+//  evaluation     y = sum c_i x^i = s_0 of the same scan with z = x
+//                 [evaluate, mont_arithmetic.cu:104-163, 283-302]
+//
+// The reference runs log2(n) Hillis-Steele passes over the whole array (one
+// mulmod per element per pass).  Here every scan is work-efficient: a block
+// owns 4096 consecutive elements (256 threads x 16), a reduce kernel folds
+// each block to one value, the block values are scanned by the same code one
+// level up (recursively), and a downsweep kernel rescans each block from its
+// carry.  About 3 mulmods per element in total, three memory sweeps.
+#include "pl_internal.hpp"
+
+namespace pl {
+
+constexpr int kScanThreads = 256;
+constexpr int kPer = 16;
+constexpr int kSeg = kScanThreads * kPer;  // 4096 elements per block
+constexpr int kLgSeg = 12;
+
+static inline unsigned sgrid(uint64_t n) { return (unsigned)((n + kSeg - 1) / kSeg); }
+
+__device__ __forceinline__ void lds_put(uint4 (*l)[kScanThreads], int t, const fr& v) {
+  l[0][t] = make_uint4(v.l[0], v.l[1], v.l[2], v.l[3]);
+  l[1][t] = make_uint4(v.l[4], v.l[5], v.l[6], v.l[7]);
+}
+__device__ __forceinline__ fr lds_get(uint4 (*l)[kScanThreads], int t) {
+  uint4 a = l[0][t], b = l[1][t];
+  fr v;
+  v.l[0] = a.x; v.l[1] = a.y; v.l[2] = a.z; v.l[3] = a.w;
+  v.l[4] = b.x; v.l[5] = b.y; v.l[6] = b.z; v.l[7] = b.w;
+  return v;
+}
+
+// ---------------------------------------------------------------------------
+// multiplicative prefix scan
+// ---------------------------------------------------------------------------
+
+// thread-level exclusive prefix across the block (Hillis-Steele over 256
+// thread totals, multiplicative); returns the product of threads < t
+__device__ fr block_excl_prod(fr v, uint4 (*l)[kScanThreads], fr* block_total) {
+  const int t = threadIdx.x;
+  fr incl = v;
+  lds_put(l, t, incl);
+  __syncthreads();
+  for (int s = 1; s < kScanThreads; s <<= 1) {
+    fr other = t >= s ? lds_get(l, t - s) : fr_one();
+    __syncthreads();
+    if (t >= s) incl = fr_mul(incl, other);
+    lds_put(l, t, incl);
+    __syncthreads();
+  }
+  fr excl = t > 0 ? lds_get(l, t - 1) : fr_one();
+  if (block_total) *block_total = lds_get(l, kScanThreads - 1);
+  __syncthreads();
+  return excl;
+}
+
+// partial[b] = product of block b's elements
+__global__ void __launch_bounds__(kScanThreads) pfx_reduce_kernel(const fr* in, uint64_t n, fr* partial) {
+  __shared__ uint4 l[2][kScanThreads];
+  const uint64_t base = ((uint64_t)blockIdx.x << kLgSeg) + (uint64_t)threadIdx.x * kPer;
+  fr acc = fr_one();
+  for (int k = 0; k < kPer; k++)
+    if (base + k < n) acc = fr_mul(acc, fr_load(in + base + k));
+  // tree product over the block
+  lds_put(l, threadIdx.x, acc);
+  __syncthreads();
+  for (int s = kScanThreads / 2; s > 0; s >>= 1) {
+    if ((int)threadIdx.x < s) {
+      acc = fr_mul(acc, lds_get(l, threadIdx.x + s));
+      lds_put(l, threadIdx.x, acc);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) fr_store(partial + blockIdx.x, acc);
+}
+
+// out_excl[i] = carry_b * prod_{block start <= j < i} in[j]; optional inclusive
+// output and block total (single-block use).  carry == nullptr means one.
+__global__ void __launch_bounds__(kScanThreads) pfx_down_kernel(const fr* in, fr* out_excl, fr* out_incl, uint64_t n,
+                                                                 const fr* carry, fr* total) {
+  __shared__ uint4 l[2][kScanThreads];
+  const uint64_t base = ((uint64_t)blockIdx.x << kLgSeg) + (uint64_t)threadIdx.x * kPer;
+  fr e[kPer];
+  static_for<0, kPer>([&](auto k_) {
+    constexpr int k = decltype(k_)::value;
+    e[k] = base + k < n ? fr_load(in + base + k) : fr_one();
+  });
+  // local inclusive products in place: e[k] = in[base] * ... * in[base+k]
+  static_for<1, kPer>([&](auto k_) {
+    constexpr int k = decltype(k_)::value;
+    e[k] = fr_mul(e[k - 1], e[k]);
+  });
+  fr btot;
+  fr pre = block_excl_prod(e[kPer - 1], l, &btot);
+  if (carry) {
+    fr c = fr_load(carry + blockIdx.x);
+    pre = fr_mul(c, pre);
+    btot = fr_mul(c, btot);
+  }
+  if (total && blockIdx.x == 0 && threadIdx.x == 0) fr_store(total, btot);
+  static_for<0, kPer>([&](auto k_) {
+    constexpr int k = decltype(k_)::value;
+    if (base + k < n) {
+      fr ex = k == 0 ? pre : fr_mul(pre, e[k - 1]);
+      if (out_incl) fr_store(out_incl + base + k, fr_mul(pre, e[k]));
+      fr_store(out_excl + base + k, ex);
+    }
+  });
+}
+
+// Exclusive prefix product of n elements; scratch needs pfx_scratch_elems(n).
+static uint64_t pfx_scratch_elems(uint64_t n) {
+  uint64_t tot = 0;
+  while (n > (uint64_t)kSeg) {
+    n = (n + kSeg - 1) / kSeg;
+    tot += 2 * n;
+  }
+  return tot + 1;
+}
+
+static hipError_t prefix_product(const fr* in, fr* out_excl, fr* out_incl, uint64_t n, fr* total, fr* scratch,
+                                 hipStream_t s) {
+  if (n <= (uint64_t)kSeg) {
+    hipLaunchKernelGGL(pfx_down_kernel, dim3(1), dim3(kScanThreads), 0, s, in, out_excl, out_incl, n,
+                       (const fr*)nullptr, total);
+    return hipGetLastError();
+  }
+  const uint64_t nb = (n + kSeg - 1) / kSeg;
+  fr* part = scratch;
+  fr* part_excl = scratch + nb;
+  hipLaunchKernelGGL(pfx_reduce_kernel, dim3(sgrid(n)), dim3(kScanThreads), 0, s, in, n, part);
+  hipError_t e = prefix_product(part, part_excl, nullptr, nb, total, scratch + 2 * nb, s);
+  if (e != hipSuccess) return e;
+  hipLaunchKernelGGL(pfx_down_kernel, dim3(sgrid(n)), dim3(kScanThreads), 0, s, in, out_excl, out_incl, n,
+                     (const fr*)part_excl, (fr*)nullptr);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// affine suffix scan  s_i = a_i + m * s_{i+1}
+// pw: device table of m^(2^j), j = 0..kLgSeg (m^4096 last)
+// ---------------------------------------------------------------------------
+
+// block suffix combine: V_t <- V_t + m^(16*2^s) V_{t+2^s}; returns the suffix
+// value starting at thread t (the block's carry folded in by the caller)
+__device__ fr block_suffix(fr v, uint4 (*l)[kScanThreads], const fr* pw) {
+  const int t = threadIdx.x;
+  lds_put(l, t, v);
+  __syncthreads();
+  for (int s = 0; (1 << s) < kScanThreads; s++) {
+    const int o = 1 << s;
+    fr other = t + o < kScanThreads ? lds_get(l, t + o) : fr_zero();
+    __syncthreads();
+    if (t + o < kScanThreads) v = fr_add(v, fr_mul(fr_load(pw + 4 + s), other));  // m^(16 * 2^s)
+    lds_put(l, t, v);
+    __syncthreads();
+  }
+  return v;
+}
+
+// partial[b] = sum over block b of a_k m^(k - start)
+__global__ void __launch_bounds__(kScanThreads) sfx_reduce_kernel(const fr* a, uint64_t n, const fr* pw, fr* partial) {
+  __shared__ uint4 l[2][kScanThreads];
+  const uint64_t base = ((uint64_t)blockIdx.x << kLgSeg) + (uint64_t)threadIdx.x * kPer;
+  const fr m = fr_load(pw);
+  fr h = fr_zero();
+  for (int k = kPer - 1; k >= 0; k--) {
+    h = fr_mul(h, m);
+    if (base + k < n) h = fr_add(h, fr_load(a + base + k));
+  }
+  // tree: V_t += m^(16*2^s) V_{t+2^s} for t = 0 mod 2^(s+1)
+  lds_put(l, threadIdx.x, h);
+  __syncthreads();
+  for (int s = 0; (1 << s) < kScanThreads; s++) {
+    const int o = 1 << s;
+    if ((threadIdx.x & (2 * o - 1)) == 0) {
+      h = fr_add(h, fr_mul(fr_load(pw + 4 + s), lds_get(l, threadIdx.x + o)));
+      lds_put(l, threadIdx.x, h);
+    }
+    __syncthreads();
+  }
+  if (threadIdx.x == 0) fr_store(partial + blockIdx.x, h);
+}
+
+// out[i] = s_{i+1} within each block given carry[b] = s at the block's end
+// (nullptr: zero); total (block 0) = s_0.  out may equal a.
+__global__ void __launch_bounds__(kScanThreads) sfx_down_kernel(const fr* a, fr* out, uint64_t n, const fr* pw,
+                                                                 const fr* carry, fr* total) {
+  __shared__ uint4 l[2][kScanThreads];
+  const int t = threadIdx.x;
+  const uint64_t base = ((uint64_t)blockIdx.x << kLgSeg) + (uint64_t)t * kPer;
+  const fr m = fr_load(pw);
+  fr e[kPer];
+  static_for<0, kPer>([&](auto k_) {
+    constexpr int k = decltype(k_)::value;
+    e[k] = base + k < n ? fr_load(a + base + k) : fr_zero();
+  });
+  // local Horner: L = sum_k e_k m^k
+  fr h = e[kPer - 1];
+  static_for<0, kPer - 1>([&](auto k_) {
+    constexpr int k = kPer - 2 - decltype(k_)::value;
+    h = fr_add(fr_mul(h, m), e[k]);
+  });
+  // fold the block carry into the last thread: s(start of thread 255) = L + m^16 K
+  fr K = carry ? fr_load(carry + blockIdx.x) : fr_zero();
+  if (t == kScanThreads - 1) h = fr_add(h, fr_mul(fr_load(pw + 4), K));
+  fr v = block_suffix(h, l, pw);  // s at the start of this thread's run
+  // carry-in at this thread's end = s at the start of thread t+1
+  fr cin = t + 1 < kScanThreads ? lds_get(l, t + 1) : K;
+  if (total && blockIdx.x == 0 && t == 0) fr_store(total, v);
+  // rescan: s_k = e_k + m s_{k+1}, out[base + k] = s_{k+1}
+  fr sn = cin;
+  static_for<0, kPer>([&](auto k_) {
+    constexpr int k = kPer - 1 - decltype(k_)::value;
+    if (base + k < n) fr_store(out + base + k, sn);
+    sn = fr_add(e[k], fr_mul(m, sn));
+  });
+}
+
+__global__ void zero_kernel(fr* p, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i < n) fr_store(p + i, fr_zero());
+}
+
+hipError_t fill_zero(fr* p, uint64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(zero_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, p, n);
+  return hipGetLastError();
+}
+
+// pw[j] = m^(2^j), j = 0..12; m = neg ? -(*src) : *src
+__global__ void sfx_powers_kernel(const fr* src, int neg, fr* pw) {
+  if (threadIdx.x != 0) return;
+  fr m = fr_load(src);
+  if (neg) m = fr_neg(m);
+  for (int j = 0; j <= kLgSeg; j++) {
+    fr_store(pw + j, m);
+    m = fr_sqr(m);
+  }
+}
+
+// pw_next[j] = pw[kLgSeg]^(2^j): powers of m^4096 for the next level
+__global__ void sfx_next_powers_kernel(const fr* pw, fr* pw_next) {
+  if (threadIdx.x != 0) return;
+  fr m = fr_load(pw + kLgSeg);
+  for (int j = 0; j <= kLgSeg; j++) {
+    fr_store(pw_next + j, m);
+    m = fr_sqr(m);
+  }
+}
+
+constexpr int kPw = kLgSeg + 1;
+
+static uint64_t sfx_scratch_elems(uint64_t n) {
+  uint64_t tot = kPw;
+  while (n > (uint64_t)kSeg) {
+    n = (n + kSeg - 1) / kSeg;
+    tot += 2 * n + kPw;
+  }
+  return tot;
+}
+
+// suffix scan with powers table pw (level's m): out[i] = s_{i+1} (if out),
+// *total = s_0 (if total).  out == nullptr: reduction only.
+static hipError_t suffix_scan(const fr* a, fr* out, uint64_t n, const fr* pw, fr* total, fr* scratch, hipStream_t s) {
+  if (n <= (uint64_t)kSeg) {
+    if (out) {
+      hipLaunchKernelGGL(sfx_down_kernel, dim3(1), dim3(kScanThreads), 0, s, a, out, n, pw, (const fr*)nullptr, total);
+    } else {
+      // one block: the reduce kernel's tree value is s_0
+      hipLaunchKernelGGL(sfx_reduce_kernel, dim3(1), dim3(kScanThreads), 0, s, a, n, pw, total);
+    }
+    return hipGetLastError();
+  }
+  const uint64_t nb = (n + kSeg - 1) / kSeg;
+  fr* part = scratch;
+  fr* carry = scratch + nb;
+  fr* pw_next = scratch + 2 * nb;
+  hipLaunchKernelGGL(sfx_reduce_kernel, dim3(sgrid(n)), dim3(kScanThreads), 0, s, a, n, pw, part);
+  hipLaunchKernelGGL(sfx_next_powers_kernel, dim3(1), dim3(64), 0, s, pw, pw_next);
+  // carry[b] = s at the end of block b = sum_{b' > b} part[b'] (m^4096)^(b'-b-1)
+  hipError_t e = suffix_scan(part, out ? carry : nullptr, nb, pw_next, total, scratch + 2 * nb + kPw, s);
+  if (e != hipSuccess || !out) return e;
+  hipLaunchKernelGGL(sfx_down_kernel, dim3(sgrid(n)), dim3(kScanThreads), 0, s, a, out, n, pw, (const fr*)carry,
+                     (fr*)nullptr);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
+// entry points used by the C-ABI
+// ---------------------------------------------------------------------------
+
+uint64_t scan_scratch_elems(uint64_t n) {
+  uint64_t a = pfx_scratch_elems(n), b = sfx_scratch_elems(n);
+  return a > b ? a : b;
+}
+
+hipError_t grand_product(fr* inout, fr* incl_out, uint64_t n, fr* scratch, hipStream_t s) {
+  return prefix_product(inout, inout, incl_out, n, nullptr, scratch, s);
+}
+
+hipError_t div_linear(fr* inout, uint64_t n, const fr* c, fr* rem, fr* scratch, hipStream_t s) {
+  fr* pw = scratch;
+  hipLaunchKernelGGL(sfx_powers_kernel, dim3(1), dim3(64), 0, s, c, 1, pw);
+  return suffix_scan(inout, inout, n, pw, rem, scratch + kPw, s);
+}
+
+hipError_t evaluate(const fr* coeffs, uint64_t n, const fr* x, fr* y, fr* scratch, hipStream_t s) {
+  fr* pw = scratch;
+  hipLaunchKernelGGL(sfx_powers_kernel, dim3(1), dim3(64), 0, s, x, 0, pw);
+  return suffix_scan(coeffs, nullptr, n, pw, y, scratch + kPw, s);
+}
+
+}  // namespace pl

@@ -45,6 +45,15 @@ SIGNATURES = {
     "pl_ntt_work_bytes": (ctypes.c_size_t, [_i, _u32]),
     "pl_ntt_num_passes": (_i, [_i]),
     "pl_ntt_pass": (_i, [_vp, _vp, _i, _u32, _i, _i, _vp, _vp]),
+    "pl_lde_domain_create": (_i, [_i, _i, _vp, ctypes.POINTER(_vp)]),
+    "pl_lde_domain_destroy": (_i, [_vp]),
+    "pl_lde_domain_bytes": (ctypes.c_size_t, [_vp]),
+    "pl_lde_work_bytes": (ctypes.c_size_t, [_i, _i]),
+    "pl_lde": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "pl_scan_work_bytes": (ctypes.c_size_t, [_u64]),
+    "pl_grand_product": (_i, [_vp, _u64, _vp, _vp, _vp]),
+    "pl_eval": (_i, [_vp, _u64, _vp, _vp, _vp, _vp]),
+    "pl_div_linear": (_i, [_vp, _u64, _vp, _vp, _vp, _vp]),
 }
 
 
@@ -123,6 +132,69 @@ def ntt_batch(x, dom, npoly, inverse=False, out=None, work=None, stream=None):
     check("pl_ntt_batch", lib().pl_ntt_batch(_ptr(x), _ptr(out), dom.lg, npoly, 1 if inverse else 0, dom.h,
                                              _ptr(work), _stream(stream)))
     return out
+
+
+class LdeDomain:
+    """Coset tables for our_lde: 2^lg_n coefficients -> 2^(lg_n+lg_blowup) values.
+    shift: None (the reference's 7^(2^lg_blowup)) or a numpy (1, 4) uint64 Montgomery element."""
+
+    def __init__(self, lg_n, lg_blowup, shift=None):
+        import numpy as np
+        self.lg_n, self.lg_blowup = lg_n, lg_blowup
+        h = ctypes.c_void_p()
+        sp = None
+        if shift is not None:
+            self._shift = np.ascontiguousarray(shift, dtype=np.uint64).reshape(4)
+            sp = ctypes.c_void_p(self._shift.ctypes.data)
+        check("pl_lde_domain_create", lib().pl_lde_domain_create(lg_n, lg_blowup, sp, ctypes.byref(h)))
+        self.h = h
+
+    @property
+    def nbytes(self):
+        return lib().pl_lde_domain_bytes(self.h)
+
+    def close(self):
+        if self.h:
+            lib().pl_lde_domain_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def lde(c, dom, out=None, work=None, stream=None):
+    import torch
+    n = 1 << dom.lg_n
+    assert c.shape == (n, 4) and c.dtype == torch.int64 and c.is_cuda
+    if out is None:
+        out = torch.empty((n << dom.lg_blowup, 4), dtype=torch.int64, device=c.device)
+    check("pl_lde", lib().pl_lde(_ptr(c), _ptr(out), dom.h, _ptr(work), _stream(stream)))
+    return out
+
+
+def grand_product(p, incl_out=None, work=None, stream=None):
+    """in place: p[i] <- prod_{j<i} p[j]"""
+    check("pl_grand_product", lib().pl_grand_product(_ptr(p), p.shape[0], _ptr(incl_out), _ptr(work),
+                                                     _stream(stream)))
+    return p
+
+
+def evaluate(poly, x, y=None, work=None, stream=None):
+    import torch
+    if y is None:
+        y = torch.empty((1, 4), dtype=torch.int64, device=x.device)
+    check("pl_eval", lib().pl_eval(_ptr(poly), poly.shape[0], _ptr(x), _ptr(y), _ptr(work), _stream(stream)))
+    return y
+
+
+def div_linear(a, c, rem=None, work=None, stream=None):
+    """in place: a <- quotient of a(X) by (X + c)"""
+    check("pl_div_linear", lib().pl_div_linear(_ptr(a), a.shape[0], _ptr(c), _ptr(rem), _ptr(work),
+                                               _stream(stream)))
+    return a
 
 
 def ntt_pass(x, out, dom, p, npoly=1, inverse=False, stream=None):

@@ -1,0 +1,83 @@
+"""CPU-side checks of the C-ABI library: it loads without a GPU, exports every
+symbol include/pl.h declares, and its pure host functions behave."""
+import ctypes
+import os
+import re
+import subprocess
+
+import pytest
+
+from conftest import ROOT
+
+HDR = os.path.join(ROOT, "include", "pl.h")
+
+
+def declared():
+    txt = open(HDR).read()
+    txt = re.sub(r"/\*.*?\*/", "", txt, flags=re.S)
+    return sorted(set(re.findall(r"^\s*(?:const\s+)?[a-z_0-9]+\**\s+\**(pl_[a-z_0-9]+)\s*\(", txt, re.M)))
+
+
+def test_header_declares_entry_points():
+    names = declared()
+    for n in ("pl_ntt", "pl_lde", "pl_grand_product", "pl_eval", "pl_div_linear", "pl_domain_create"):
+        assert n in names
+
+
+def test_library_exports_every_declared_symbol():
+    import plffi
+    if not os.path.exists(plffi.LIB_PATH):
+        pytest.skip("libpl.so not built")
+    out = subprocess.check_output(["nm", "-D", "--defined-only", plffi.LIB_PATH]).decode()
+    exported = set(l.split()[-1] for l in out.splitlines() if " T " in l)
+    missing = [n for n in declared() if n not in exported]
+    assert not missing, missing
+    # and the ctypes binding covers them all with signatures
+    assert set(declared()) <= set(plffi.SIGNATURES), set(declared()) - set(plffi.SIGNATURES)
+
+
+def test_library_loads_and_host_functions():
+    import plffi
+    if not os.path.exists(plffi.LIB_PATH):
+        pytest.skip("libpl.so not built")
+    L = plffi.lib()
+    assert L.pl_version() >= 100
+    assert L.pl_status_string(0) == b"ok"
+    assert L.pl_status_string(-3) == b"unsupported size"
+    # plan: ceil(lg/8) passes of 4..8 points per column; tiny sizes one pass
+    assert [L.pl_ntt_num_passes(lg) for lg in (1, 3, 4, 8, 9, 16, 17, 24, 25, 28)] == [1, 1, 1, 1, 2, 2, 3, 3, 4, 4]
+    assert L.pl_ntt_num_passes(29) == -3 and L.pl_ntt_num_passes(0) == -3
+    assert L.pl_ntt_work_bytes(8, 1) == 0          # single pass runs in place
+    assert L.pl_ntt_work_bytes(24, 1) == 32 << 24  # one intermediate buffer
+    assert L.pl_ntt_work_bytes(12, 3) == 3 * (32 << 12)
+    assert L.pl_lde_work_bytes(24, 2) == 32 << 26
+    assert L.pl_scan_work_bytes(1 << 20) >= 32 * 2 * 256
+
+
+def test_no_device_is_an_error_not_a_crash():
+    import plffi
+    if not os.path.exists(plffi.LIB_PATH):
+        pytest.skip("libpl.so not built")
+    try:
+        import torch
+        if torch.cuda.is_available():
+            pytest.skip("a device is present")
+    except ImportError:
+        pass
+    h = ctypes.c_void_p()
+    st = plffi.lib().pl_domain_create(12, ctypes.byref(h))
+    assert st == -5 and not h.value
+
+
+def test_argument_errors_without_device():
+    import plffi
+    if not os.path.exists(plffi.LIB_PATH):
+        pytest.skip("libpl.so not built")
+    L = plffi.lib()
+    h = ctypes.c_void_p()
+    assert L.pl_domain_create(0, ctypes.byref(h)) == -1
+    assert L.pl_domain_create(29, ctypes.byref(h)) == -3
+    assert L.pl_lde_domain_create(24, 5, None, ctypes.byref(h)) == -3
+    assert L.pl_ntt(None, None, 12, 0, None, None, None) == -1
+    assert L.pl_grand_product(None, 10, None, None, None) == -1
+    assert L.pl_div_linear(None, 0, None, None, None, None) == -1

@@ -1,0 +1,176 @@
+"""GPU parity of the LDE, grand product, evaluation and division against the
+oracle and the golden vectors, through the C-ABI.  Bit-exact."""
+import numpy as np
+import pytest
+
+import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pl():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    import plffi
+    plffi.lib()
+    return plffi
+
+
+def test_lde_golden_small(pl, golden):
+    k = golden("lde_small.npz")
+    for lg_n in range(1, 6):
+        c = pl.to_dev(k["in_%d" % lg_n])
+        for lb in range(1, 4):
+            print("lde golden", lg_n, lb, flush=True)
+            ref = pl.LdeDomain(lg_n, lb)
+            np.testing.assert_array_equal(pl.to_host(pl.lde(c, ref)), k["lde_ref_%d_%d" % (lg_n, lb)])
+            g = pl.LdeDomain(lg_n, lb, shift=po.from_u64(7))
+            np.testing.assert_array_equal(pl.to_host(pl.lde(c, g)), k["lde_g_%d_%d" % (lg_n, lb)])
+
+
+@pytest.mark.parametrize("lg_n,lb", [(8, 1), (9, 2), (12, 2), (12, 3), (16, 2), (17, 1), (20, 2)])
+def test_lde_vs_oracle(pl, lg_n, lb):
+    print("lde", lg_n, lb, flush=True)
+    c = po.uniform(1 << lg_n, po.seed(po.OP_LDE, lg_n, lb))
+    dom = pl.LdeDomain(lg_n, lb)
+    out = pl.to_host(pl.lde(pl.to_dev(c), dom))
+    np.testing.assert_array_equal(out, po.lde(c, lg_n, lb, po.ref_lde_shift(lb)))
+
+
+def test_lde_in_place(pl):
+    import torch
+    lg_n, lb = 10, 2
+    c = po.uniform(1 << lg_n, 99)
+    buf = torch.zeros(((1 << lg_n) << lb, 4), dtype=torch.int64, device="cuda")
+    buf[: 1 << lg_n] = pl.to_dev(c)
+    dom = pl.LdeDomain(lg_n, lb)
+    pl.lde(buf[: 1 << lg_n], dom, out=buf)
+    np.testing.assert_array_equal(pl.to_host(buf), po.lde(c, lg_n, lb, po.ref_lde_shift(lb)))
+
+
+@pytest.mark.parametrize("lg_n", [22, 24])
+def test_lde_large_properties(pl, lg_n):
+    # lambda=2 at the benchmark sizes: out[j*4 + t] is the size-N coset NTT
+    # with shift 7^4 * w_M^t; the inverse NTT of one coset column times
+    # shift_t^-i must give back c.  Check t = 0 and t = 3 fully.
+    import torch
+    lb = 2
+    n = 1 << lg_n
+    c = po.uniform(n, po.seed(po.OP_LDE, lg_n, lb))
+    cd = pl.to_dev(c)
+    print("lde large", lg_n, flush=True)
+    out = pl.lde(cd, pl.LdeDomain(lg_n, lb))
+    dom = pl.Domain(lg_n)
+    for t in (0, 3):
+        col = out[t::4].contiguous()
+        back = pl.to_host(pl.ntt(col, dom, inverse=True))
+        # back[i] = c_i * s_t^i ; compare at a few positions against the oracle
+        s = po.mul(po.ref_lde_shift(lb), po.pow_(po.root_of_unity(lg_n + lb), t))
+        for i in (0, 1, 2, n // 3, n - 1):
+            np.testing.assert_array_equal(back[i:i + 1], po.mul(c[i:i + 1], po.pow_(s, i)))
+    torch.cuda.synchronize()
+
+
+def test_grand_product_golden(pl, golden):
+    k = golden("scan_small.npz")
+    for n in (1, 2, 3, 64, 257):
+        p = pl.to_dev(k["gp_in_%d" % n])
+        pl.grand_product(p)
+        np.testing.assert_array_equal(pl.to_host(p), k["gp_out_%d" % n])
+
+
+@pytest.mark.parametrize("n", [4095, 4096, 4097, 100000, 1 << 20, (1 << 24) + 5])
+def test_grand_product_vs_oracle(pl, n):
+    import torch
+    print("gp", n, flush=True)
+    p = po.uniform(n, po.seed(po.OP_GP, 0, n & 0xff))
+    pd = pl.to_dev(p)
+    incl = torch.empty_like(pd)
+    pl.grand_product(pd, incl_out=incl)
+    exp = po.grand_product(p)
+    got = pl.to_host(pd)
+    np.testing.assert_array_equal(got, exp)
+    # inclusive = exclusive shifted by one, last = total product
+    gi = pl.to_host(incl)
+    np.testing.assert_array_equal(gi[:-1], exp[1:])
+    np.testing.assert_array_equal(gi[-1:], po.mul(exp[-1:], p[-1:]))
+
+
+def test_grand_product_zeros(pl):
+    # zeros in the input (a quarter of the reference's input.bin is zero)
+    n = 9000
+    p = po.uniform(n, 5)
+    p[100] = 0
+    p[5000] = 0
+    pd = pl.to_dev(p)
+    pl.grand_product(pd)
+    np.testing.assert_array_equal(pl.to_host(pd), po.grand_product(p))
+
+
+def test_eval_golden(pl, golden):
+    k = golden("scan_small.npz")
+    for n in (1, 2, 3, 64, 100):
+        c = pl.to_dev(k["eval_in_%d" % n])
+        for tag in ("r", "0", "1"):
+            y = pl.evaluate(c, pl.to_dev(k["eval_x_%s_%d" % (tag, n)]))
+            np.testing.assert_array_equal(pl.to_host(y), k["eval_y_%s_%d" % (tag, n)])
+
+
+@pytest.mark.parametrize("n", [4096, 4097, 123457, 1 << 20, 1 << 24])
+def test_eval_vs_oracle(pl, n):
+    print("eval", n, flush=True)
+    c = po.uniform(n, po.seed(po.OP_EVAL, 0, n & 0xff))
+    x = po.uniform(1, 7)
+    y = pl.evaluate(pl.to_dev(c), pl.to_dev(x))
+    np.testing.assert_array_equal(pl.to_host(y), po.eval_(c, x))
+
+
+def test_eval_empty(pl):
+    import torch
+    y = torch.full((1, 4), 7, dtype=torch.int64, device="cuda")
+    x = pl.to_dev(po.uniform(1, 3))
+    pl.lib().pl_eval(None, 0, pl._ptr(x), pl._ptr(y), None, pl._stream(None))
+    assert pl.to_host(y).sum() == 0
+
+
+def test_div_golden(pl, golden):
+    import torch
+    k = golden("scan_small.npz")
+    for n in (1, 2, 3, 64, 100):
+        a = pl.to_dev(k["div_in_%d" % n])
+        rem = torch.empty((1, 4), dtype=torch.int64, device="cuda")
+        pl.div_linear(a, pl.to_dev(k["div_c_%d" % n]), rem=rem)
+        np.testing.assert_array_equal(pl.to_host(a), k["div_q_%d" % n])
+        np.testing.assert_array_equal(pl.to_host(rem), k["div_rem_%d" % n])
+
+
+@pytest.mark.parametrize("n", [4096, 4097, 77777, 1 << 20, (1 << 24) + 3])
+def test_div_vs_oracle(pl, n):
+    import torch
+    print("div", n, flush=True)
+    a = po.uniform(n, po.seed(po.OP_DIV, 0, n & 0xff))
+    c = po.uniform(1, 11)
+    ad = pl.to_dev(a)
+    cd = pl.to_dev(c)
+    rem = torch.empty((1, 4), dtype=torch.int64, device="cuda")
+    pl.div_linear(ad, cd, rem=rem)
+    q, r = po.div(a, c)
+    np.testing.assert_array_equal(pl.to_host(ad), q)
+    np.testing.assert_array_equal(pl.to_host(rem), r)
+    np.testing.assert_array_equal(pl.to_host(cd), c)  # divisor untouched
+
+
+def test_miss_ops_literals(pl, golden):
+    import json, os
+    from conftest import GOLDEN
+    consts = json.load(open(os.path.join(GOLDEN, "ref_constants.json")))
+    k = golden("scan_small.npz")
+    pt = int(consts["miss_ops_point"], 16)
+    x = np.array([[(pt >> (64 * i)) & 0xFFFFFFFFFFFFFFFF for i in range(4)]], dtype=np.uint64)
+    poly = pl.to_dev(k["mo_poly"])
+    np.testing.assert_array_equal(pl.to_host(pl.evaluate(poly, pl.to_dev(x))), k["mo_eval"])
+    cv = int(consts["miss_ops_divisor_gpu"], 16)
+    c = np.array([[(cv >> (64 * i)) & 0xFFFFFFFFFFFFFFFF for i in range(4)]], dtype=np.uint64)
+    pl.div_linear(poly, pl.to_dev(c))
+    np.testing.assert_array_equal(pl.to_host(poly), k["mo_div_q"])
