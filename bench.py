@@ -37,6 +37,8 @@ def parse():
     ap.add_argument("--lg", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-lde", action="store_true")
+    ap.add_argument("--copy-probe", action="store_true",
+                    help="only run a plain device copy of 2^lg * 32 bytes (HBM counter calibration)")
     return ap.parse_args()
 
 
@@ -68,6 +70,14 @@ def main():
     stream = torch.cuda.current_stream()
     lg = args.lg
     n = 1 << lg
+    if args.copy_probe:
+        a = torch.ones((n, 4), dtype=torch.int64, device="cuda")
+        b = torch.empty_like(a)
+        for _ in range(3):
+            b.copy_(a)
+        torch.cuda.synchronize()
+        print(json.dumps({"copy_probe_bytes_read": n * 32, "copy_probe_bytes_written": n * 32, "launches": 3}))
+        return
     dom = plffi.Domain(lg)
     x = plffi.to_dev(po.uniform(n, po.seed(po.OP_NTT, lg)))
     y = torch.empty_like(x)

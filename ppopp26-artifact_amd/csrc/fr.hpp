@@ -222,8 +222,14 @@ __device__ __forceinline__ fr fr_mul_t(const fr& a, const fr& b) {
   return fr_reduce_once(t);
 }
 
+#ifdef PL_MUL_NOINLINE
+__device__ __noinline__ fr fr_mul_call(fr a, fr b) { return fr_mul_t<false>(a, b); }
+__device__ __forceinline__ fr fr_mul(const fr& a, const fr& b) { return fr_mul_call(a, b); }
+__device__ __forceinline__ fr fr_mul_u(const fr& a, const fr& b) { return fr_mul_call(a, b); }
+#else
 __device__ __forceinline__ fr fr_mul(const fr& a, const fr& b) { return fr_mul_t<false>(a, b); }
 __device__ __forceinline__ fr fr_mul_u(const fr& a, const fr& b) { return fr_mul_t<true>(a, b); }
+#endif
 __device__ __forceinline__ fr fr_sqr(const fr& a) { return fr_mul(a, a); }
 
 // a^e, e a 64-bit exponent (square and multiply, MSB first)

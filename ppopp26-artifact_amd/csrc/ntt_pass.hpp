@@ -104,18 +104,53 @@ struct PassArgs {
   fr scale;
 };
 
-constexpr int kThreads = 128;
-constexpr int kTile = 2048;
+// Tile geometry (compile-time; see the file header):
+//   PL_NTT_LGTILE  log2 elements per workgroup tile (16 per thread)
+//   PL_NTT_PHASES  LDS exchange rounds (8/PHASES limbs each)
+//   PL_NTT_WAVES   waves per SIMD the register allocation must allow
+#ifndef PL_NTT_LGTILE
+#define PL_NTT_LGTILE 11
+#endif
+#ifndef PL_NTT_PHASES
+#define PL_NTT_PHASES 2
+#endif
+#ifndef PL_NTT_WAVES
+#define PL_NTT_WAVES 2
+#endif
+constexpr int kLgTile = PL_NTT_LGTILE;
+constexpr int kTile = 1 << kLgTile;
+constexpr int kThreads = kTile / 16;
+constexpr int kPhases = PL_NTT_PHASES;
+constexpr int kW = 8 / kPhases;  // limbs per exchange round
+
+template <int W> struct lds_vec;
+template <> struct lds_vec<4> { typedef uint4 type; };
+template <> struct lds_vec<2> { typedef uint2 type; };
+template <> struct lds_vec<1> { typedef uint32_t type; };
+
+template <int W>
+__device__ __forceinline__ typename lds_vec<W>::type pack_limbs(const uint32_t* v) {
+  if constexpr (W == 4) return make_uint4(v[0], v[1], v[2], v[3]);
+  else if constexpr (W == 2) return make_uint2(v[0], v[1]);
+  else return v[0];
+}
+template <int W>
+__device__ __forceinline__ void unpack_limbs(uint32_t* v, const typename lds_vec<W>::type& q) {
+  if constexpr (W == 4) { v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w; }
+  else if constexpr (W == 2) { v[0] = q.x; v[1] = q.y; }
+  else v[0] = q;
+}
 
 // One pass over columns of M' = 16*A points, A = 2^LA.
 template <int LA, bool INV, bool LAST>
-__global__ void __launch_bounds__(kThreads, 2) ntt_pass_kernel(PassArgs a) {
+__global__ void __launch_bounds__(kThreads, PL_NTT_WAVES) ntt_pass_kernel(PassArgs a) {
   constexpr int A = 1 << LA;
   constexpr int m = LA + 4;
-  constexpr int LGC = 11 - m;  // log2 lines per tile
+  constexpr int LGC = kLgTile - m;  // log2 lines per tile
   constexpr int C = 1 << LGC;
   constexpr int F2 = 16 / A;   // DFT_A groups per thread in sub-step 2
-  __shared__ uint4 lds[kTile];  // one 16-byte half of every tile element
+  typedef typename lds_vec<kW>::type vec;
+  __shared__ vec lds[kTile];  // kW limbs of every tile element
 
   const int t = threadIdx.x;
   const uint64_t line0 = (uint64_t)blockIdx.x << LGC;
@@ -187,29 +222,30 @@ __global__ void __launch_bounds__(kThreads, 2) ntt_pass_kernel(PassArgs a) {
       x[brev(k2, 4)] = fr_mul(x[brev(k2, 4)], fr_load(a.inner + (e << (8 - m))));
     });
   }
-  // Exchange through LDS in two 16-byte halves (32 KiB per workgroup, so
-  // four workgroups = two waves per SIMD fit a CU).  LDS slot of
-  // W[line][rho1][k2]; consecutive lanes hit consecutive slots.
+  // Exchange through LDS in kPhases rounds of kW limbs (2 rounds of 16 bytes
+  // = 32 KiB for a 2048-element tile, so four workgroups = two waves per SIMD
+  // fit a CU).  LDS slot of W[line][rho1][k2]; consecutive lanes hit
+  // consecutive slots.  Each round's gathered limbs go back into the
+  // registers that round vacated: y lives in x (x[f*A + r1] afterwards).
+  // The line index is XOR-swizzled with rho1 so that both the line-major
+  // (intermediate passes) and the rho-major (last pass) write mappings and
+  // the line-major reads are free of LDS bank conflicts.
   const int lc2 = t & (C - 1);
   const int g = t >> LGC;
-  // Each half of W goes out of x and the gathered half of y comes back into
-  // the registers it vacated: y lives in x (x[f*A + r1] after the exchange).
   fr (&y)[16] = x;
-  static_for<0, 2>([&](auto h_) {
+  static_for<0, kPhases>([&](auto h_) {
     constexpr int h = decltype(h_)::value;
-    uint32_t hv[16][4];
+    uint32_t hv[16][kW];
     static_for<0, 16>([&](auto k2_) {
       constexpr int k2 = decltype(k2_)::value;
-      hv[k2][0] = x[brev(k2, 4)].l[4 * h + 0];
-      hv[k2][1] = x[brev(k2, 4)].l[4 * h + 1];
-      hv[k2][2] = x[brev(k2, 4)].l[4 * h + 2];
-      hv[k2][3] = x[brev(k2, 4)].l[4 * h + 3];
+#pragma unroll
+      for (int w = 0; w < kW; w++) hv[k2][w] = x[brev(k2, 4)].l[kW * h + w];
     });
-    if constexpr (h == 1) __syncthreads();  // phase-0 reads done before overwrite
+    if constexpr (h > 0) __syncthreads();  // previous round's reads done before overwrite
     static_for<0, 16>([&](auto k2_) {
       constexpr int k2 = decltype(k2_)::value;
-      const int slot = ((k2 * A + rho1) << LGC) + lc;
-      lds[slot] = make_uint4(hv[k2][0], hv[k2][1], hv[k2][2], hv[k2][3]);
+      const int slot = ((k2 * A + rho1) << LGC) + (lc ^ (rho1 & (C - 1)));
+      lds[slot] = pack_limbs<kW>(hv[k2]);
     });
     __syncthreads();
     // sub-step 2 gathers: DFT_A over rho1 for F2 values of k2 of one line
@@ -218,12 +254,8 @@ __global__ void __launch_bounds__(kThreads, 2) ntt_pass_kernel(PassArgs a) {
       const int k2 = g * F2 + f;
       static_for<0, A>([&](auto r1_) {
         constexpr int r1 = decltype(r1_)::value;
-        const int slot = ((k2 * A + r1) << LGC) + lc2;
-        const uint4 q = lds[slot];
-        y[f * A + r1].l[4 * h + 0] = q.x;
-        y[f * A + r1].l[4 * h + 1] = q.y;
-        y[f * A + r1].l[4 * h + 2] = q.z;
-        y[f * A + r1].l[4 * h + 3] = q.w;
+        const int slot = ((k2 * A + r1) << LGC) + (lc2 ^ (r1 & (C - 1)));
+        unpack_limbs<kW>(&y[f * A + r1].l[kW * h], lds[slot]);
       });
     });
   });
@@ -283,7 +315,7 @@ __global__ void __launch_bounds__(kThreads, 2) ntt_pass_kernel(PassArgs a) {
 template <int LA, bool INV, bool LAST>
 static void launch_pass(const PassArgs& a, hipStream_t s) {
   constexpr int m = LA + 4;
-  uint64_t tiles = (a.nlines + (kTile >> m) - 1) >> (11 - m);
+  uint64_t tiles = (a.nlines + (kTile >> m) - 1) >> (kLgTile - m);
   hipLaunchKernelGGL((ntt_pass_kernel<LA, INV, LAST>), dim3((unsigned)tiles), dim3(kThreads), 0, s, a);
 }
 
