@@ -118,6 +118,37 @@ int pl_eval(const void* d_poly, uint64_t n, const void* d_x, void* d_y, void* d_
  * discards it).  c is a device element and is not modified. */
 int pl_div_linear(void* d_inout, uint64_t n, const void* d_c, void* d_rem, void* d_work, pl_stream_t stream);
 
+/* ---- element-wise operators ---------------------------------------------
+ * [add_mod / sub_mod / mul_mod / div_mod and the _scalar forms, neg_mod,
+ *  inv_mod, to_mont, to_base: PLONK/utils/function.cuh:12-48, function.cu:5-152]
+ * out[i] = a[i] op b[i]; for the *_SCALAR ops b is one device element.
+ * DIV and INV follow the reference's x/0 = 0 (mont_t.cuh:1117-1126) and use
+ * one field inversion per call (Montgomery's trick, block-hierarchical).
+ * out may alias a or b.  d_work: NULL or pl_vec_work_bytes(n) bytes. */
+enum pl_vec_op_code {
+  PL_VEC_ADD = 0, PL_VEC_SUB = 1, PL_VEC_MUL = 2, PL_VEC_DIV = 3,
+  PL_VEC_ADD_SCALAR = 4, PL_VEC_SUB_SCALAR = 5, PL_VEC_MUL_SCALAR = 6, PL_VEC_DIV_SCALAR = 7,
+  PL_VEC_NEG = 8, PL_VEC_INV = 9, PL_VEC_TO_MONT = 10, PL_VEC_FROM_MONT = 11
+};
+size_t pl_vec_work_bytes(uint64_t n);
+int pl_vec_op(int op, const void* d_a, const void* d_b, void* d_out, uint64_t n, void* d_work,
+              pl_stream_t stream);
+
+/* ---- PLONK permutation ratio --------------------------------------------
+ * (the quotient term shape of zksnark_compute_permutation_constraint.cu:13-14,
+ *  built from element-wise calls in the reference)
+ * ratio[i] = prod_j (w_j[i] + beta*k_j*w^i + gamma) / prod_j (w_j[i] + beta*s_j[i] + gamma)
+ * for j = 0..2, w = omega_{2^lg} from the domain d (d->lg == lg), x/0 = 0.
+ * d_w, d_s: three device columns each; beta, gamma device elements; h_k three
+ * host Montgomery constants (k1, k2, k3).  d_work: NULL or
+ * pl_vec_work_bytes(2^lg) bytes.  pl_perm_grand_product then runs the
+ * exclusive grand product over the ratio in place (the PLONK z column). */
+int pl_perm_ratio(const void* const* d_w, const void* const* d_s, const void* d_beta, const void* d_gamma,
+                  const void* h_k, int lg, const pl_domain* d, void* d_ratio, void* d_work, pl_stream_t stream);
+int pl_perm_grand_product(const void* const* d_w, const void* const* d_s, const void* d_beta,
+                          const void* d_gamma, const void* h_k, int lg, const pl_domain* d, void* d_z,
+                          void* d_work, pl_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

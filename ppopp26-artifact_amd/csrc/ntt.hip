@@ -122,12 +122,14 @@ hipError_t domain_build(Domain& d, int lg, hipStream_t s) {
   d.lg = lg;
   d.plan = make_plan(lg);
   const NttPlan& P = d.plan;
-  size_t count = 2 * 256;  // inner tables
+  size_t count = 2 * 256 + 2 * 16384;  // inner tables, w_N power windows
   for (int p = 0; p + 1 < P.P; p++) count += 2 * (1ull << P.lgM[p]);
   d.bytes = count * sizeof(fr);
   hipError_t e = hipMalloc((void**)&d.block, d.bytes);
   if (e != hipSuccess) return e;
   fr* cur = d.block;
+  d.pw0 = cur; cur += 16384;
+  d.pw1 = cur; cur += 16384;
   d.inner_fwd = cur; cur += 256;
   d.inner_inv = cur; cur += 256;
   for (int p = 0; p + 1 < P.P; p++) {
@@ -140,6 +142,14 @@ hipError_t domain_build(Domain& d, int lg, hipStream_t s) {
   host::fr64 w256 = host::root_of_unity(8);
   host::fr64 w256i = host::inv(w256);
   d.n_inv = host::domain_size_inverse(lg);
+  {
+    // powers of w_N in two 2^14 windows (permutation ratio, coset tables)
+    fr b = to_dev(host::root_of_unity(lg)), one = to_dev(host::one());
+    hipLaunchKernelGGL(pow_direct_kernel, dim3(grid_for(16384, 256)), dim3(256), 0, s, d.pw0, (uint64_t)16384, b, 0u,
+                       one);
+    hipLaunchKernelGGL(pow_direct_kernel, dim3(grid_for(16384, 256)), dim3(256), 0, s, d.pw1, (uint64_t)16384, b, 14u,
+                       one);
+  }
   if ((e = fill_pow(d.inner_fwd, 256, 0, w256, nullptr, win, s)) != hipSuccess) goto done;
   if ((e = fill_pow(d.inner_inv, 256, 0, w256i, nullptr, win, s)) != hipSuccess) goto done;
   for (int p = 0; p + 1 < P.P; p++) {

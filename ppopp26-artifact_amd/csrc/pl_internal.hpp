@@ -33,6 +33,8 @@ struct Domain {
   NttPlan plan;
   fr* tw_fwd[PL_MAX_PASSES] = {};  // pass twiddles, 2^lgM[p] entries, p < P-1
   fr* tw_inv[PL_MAX_PASSES] = {};  // inverse; pass 0 also carries 2^-lg when P >= 2
+  fr* pw0 = nullptr;               // w_N^i = pw0[i & 16383] * pw1[i >> 14]
+  fr* pw1 = nullptr;
   fr* inner_fwd = nullptr;         // omega_256^e, e < 256
   fr* inner_inv = nullptr;
   fr* block = nullptr;             // single allocation holding all tables
@@ -79,6 +81,13 @@ hipError_t fill_zero(fr* p, uint64_t n, hipStream_t s);
 hipError_t grand_product(fr* inout, fr* incl_out, uint64_t n, fr* scratch, hipStream_t s);
 hipError_t div_linear(fr* inout, uint64_t n, const fr* c, fr* rem, fr* scratch, hipStream_t s);
 hipError_t evaluate(const fr* coeffs, uint64_t n, const fr* x, fr* y, fr* scratch, hipStream_t s);
+
+// element-wise ops, batch inversion, permutation ratio (ops.hip)
+uint64_t ops_scratch_elems(uint64_t n);
+hipError_t vec_op(int op, const fr* a, const fr* b, fr* out, uint64_t n, fr* scratch, hipStream_t s);
+hipError_t perm_ratio(const fr* const w[3], const fr* const sg[3], const fr* beta, const fr* gamma,
+                      const fr* w0, const fr* w1, const host::fr64 k[3], fr* ratio, uint64_t n, fr* scratch,
+                      hipStream_t s);
 
 // element-wise helper kernels
 hipError_t fill_pow_table(fr* out, uint64_t n, const host::fr64& base, const host::fr64& scale, hipStream_t s);

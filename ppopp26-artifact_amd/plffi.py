@@ -54,7 +54,15 @@ SIGNATURES = {
     "pl_grand_product": (_i, [_vp, _u64, _vp, _vp, _vp]),
     "pl_eval": (_i, [_vp, _u64, _vp, _vp, _vp, _vp]),
     "pl_div_linear": (_i, [_vp, _u64, _vp, _vp, _vp, _vp]),
+    "pl_vec_work_bytes": (ctypes.c_size_t, [_u64]),
+    "pl_vec_op": (_i, [_i, _vp, _vp, _vp, _u64, _vp, _vp]),
+    "pl_perm_ratio": (_i, [ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp]),
+    "pl_perm_grand_product": (_i, [ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp, _vp, _vp, _i, _vp, _vp, _vp,
+                                   _vp]),
 }
+
+VEC_OPS = {"add": 0, "sub": 1, "mul": 2, "div": 3, "add_scalar": 4, "sub_scalar": 5, "mul_scalar": 6,
+           "div_scalar": 7, "neg": 8, "inv": 9, "to_mont": 10, "from_mont": 11}
 
 
 def lib():
@@ -195,6 +203,35 @@ def div_linear(a, c, rem=None, work=None, stream=None):
     check("pl_div_linear", lib().pl_div_linear(_ptr(a), a.shape[0], _ptr(c), _ptr(rem), _ptr(work),
                                                _stream(stream)))
     return a
+
+
+def vec_op(op, a, b=None, out=None, work=None, stream=None):
+    import torch
+    if out is None:
+        out = torch.empty_like(a)
+    check("pl_vec_op", lib().pl_vec_op(VEC_OPS[op], _ptr(a), _ptr(b), _ptr(out), a.shape[0], _ptr(work),
+                                       _stream(stream)))
+    return out
+
+
+def _ptr_array(ts):
+    arr = (ctypes.c_void_p * len(ts))(*[t.data_ptr() for t in ts])
+    return ctypes.cast(arr, ctypes.POINTER(ctypes.c_void_p)), arr
+
+
+def perm_ratio(w, s, beta, gamma, k, dom, out=None, grand_product=False, work=None, stream=None):
+    """w, s: three device columns each; k: numpy (3, 4) uint64 Montgomery constants"""
+    import numpy as np
+    import torch
+    if out is None:
+        out = torch.empty_like(w[0])
+    kk = np.ascontiguousarray(k, dtype=np.uint64).reshape(3, 4)
+    wp, _w = _ptr_array(w)
+    sp, _s = _ptr_array(s)
+    fn = lib().pl_perm_grand_product if grand_product else lib().pl_perm_ratio
+    check("pl_perm_ratio", fn(wp, sp, _ptr(beta), _ptr(gamma), ctypes.c_void_p(kk.ctypes.data), dom.lg, dom.h,
+                              _ptr(out), _ptr(work), _stream(stream)))
+    return out
 
 
 def ntt_pass(x, out, dom, p, npoly=1, inverse=False, stream=None):

@@ -1,0 +1,113 @@
+"""GPU parity of the element-wise operators, the zero-safe batch inversion and
+the PLONK permutation ratio / grand product, through the C-ABI.  Bit-exact."""
+import numpy as np
+import pytest
+
+import pyoracle as po
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.fixture(scope="module")
+def pl():
+    import torch
+    assert torch.cuda.is_available(), "GPU tests need a HIP device"
+    import plffi
+    plffi.lib()
+    return plffi
+
+
+def host_op(op, a, b):
+    if op == "add":
+        return po.add(a, b)
+    if op == "sub":
+        return po.sub(a, b)
+    if op == "mul":
+        return po.mul(a, b)
+    if op == "div":
+        return po.mul(a, po.inv(b))
+    raise ValueError(op)
+
+
+@pytest.mark.parametrize("op", ["add", "sub", "mul", "div"])
+@pytest.mark.parametrize("n", [1, 17, 4096, 4097, 70000])
+def test_binary_ops(pl, op, n):
+    a = po.uniform(n, 100 + n)
+    b = po.uniform(n, 200 + n)
+    if n > 3:
+        b[2] = 0  # x/0 = 0
+    out = pl.to_host(pl.vec_op(op, pl.to_dev(a), pl.to_dev(b)))
+    np.testing.assert_array_equal(out, host_op(op, a, b))
+
+
+@pytest.mark.parametrize("op", ["add_scalar", "sub_scalar", "mul_scalar", "div_scalar"])
+def test_scalar_ops(pl, op):
+    n = 5000
+    a = po.uniform(n, 300)
+    s = po.uniform(1, 301)
+    out = pl.to_host(pl.vec_op(op, pl.to_dev(a), pl.to_dev(s)))
+    np.testing.assert_array_equal(out, host_op(op.split("_")[0], a, np.repeat(s, n, axis=0)))
+
+
+def test_unary_ops(pl):
+    n = 3000
+    a = po.uniform(n, 400)
+    a[5] = 0
+    ad = pl.to_dev(a)
+    np.testing.assert_array_equal(pl.to_host(pl.vec_op("neg", ad)), po.neg(a))
+    np.testing.assert_array_equal(pl.to_host(pl.vec_op("inv", ad)), po.inv(a))
+    tm = np.zeros((n, 4), dtype=np.uint64)
+    for i in range(n):
+        po.lib().or_to_mont(po.P(tm[i:i + 1]), po.P(np.ascontiguousarray(a[i:i + 1])))
+    np.testing.assert_array_equal(pl.to_host(pl.vec_op("to_mont", ad)), tm)
+    np.testing.assert_array_equal(pl.to_host(pl.vec_op("from_mont", pl.to_dev(tm))), a)
+
+
+@pytest.mark.parametrize("n", [1, 2, 4096, 4097, (1 << 20) + 1, 3 << 22])
+def test_batch_inverse_large(pl, n):
+    # x * x^-1 = 1 except zeros, at sizes spanning one to three block levels
+    import torch
+    print("inv", n, flush=True)
+    a = po.uniform(n, 500 + (n & 0xfff))
+    a[n // 2] = 0
+    ad = pl.to_dev(a)
+    inv = pl.vec_op("inv", ad)
+    prod = pl.to_host(pl.vec_op("mul", ad, inv))
+    one = po.one()[0]
+    expect = np.repeat(po.one(), n, axis=0)
+    expect[n // 2] = 0
+    np.testing.assert_array_equal(prod, expect)
+    # spot values against the oracle
+    idx = [0, n - 1, n // 3]
+    ih = pl.to_host(inv)
+    for i in idx:
+        np.testing.assert_array_equal(ih[i:i + 1], po.inv(a[i:i + 1]))
+    torch.cuda.synchronize()
+
+
+def test_perm_ratio_golden(pl, golden):
+    k = golden("scan_small.npz")
+    ks = np.concatenate([po.from_u64(1), po.from_u64(7), po.from_u64(13)])
+    for lg in (3, 6):
+        dom = pl.Domain(lg)
+        w = [pl.to_dev(k["perm_%s_%d" % (nm, lg)]) for nm in ("a", "b", "c")]
+        s = [pl.to_dev(k["perm_%s_%d" % (nm, lg)]) for nm in ("s1", "s2", "s3")]
+        out = pl.perm_ratio(w, s, pl.to_dev(k["perm_beta_%d" % lg]), pl.to_dev(k["perm_gamma_%d" % lg]), ks, dom)
+        np.testing.assert_array_equal(pl.to_host(out), k["perm_ratio_%d" % lg])
+
+
+@pytest.mark.parametrize("lg", [12, 20])
+def test_perm_grand_product_vs_oracle(pl, lg):
+    n = 1 << lg
+    print("perm gp", lg, flush=True)
+    cols = [po.uniform(n, po.seed(po.OP_GP, lg, 10 + j)) for j in range(6)]
+    beta, gamma = po.uniform(1, po.seed(po.OP_GP, lg, 20)), po.uniform(1, po.seed(po.OP_GP, lg, 21))
+    ks = [po.from_u64(1), po.from_u64(7), po.from_u64(13)]
+    dom = pl.Domain(lg)
+    dev = [pl.to_dev(c) for c in cols]
+    ratio = pl.perm_ratio(dev[:3], dev[3:], pl.to_dev(beta), pl.to_dev(gamma), np.concatenate(ks), dom)
+    exp_ratio = po.perm_ratio(cols, beta, gamma, ks, lg)
+    np.testing.assert_array_equal(pl.to_host(ratio), exp_ratio)
+    z = pl.perm_ratio(dev[:3], dev[3:], pl.to_dev(beta), pl.to_dev(gamma), np.concatenate(ks), dom,
+                      grand_product=True)
+    np.testing.assert_array_equal(pl.to_host(z), po.grand_product(exp_ratio))
