@@ -51,6 +51,10 @@ const char* pl_last_hip_error(void);
 int pl_domain_create(int lg, pl_domain** out);
 int pl_domain_destroy(pl_domain* d);
 int pl_domain_lg(const pl_domain* d);
+/* The reference's windowed power tables, the semi-public prefix of its
+ * Params buffer (parameters.cuh:7-33, read by lde.cu:658-659): 4 x 16384
+ * elements w_{2^28}^i, w_{2^28}^(i*2^14), 7^i, 7^(i*2^14). */
+int pl_params_fill(void* d_out, pl_stream_t stream);
 size_t pl_domain_bytes(const pl_domain* d);
 
 /* ---- NTT / iNTT -------------------------------------------------------
@@ -117,6 +121,16 @@ int pl_eval(const void* d_poly, uint64_t n, const void* d_x, void* d_y, void* d_
  * q[n-1] = 0.  d_rem: NULL, or receives the remainder a(z) (the reference
  * discards it).  c is a device element and is not modified. */
 int pl_div_linear(void* d_inout, uint64_t n, const void* d_c, void* d_rem, void* d_work, pl_stream_t stream);
+
+/* ---- baselines ------------------------------------------------------------
+ * [standard4_ntt, ntt.cu:1039-1138; naive_lde, lde.cu:1367-1490]
+ * Workload baselines for the CLIs: bit reversal plus one radix-2 launch per
+ * stage.  pl_ntt_naive: same result as pl_ntt (forward), d_in != d_out.
+ * pl_lde_naive: the reference's naive_lde result, out[k] = NTT_N(c)[k mod N]
+ * for k < N << lg_blowup (periodic, no coset). */
+int pl_ntt_naive(const void* d_in, void* d_out, int lg, const pl_domain* d, pl_stream_t stream);
+int pl_lde_naive(const void* d_coeffs, void* d_out, int lg_n, int lg_blowup, const pl_domain* d,
+                 pl_stream_t stream);
 
 /* ---- element-wise operators ---------------------------------------------
  * [add_mod / sub_mod / mul_mod / div_mod and the _scalar forms, neg_mod,

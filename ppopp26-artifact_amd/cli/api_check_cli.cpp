@@ -1,0 +1,77 @@
+// Exercises the C++ operator API (SyncedMemory + function.hpp) on seeded
+// inputs and dumps raw results, so tests can check them against the oracle:
+//   api_check <out_dir> <lg>
+// Files (raw 32-byte elements): x.bin y.bin (inputs), ntt.bin intt.bin,
+// gp.bin gp_incl.bin, eval.bin, div.bin, add.bin mul.bin div_ew.bin,
+// add_cpu.bin (CPU-dispatched add), lde.bin (our_lde semantics, lambda=2).
+#include <string>
+
+#include "common.hpp"
+
+static void dump(const std::string& path, SyncedMemory m) {
+  FILE* f = std::fopen(path.c_str(), "wb");
+  if (!f) throw std::runtime_error("cannot write " + path);
+  std::fwrite(m.cpu_data(), 1, m.size(), f);
+  std::fclose(f);
+}
+
+static SyncedMemory seeded(uint64_t n, uint64_t seed) {
+  SyncedMemory m(n * 32);
+  cli::fill_seeded(m.mutable_cpu_data(), n, seed);
+  return m;
+}
+
+int main(int argc, char** argv) {
+  if (argc != 3) {
+    std::cerr << "Usage: " << argv[0] << " <out_dir> <lg>\n";
+    return 1;
+  }
+  try {
+    const std::string dir(argv[1]);
+    const int lg = std::atoi(argv[2]);
+    const uint64_t n = 1ull << lg;
+    SyncedMemory x = seeded(n, 1001), y = seeded(n, 1002), pt = seeded(1, 1003), c = seeded(1, 1004);
+    dump(dir + "/x.bin", x);
+    dump(dir + "/y.bin", y);
+    dump(dir + "/pt.bin", pt);
+    dump(dir + "/c.bin", c);
+    Ntt ntt(32);
+    Intt intt(32);
+    x.mutable_gpu_data();
+    SyncedMemory X = ntt.forward(x);
+    dump(dir + "/ntt.bin", X);
+    SyncedMemory xi = intt.forward(X);
+    dump(dir + "/intt.bin", xi);
+    // accumulate_mul_poly in place + inclusive output
+    SyncedMemory g = seeded(n, 1001), gi(n * 32);
+    g.mutable_gpu_data();
+    accumulate_mul_poly_(g, gi);
+    dump(dir + "/gp.bin", g);
+    dump(dir + "/gp_incl.bin", gi);
+    SyncedMemory e = evaluate(x, pt);
+    dump(dir + "/eval.bin", e);
+    SyncedMemory d = seeded(n, 1001);
+    d.mutable_gpu_data();
+    SyncedMemory q = poly_div_poly(d, c);
+    dump(dir + "/div.bin", q);
+    y.mutable_gpu_data();
+    dump(dir + "/add.bin", add_mod(x, y));
+    dump(dir + "/mul.bin", mul_mod(x, y));
+    dump(dir + "/div_ew.bin", div_mod(x, y));
+    SyncedMemory xc = seeded(n, 1001), yc = seeded(n, 1002);  // heads at CPU -> CPU path
+    dump(dir + "/add_cpu.bin", add_mod(xc, yc));
+    dump(dir + "/mul_cpu.bin", mul_mod(xc, yc));
+    // our_lde with lambda = 2 through the C-ABI
+    pl_lde_domain* ld = nullptr;
+    cli::check(pl_lde_domain_create(lg, 2, nullptr, &ld), "pl_lde_domain_create");
+    SyncedMemory L(n * 4 * 32);
+    cli::check(pl_lde(x.mutable_gpu_data(), L.mutable_gpu_data(), ld, nullptr, nullptr), "pl_lde");
+    pl_lde_domain_destroy(ld);
+    dump(dir + "/lde.bin", L);
+    std::cout << "api_check ok" << std::endl;
+    return 0;
+  } catch (const std::exception& ex) {
+    std::cerr << "error: " << ex.what() << std::endl;
+    return 1;
+  }
+}

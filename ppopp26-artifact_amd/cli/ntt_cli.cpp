@@ -1,0 +1,171 @@
+// NTT command line (the reference's ntt.cu driver, ntt.cu:1182-1505):
+//   NTT --bench-split <lg_N> | --bench-baseline <lg_N> | --check-split <lg_N>
+//       | --check-baseline <lg_N> | --bench-sweep <lg_N> [reps]
+// lg_N in {22, 24, 26}.  The input lives in pinned host memory as in the
+// reference; `our_ntt` copies it to HBM (2^lg_k chunks), runs the device
+// NTT and copies the result back, and the printed time covers all of it.
+#include <vector>
+
+#include "common.hpp"
+
+struct NttScaleConfig {
+  int lg_N, lg_i, ours_lg_k, baseline_lg_k_begin, baseline_lg_k_end, reps;
+  const char* input_file;
+};
+
+static NttScaleConfig ntt_scale_config(int lg_N) {
+  switch (lg_N) {
+    case 22: return {22, 12, 4, 1, 8, 3, "../../input.bin"};
+    case 24: return {24, 14, 4, 1, 8, 3, "../../input-24.bin"};
+    case 26: return {26, 16, 2, 1, 8, 3, "../../input-26.bin"};
+    default:
+      std::cerr << "Unsupported lg_N = " << lg_N << ". Supported NTT test scales are 22, 24, and 26." << std::endl;
+      std::exit(1);
+  }
+}
+
+static void print_workspace() { std::cout << "[workspace] mode=device-resident, full-GPU vector buffer" << std::endl; }
+
+// our_ntt (ntt.cu:979-1037): natural order in and out, in place in the host
+// buffer of `input`.  lg_i / lg_j only shape the printed split; lg_k sets the
+// number of copy chunks.  Prints host-to-host and device-only times.
+static float our_ntt(SyncedMemory input, int lg_N, int lg_i, int lg_j, int lg_k, Ntt& ntt, void* d_buf,
+                     void* d_work, hipStream_t s) {
+  (void)lg_i;
+  (void)lg_j;
+  const size_t bytes = (size_t)32 << lg_N;
+  void* h = input.mutable_cpu_data();
+  cli::EventTimer all(s), dev(s);
+  cli::hipc(hipDeviceSynchronize(), "sync");
+  all.start();
+  cli::copy_chunked(d_buf, h, bytes, 1 << lg_k, hipMemcpyHostToDevice, s);
+  dev.start();
+  cli::check(pl_ntt(d_buf, d_buf, lg_N, 0, ntt.domain(lg_N), d_work, (pl_stream_t)s), "pl_ntt");
+  float dev_ms = dev.stop_ms();
+  cli::copy_chunked(h, d_buf, bytes, 1 << lg_k, hipMemcpyDeviceToHost, s);
+  float ms = all.stop_ms();
+  std::cout << "our-ntt execution time: " << ms << " ms" << std::endl;
+  std::cout << "[our_ntt] device-resident NTT: " << dev_ms << " ms" << std::endl;
+  return ms;
+}
+
+// baseline (standard4_ntt's role, ntt.cu:1039-1138): bit reversal plus one
+// radix-2 launch per stage, host-to-host
+static float baseline_ntt(SyncedMemory input, int lg_N, int lg_k, Ntt& ntt, void* d_a, void* d_b, hipStream_t s) {
+  const size_t bytes = (size_t)32 << lg_N;
+  void* h = input.mutable_cpu_data();
+  cli::EventTimer all(s);
+  cli::hipc(hipDeviceSynchronize(), "sync");
+  all.start();
+  cli::copy_chunked(d_a, h, bytes, 1 << lg_k, hipMemcpyHostToDevice, s);
+  cli::check(pl_ntt_naive(d_a, d_b, lg_N, ntt.domain(lg_N), (pl_stream_t)s), "pl_ntt_naive");
+  cli::copy_chunked(h, d_b, bytes, 1 << lg_k, hipMemcpyDeviceToHost, s);
+  float ms = all.stop_ms();
+  std::cout << "standard4-ntt execution time: " << ms << " ms" << std::endl;
+  return ms;
+}
+
+struct DevBufs {
+  void* a = nullptr;
+  void* b = nullptr;
+  explicit DevBufs(size_t bytes) {
+    cli::hipc(hipMalloc(&a, bytes), "hipMalloc");
+    cli::hipc(hipMalloc(&b, bytes), "hipMalloc");
+  }
+  ~DevBufs() {
+    (void)hipFree(a);
+    (void)hipFree(b);
+  }
+};
+
+static int run_check(const NttScaleConfig& cfg, bool baseline) {
+  const uint64_t N = 1ull << cfg.lg_N;
+  Ntt ntt(32);
+  SyncedMemory reference_input(N * 32), ours_input(N * 32);
+  cli::read_file(cfg.input_file, reference_input.mutable_cpu_data(), N * 32, cli::seed_of(1, cfg.lg_N, 0));
+  std::memcpy(ours_input.mutable_cpu_data(), reference_input.cpu_data(), N * 32);
+  SyncedMemory expected = ntt.forward(reference_input);
+  cli::hipc(hipDeviceSynchronize(), "sync");
+  expected.mutable_cpu_data();
+  DevBufs buf(N * 32);
+  if (baseline) {
+    std::cout << "[workspace] mode=naive, bit reversal + one radix-2 launch per stage" << std::endl;
+    baseline_ntt(ours_input, cfg.lg_N, cfg.ours_lg_k, ntt, buf.a, buf.b, nullptr);
+  } else {
+    print_workspace();
+    our_ntt(ours_input, cfg.lg_N, cfg.lg_i, cfg.lg_N - cfg.lg_i, cfg.ours_lg_k, ntt, buf.a, buf.b, nullptr);
+  }
+  cli::hipc(hipDeviceSynchronize(), "sync");
+  return cli::compare_outputs(expected.cpu_data(), ours_input.cpu_data(), N) ? 0 : 1;
+}
+
+static int run_bench(const NttScaleConfig& cfg, int k_begin, int k_end, int reps, bool baseline) {
+  const uint64_t N = 1ull << cfg.lg_N;
+  const int lg_j = cfg.lg_N - cfg.lg_i;
+  Ntt ntt(32);
+  SyncedMemory input(N * 32);
+  void* h = input.mutable_cpu_data();
+  DevBufs buf(N * 32);
+  if (baseline)
+    std::cout << "[workspace] mode=naive, bit reversal + one radix-2 launch per stage" << std::endl;
+  else
+    print_workspace();
+  for (int lg_k = k_begin; lg_k <= k_end; lg_k++) {
+    const size_t chunk_bytes = ((size_t)1 << (cfg.lg_i + lg_k)) * 32;
+    std::cout << "=== lg_N = " << cfg.lg_N << ", split = " << cfg.lg_i << "+" << lg_j
+              << ", mode = " << (baseline ? "baseline-naive" : "device-resident")
+              << ", chunk_size = " << cli::format_bytes(chunk_bytes) << " ===" << std::endl;
+    for (int rep = 0; rep < reps; rep++) {
+      cli::read_file(cfg.input_file, h, N * 32, cli::seed_of(1, cfg.lg_N, 0));
+      if (baseline) {
+        std::cout << "[standard4_ntt] run " << (rep + 1) << "/" << reps << std::endl;
+        baseline_ntt(input, cfg.lg_N, lg_k, ntt, buf.a, buf.b, nullptr);
+      } else {
+        std::cout << "[our_ntt] run " << (rep + 1) << "/" << reps << std::endl;
+        our_ntt(input, cfg.lg_N, cfg.lg_i, lg_j, lg_k, ntt, buf.a, buf.b, nullptr);
+      }
+    }
+  }
+  return 0;
+}
+
+static void usage(const char* exe) {
+  std::cerr << "Usage:\n"
+            << "  " << exe << " --bench-split <lg_N>\n"
+            << "  " << exe << " --bench-baseline <lg_N>\n"
+            << "  " << exe << " --check-split <lg_N>\n"
+            << "  " << exe << " --check-baseline <lg_N>\n"
+            << "  " << exe << " --bench-sweep <lg_N> [reps]\n";
+}
+
+int main(int argc, char** argv) {
+  if (argc < 2) {
+    usage(argv[0]);
+    return 1;
+  }
+  try {
+    const std::string cmd(argv[1]);
+    const int lg_N = argc >= 3 ? std::atoi(argv[2]) : 22;
+    if (cmd == "--bench-baseline") {
+      NttScaleConfig c = ntt_scale_config(lg_N);
+      return run_bench(c, c.baseline_lg_k_begin, c.baseline_lg_k_end, c.reps, true);
+    }
+    if (cmd == "--check-baseline") return run_check(ntt_scale_config(lg_N), true);
+    if (cmd == "--check-split") return run_check(ntt_scale_config(lg_N), false);
+    if (cmd == "--bench-split") {
+      NttScaleConfig c = ntt_scale_config(lg_N);
+      return run_bench(c, c.ours_lg_k, c.ours_lg_k, c.reps, false);
+    }
+    if (cmd == "--bench-sweep") {
+      NttScaleConfig c = ntt_scale_config(lg_N);
+      const int reps = argc >= 4 ? std::atoi(argv[3]) : 1;
+      return run_bench(c, 1, 8, reps, false);
+    }
+    std::cerr << "Unknown command: " << cmd << "\n";
+    usage(argv[0]);
+    return 1;
+  } catch (const std::exception& e) {
+    std::cerr << "error: " << e.what() << std::endl;
+    return 1;
+  }
+}

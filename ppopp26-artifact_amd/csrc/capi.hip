@@ -69,6 +69,12 @@ int pl_domain_destroy(pl_domain* d) {
   return PL_OK;
 }
 
+int pl_params_fill(void* d_out, pl_stream_t stream) {
+  if (!d_out || !aligned16(d_out)) return PL_ERR_ARG;
+  hipError_t e = pl::params_fill((pl::fr*)d_out, (hipStream_t)stream);
+  return e == hipSuccess ? PL_OK : hip_fail(e);
+}
+
 int pl_domain_lg(const pl_domain* d) { return d ? d->d.lg : PL_ERR_ARG; }
 
 size_t pl_domain_bytes(const pl_domain* d) { return d ? d->d.bytes : 0; }
@@ -307,6 +313,21 @@ int pl_perm_grand_product(const void* const* d_w, const void* const* d_s, const 
   int st = pl_perm_ratio(d_w, d_s, d_beta, d_gamma, h_k, lg, d, d_z, d_work, stream);
   if (st != PL_OK) return st;
   return pl_grand_product(d_z, 1ull << lg, nullptr, d_work, stream);
+}
+
+int pl_ntt_naive(const void* d_in, void* d_out, int lg, const pl_domain* d, pl_stream_t stream) {
+  if (!d || !d_in || !d_out || d_in == d_out || d->d.lg != lg) return PL_ERR_ARG;
+  if (!aligned16(d_in) || !aligned16(d_out)) return PL_ERR_ARG;
+  hipError_t e = pl::ntt_naive(d->d, (const pl::fr*)d_in, (pl::fr*)d_out, (hipStream_t)stream);
+  return e == hipSuccess ? PL_OK : hip_fail(e);
+}
+
+int pl_lde_naive(const void* d_coeffs, void* d_out, int lg_n, int lg_blowup, const pl_domain* d, pl_stream_t stream) {
+  if (!d || !d_coeffs || !d_out || d_coeffs == d_out || d->d.lg != lg_n || lg_blowup < 1 || lg_n + lg_blowup > 28)
+    return PL_ERR_ARG;
+  if (!aligned16(d_coeffs) || !aligned16(d_out)) return PL_ERR_ARG;
+  hipError_t e = pl::lde_naive(d->d, (const pl::fr*)d_coeffs, (pl::fr*)d_out, lg_blowup, (hipStream_t)stream);
+  return e == hipSuccess ? PL_OK : hip_fail(e);
 }
 
 }  // extern "C"

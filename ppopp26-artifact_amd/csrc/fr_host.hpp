@@ -66,6 +66,39 @@ inline fr64 mul(const fr64& a, const fr64& b) {
   return r;
 }
 
+inline fr64 add(const fr64& a, const fr64& b) {
+  fr64 r;
+  unsigned __int128 c = 0;
+  for (int i = 0; i < 4; i++) {
+    c = (unsigned __int128)a.l[i] + b.l[i] + (uint64_t)c;
+    r.l[i] = (uint64_t)c;
+    c >>= 64;
+  }
+  if (c || geq_p(r.l)) sub_p(r.l);
+  return r;
+}
+
+inline fr64 sub(const fr64& a, const fr64& b) {
+  fr64 r;
+  unsigned __int128 br = 0;
+  for (int i = 0; i < 4; i++) {
+    unsigned __int128 d = (unsigned __int128)a.l[i] - b.l[i] - (uint64_t)br;
+    r.l[i] = (uint64_t)d;
+    br = (d >> 64) & 1;
+  }
+  if (br) {
+    unsigned __int128 c = 0;
+    for (int i = 0; i < 4; i++) {
+      c = (unsigned __int128)r.l[i] + kP[i] + (uint64_t)c;
+      r.l[i] = (uint64_t)c;
+      c >>= 64;
+    }
+  }
+  return r;
+}
+
+inline bool is_zero(const fr64& a) { return (a.l[0] | a.l[1] | a.l[2] | a.l[3]) == 0; }
+
 inline fr64 one() {
   fr64 r;
   memcpy(r.l, kOne, 32);

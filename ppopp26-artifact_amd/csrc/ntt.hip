@@ -189,6 +189,20 @@ hipError_t lde_domain_build(LdeDomain& d, int lg_n, int lg_blowup, const host::f
   return e;
 }
 
+// The reference's windowed power tables (parameters.cuh:7-33; the first
+// 4 * 2^14 elements of its Params): w_{2^28}^i, w_{2^28}^(i 2^14), 7^i, 7^(i 2^14)
+hipError_t params_fill(fr* out, hipStream_t s) {
+  const fr one = to_dev(host::one());
+  const fr w = to_dev(host::root_of_unity(28));
+  const fr g = to_dev(host::from_u64(7));
+  const uint64_t W = 16384;
+  hipLaunchKernelGGL(pow_direct_kernel, dim3(grid_for(W, 256)), dim3(256), 0, s, out, W, w, 0u, one);
+  hipLaunchKernelGGL(pow_direct_kernel, dim3(grid_for(W, 256)), dim3(256), 0, s, out + W, W, w, 14u, one);
+  hipLaunchKernelGGL(pow_direct_kernel, dim3(grid_for(W, 256)), dim3(256), 0, s, out + 2 * W, W, g, 0u, one);
+  hipLaunchKernelGGL(pow_direct_kernel, dim3(grid_for(W, 256)), dim3(256), 0, s, out + 3 * W, W, g, 14u, one);
+  return hipGetLastError();
+}
+
 void lde_domain_free(LdeDomain& d) {
   domain_free(d.ntt);
   if (d.cp) hipFree(d.cp);

@@ -72,6 +72,9 @@ hipError_t ntt_launch(const Domain& d, const NttCall& c, hipStream_t s);
 hipError_t ntt_launch_one(const Domain& d, const NttCall& c, int p, const fr* in, uint64_t in_pstride,
                           const fr* premul, fr* out, hipStream_t s);
 hipError_t domain_build(Domain& d, int lg, hipStream_t s);
+hipError_t params_fill(fr* out, hipStream_t s);
+hipError_t ntt_naive(const Domain& dom, const fr* in, fr* out, hipStream_t s);
+hipError_t lde_naive(const Domain& dom, const fr* in, fr* out, int lg_blowup, hipStream_t s);
 void domain_free(Domain& d);
 
 // scans (scan.hip)

@@ -39,6 +39,7 @@ SIGNATURES = {
     "pl_domain_create": (_i, [_i, ctypes.POINTER(_vp)]),
     "pl_domain_destroy": (_i, [_vp]),
     "pl_domain_lg": (_i, [_vp]),
+    "pl_params_fill": (_i, [_vp, _vp]),
     "pl_domain_bytes": (ctypes.c_size_t, [_vp]),
     "pl_ntt": (_i, [_vp, _vp, _i, _i, _vp, _vp, _vp]),
     "pl_ntt_batch": (_i, [_vp, _vp, _i, _u32, _i, _vp, _vp, _vp]),
@@ -55,6 +56,8 @@ SIGNATURES = {
     "pl_eval": (_i, [_vp, _u64, _vp, _vp, _vp, _vp]),
     "pl_div_linear": (_i, [_vp, _u64, _vp, _vp, _vp, _vp]),
     "pl_vec_work_bytes": (ctypes.c_size_t, [_u64]),
+    "pl_ntt_naive": (_i, [_vp, _vp, _i, _vp, _vp]),
+    "pl_lde_naive": (_i, [_vp, _vp, _i, _i, _vp, _vp]),
     "pl_vec_op": (_i, [_i, _vp, _vp, _vp, _u64, _vp, _vp]),
     "pl_perm_ratio": (_i, [ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp]),
     "pl_perm_grand_product": (_i, [ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp, _vp, _vp, _i, _vp, _vp, _vp,

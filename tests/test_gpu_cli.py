@@ -1,0 +1,82 @@
+"""The drop-in layers on the GPU: the C++ operator API (SyncedMemory,
+Ntt/Intt, accumulate_mul_poly, evaluate, poly_div_poly, element-wise ops) and
+the NTT / LDE / miss_ops command lines with the reference's flags."""
+import os
+import subprocess
+
+import numpy as np
+import pytest
+
+import pyoracle as po
+from conftest import ROOT
+
+pytestmark = pytest.mark.gpu
+BIN = os.path.join(ROOT, "ppopp26-artifact_amd", "bin")
+
+
+def run(args, cwd=None, timeout=300):
+    print("run", args, flush=True)
+    p = subprocess.run([os.path.join(BIN, args[0])] + args[1:], cwd=cwd, capture_output=True, text=True,
+                       timeout=timeout)
+    print(p.stdout[-3000:], p.stderr[-2000:], flush=True)
+    return p
+
+
+def load(path):
+    return np.fromfile(path, dtype=np.uint64).reshape(-1, 4)
+
+
+def test_cpp_api(tmp_path):
+    lg = 14
+    p = run(["api_check", str(tmp_path), str(lg)])
+    assert p.returncode == 0
+    f = lambda n: load(os.path.join(tmp_path, n))  # noqa: E731
+    x, y, pt, c = f("x.bin"), f("y.bin"), f("pt.bin"), f("c.bin")
+    np.testing.assert_array_equal(f("ntt.bin"), po.ntt(x, lg))
+    np.testing.assert_array_equal(f("intt.bin"), x)
+    gp = po.grand_product(x)
+    np.testing.assert_array_equal(f("gp.bin"), gp)
+    np.testing.assert_array_equal(f("gp_incl.bin")[:-1], gp[1:])
+    np.testing.assert_array_equal(f("eval.bin"), po.eval_(x, pt))
+    np.testing.assert_array_equal(f("div.bin"), po.div(x, c)[0])
+    np.testing.assert_array_equal(f("add.bin"), po.add(x, y))
+    np.testing.assert_array_equal(f("mul.bin"), po.mul(x, y))
+    np.testing.assert_array_equal(f("div_ew.bin"), po.mul(x, po.inv(y)))
+    np.testing.assert_array_equal(f("add_cpu.bin"), po.add(x, y))
+    np.testing.assert_array_equal(f("mul_cpu.bin"), po.mul(x, y))
+    np.testing.assert_array_equal(f("lde.bin"), po.lde(x, lg, 2, po.ref_lde_shift(2)))
+
+
+@pytest.mark.parametrize("mode", ["--check-split", "--check-baseline"])
+def test_ntt_cli_check(mode):
+    p = run(["NTT", mode, "22"])
+    assert p.returncode == 0
+    assert "outputs match for 4194304 elements" in p.stdout
+
+
+def test_ntt_cli_bench_split():
+    p = run(["NTT", "--bench-split", "24"])
+    assert p.returncode == 0
+    assert p.stdout.count("our-ntt execution time:") == 3
+
+
+def test_ntt_cli_errors():
+    assert run(["NTT"]).returncode == 1
+    p = run(["NTT", "--bench-split", "21"])
+    assert p.returncode == 1 and "Unsupported lg_N = 21" in p.stderr
+
+
+def test_lde_cli():
+    p = run(["LDE", "--bench", "20", "2"])
+    assert p.returncode == 0
+    assert p.stdout.count("our-lde execution time:") == 3
+    assert "[our_lde] avg lg_N=20, lambda=2" in p.stdout
+    p = run(["LDE", "--bench", "21"])
+    assert p.returncode == 1 and "error: unsupported LDE test config" in p.stderr
+
+
+def test_miss_ops_cli():
+    p = run(["miss_ops", "--lg", "16", "--perm"])
+    assert p.returncode == 0
+    for tag in ("[naive_gp]", "[ours_gp]", "[ours_perm_gp]", "[ours_eval]", "[naive_div]", "[ours_div]"):
+        assert p.stdout.count(tag + " execution time:") == 3, tag
