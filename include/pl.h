@@ -163,6 +163,44 @@ int pl_perm_grand_product(const void* const* d_w, const void* const* d_s, const 
                           const void* d_gamma, const void* h_k, int lg, const pl_domain* d, void* d_z,
                           void* d_work, pl_stream_t stream);
 
+/* ---- batched form: the uneven-size polynomials of one proof ---------------
+ * (no single counterpart: the reference runs one our_ntt / our_lde /
+ *  evaluate / poly_div_poly call per polynomial, ntt.cu:979-1037,
+ *  lde.cu:600-688, mont_arithmetic.cu:104-163, zksnark_poly_div.cu)
+ * A batch is planned once over caller-owned device buffers: the transforms'
+ * passes are grouped by depth and radix into ragged launches (one launch
+ * covers the tiles of many polynomials), evaluation + division pairs are one
+ * suffix scan each on a forked stream, and with PL_BATCH_GRAPH the whole
+ * sequence is captured as one hipGraph.  pl_batch_run allocates nothing and
+ * may be repeated; the buffers named by the items must stay valid until
+ * pl_batch_destroy.  Results are bit-identical to the per-call entry points:
+ *   NTT / INTT  out = pl_ntt(in, lg, inverse)          (out may alias in)
+ *   LDE         out = pl_lde(in) over 7^(2^lg_blowup) * <w_M>, M = 2^(lg + lg_blowup)
+ *   EVAL_DIV    y = p(z); out = quotient of (p(X) - p(z)) by (X - z), out[n-1] = 0,
+ *               p = in (n = 2^lg coefficients), z a device element; out != in
+ * pl_batch_run_plain runs the same items as one launch sequence per
+ * polynomial on `stream` (the per-call entry points' kernels), for
+ * comparison. */
+enum pl_batch_op { PL_BATCH_NTT = 0, PL_BATCH_INTT = 1, PL_BATCH_LDE = 2, PL_BATCH_EVAL_DIV = 3 };
+enum pl_batch_flags { PL_BATCH_GRAPH = 1 };
+typedef struct pl_batch_item {
+  int op;         /* pl_batch_op */
+  int lg;         /* log2 of the input length, 0..28 (LDE: lg + lg_blowup <= 28) */
+  int lg_blowup;  /* LDE only, 1..4 */
+  const void* d_in;
+  void* d_out;
+  const void* d_z; /* EVAL_DIV: the point */
+  void* d_y;       /* EVAL_DIV: p(z) */
+} pl_batch_item;
+typedef struct pl_batch pl_batch;
+int pl_batch_create(const pl_batch_item* items, int n_items, int flags, pl_batch** out);
+int pl_batch_run(pl_batch* b, pl_stream_t stream);
+int pl_batch_run_plain(pl_batch* b, pl_stream_t stream);
+/* launches per pl_batch_run (graph nodes when captured) and device bytes the
+ * plan owns (scratch + launch tables; domains are shared library-wide) */
+int pl_batch_info(const pl_batch* b, int* n_launches, size_t* bytes);
+void pl_batch_destroy(pl_batch* b);
+
 #ifdef __cplusplus
 }
 #endif

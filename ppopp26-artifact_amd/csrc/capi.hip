@@ -26,6 +26,10 @@ bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
 
 }  // namespace
 
+namespace pl {
+int hip_status(hipError_t e) { return hip_fail(e); }
+}  // namespace pl
+
 extern "C" {
 
 const char* pl_status_string(int status) {

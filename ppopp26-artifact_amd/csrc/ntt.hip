@@ -224,14 +224,12 @@ static void launch_pass_la(int la, bool inv, bool last, const PassArgs& a, hipSt
   }
 }
 
-// Pass p of the plan, reading `in` (poly stride in_pstride, optional premul
-// table) and writing `out`.  Intermediate passes keep the in-place layout, so
-// in may equal out for them; the last pass writes natural order.
-hipError_t ntt_launch_one(const Domain& d, const NttCall& c, int p, const fr* in, uint64_t in_pstride,
-                          const fr* premul, fr* out, hipStream_t s) {
+// Kernel arguments of pass p of the plan, reading `in` (poly stride
+// in_pstride, optional premul table) and writing `out`.
+static void fill_pass_args(const Domain& d, const NttCall& c, int p, const fr* in, uint64_t in_pstride,
+                           const fr* premul, fr* out, PassArgs& a) {
   const NttPlan& P = d.plan;
   const int lg = d.lg;
-  PassArgs a;
   memset(&a, 0, sizeof(a));
   a.inner = c.inverse ? d.inner_inv : d.inner_fwd;
   a.lgN = lg;
@@ -242,23 +240,9 @@ hipError_t ntt_launch_one(const Domain& d, const NttCall& c, int p, const fr* in
   a.out = out;
   a.in_pstride = in_pstride;
   a.premul = premul;
-  if (premul && (P.P == 0 || p == P.P - 1)) {
-    // the last-pass and tiny kernels carry no factor table (register budget):
-    // scale the input into the scratch buffer first.  The scratch copy also
-    // makes a shared-input call (in_pstride 0) safe when out aliases in.
-    if (!c.tmp) return hipErrorInvalidValue;
-    uint64_t total = (uint64_t)c.npoly << lg;
-    hipLaunchKernelGGL(premul_kernel, dim3(grid_for(total, 256)), dim3(256), 0, s, in, c.tmp, in_pstride, premul,
-                       c.pm_mask, (uint32_t)lg, total);
-    a.in = c.tmp;
-    a.in_pstride = 1ull << lg;
-    a.premul = nullptr;
-  }
   if (P.P == 0) {
     a.do_scale = c.inverse ? 1 : 0;
-    uint64_t total = (uint64_t)c.npoly << lg;
-    hipLaunchKernelGGL(ntt_tiny_kernel, dim3(grid_for(total, 256)), dim3(256), 0, s, a, c.npoly);
-    return hipGetLastError();
+    return;
   }
   const bool last = p == P.P - 1;
   a.tw = last ? nullptr : (c.inverse ? d.tw_inv[p] : d.tw_fwd[p]);
@@ -270,8 +254,44 @@ hipError_t ntt_launch_one(const Domain& d, const NttCall& c, int p, const fr* in
     a.ndig = p;
     for (int q = 0; q < p; q++) a.lgrad[q] = P.m[q];
   }
-  launch_pass_la(P.m[p] - 4, c.inverse, last, a, s);
+}
+
+// Pass p of the plan.  Intermediate passes keep the in-place layout, so in
+// may equal out for them; the last pass writes natural order.
+hipError_t ntt_launch_one(const Domain& d, const NttCall& c, int p, const fr* in, uint64_t in_pstride,
+                          const fr* premul, fr* out, hipStream_t s) {
+  const NttPlan& P = d.plan;
+  const int lg = d.lg;
+  if (premul && (P.P == 0 || p == P.P - 1)) {
+    // the last-pass and tiny kernels carry no factor table (register budget):
+    // scale the input into the scratch buffer first.  The scratch copy also
+    // makes a shared-input call (in_pstride 0) safe when out aliases in.
+    if (!c.tmp) return hipErrorInvalidValue;
+    uint64_t total = (uint64_t)c.npoly << lg;
+    hipLaunchKernelGGL(premul_kernel, dim3(grid_for(total, 256)), dim3(256), 0, s, in, c.tmp, in_pstride, premul,
+                       c.pm_mask, (uint32_t)lg, total);
+    in = c.tmp;
+    in_pstride = 1ull << lg;
+    premul = nullptr;
+  }
+  PassArgs a;
+  fill_pass_args(d, c, p, in, in_pstride, premul, out, a);
+  if (P.P == 0) {
+    uint64_t total = (uint64_t)c.npoly << lg;
+    hipLaunchKernelGGL(ntt_tiny_kernel, dim3(grid_for(total, 256)), dim3(256), 0, s, a, c.npoly);
+    return hipGetLastError();
+  }
+  launch_pass_la(P.m[p] - 4, c.inverse, p == P.P - 1, a, s);
   return hipGetLastError();
+}
+
+bool pass_args_regular(const Domain& d, const NttCall& c, int p, PassArgs& a) {
+  const NttPlan& P = d.plan;
+  if (P.P == 0 || (c.premul && P.P == 1) || p < 0 || p >= P.P) return false;
+  const bool last = p == P.P - 1;
+  fill_pass_args(d, c, p, p == 0 ? c.in : c.tmp, p == 0 ? c.in_pstride : (1ull << d.lg), p == 0 ? c.premul : nullptr,
+                 last ? c.out : c.tmp, a);
+  return true;
 }
 
 hipError_t ntt_launch(const Domain& d, const NttCall& c, hipStream_t s) {

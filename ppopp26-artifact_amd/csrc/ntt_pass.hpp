@@ -141,9 +141,9 @@ __device__ __forceinline__ void unpack_limbs(uint32_t* v, const typename lds_vec
   else v[0] = q;
 }
 
-// One pass over columns of M' = 16*A points, A = 2^LA.
+// One pass over columns of M' = 16*A points, A = 2^LA; blk = tile index.
 template <int LA, bool INV, bool LAST>
-__global__ void __launch_bounds__(kThreads, PL_NTT_WAVES) ntt_pass_kernel(PassArgs a) {
+__device__ __forceinline__ void ntt_pass_body(const PassArgs& a, const uint64_t blk) {
   constexpr int A = 1 << LA;
   constexpr int m = LA + 4;
   constexpr int LGC = kLgTile - m;  // log2 lines per tile
@@ -153,7 +153,7 @@ __global__ void __launch_bounds__(kThreads, PL_NTT_WAVES) ntt_pass_kernel(PassAr
   __shared__ vec lds[kTile];  // kW limbs of every tile element
 
   const int t = threadIdx.x;
-  const uint64_t line0 = (uint64_t)blockIdx.x << LGC;
+  const uint64_t line0 = blk << LGC;
   const uint32_t lgL = a.lgN - m;  // log2 lines per polynomial = log2(N / M')
 
   // ---- sub-step 1: 16 points of one (line, rho1), DFT_16 over rho2 --------
@@ -311,12 +311,38 @@ __global__ void __launch_bounds__(kThreads, PL_NTT_WAVES) ntt_pass_kernel(PassAr
 }
 
 
+template <int LA, bool INV, bool LAST>
+__global__ void __launch_bounds__(kThreads, PL_NTT_WAVES) ntt_pass_kernel(PassArgs a) {
+  ntt_pass_body<LA, INV, LAST>(a, blockIdx.x);
+}
+
+// Ragged form: one launch runs the tiles of many polynomials of different
+// sizes (one PassArgs per segment).  block_seg maps a block to its segment,
+// seg_block0 gives the segment's first block.
+template <int LA, bool INV, bool LAST>
+__global__ void __launch_bounds__(kThreads, PL_NTT_WAVES)
+    ntt_pass_ragged_kernel(const PassArgs* __restrict__ segs, const uint32_t* __restrict__ block_seg,
+                           const uint32_t* __restrict__ seg_block0) {
+  const uint32_t seg = block_seg[blockIdx.x];
+  // read the arguments in place (uniform loads); a private copy would put
+  // the dynamically indexed lgrad[] in scratch
+  const PassArgs& a = segs[seg];
+  ntt_pass_body<LA, INV, LAST>(a, blockIdx.x - seg_block0[seg]);
+}
+
 // Host launcher for one radix: LA = log2(column size) - 4.
 template <int LA, bool INV, bool LAST>
 static void launch_pass(const PassArgs& a, hipStream_t s) {
   constexpr int m = LA + 4;
   uint64_t tiles = (a.nlines + (kTile >> m) - 1) >> (kLgTile - m);
   hipLaunchKernelGGL((ntt_pass_kernel<LA, INV, LAST>), dim3((unsigned)tiles), dim3(kThreads), 0, s, a);
+}
+
+template <int LA, bool INV, bool LAST>
+static void launch_pass_ragged(const PassArgs* segs, const uint32_t* block_seg, const uint32_t* seg_block0,
+                               uint32_t nblocks, hipStream_t s) {
+  hipLaunchKernelGGL((ntt_pass_ragged_kernel<LA, INV, LAST>), dim3(nblocks), dim3(kThreads), 0, s, segs, block_seg,
+                     seg_block0);
 }
 
 #define PL_DEFINE_PASS_LAUNCHER(LA)                                        \
@@ -328,8 +354,30 @@ static void launch_pass(const PassArgs& a, hipStream_t s) {
       if (last) launch_pass<LA, false, true>(a, s);                        \
       else launch_pass<LA, false, false>(a, s);                            \
     }                                                                      \
+  }                                                                        \
+  void launch_pass_ragged_la##LA(bool inv, bool last, const PassArgs* segs, const uint32_t* bs, \
+                                 const uint32_t* sb, uint32_t nb, hipStream_t s) {            \
+    if (inv) {                                                             \
+      if (last) launch_pass_ragged<LA, true, true>(segs, bs, sb, nb, s);   \
+      else launch_pass_ragged<LA, true, false>(segs, bs, sb, nb, s);       \
+    } else {                                                               \
+      if (last) launch_pass_ragged<LA, false, true>(segs, bs, sb, nb, s);  \
+      else launch_pass_ragged<LA, false, false>(segs, bs, sb, nb, s);      \
+    }                                                                      \
   }
 
+#define PL_DECLARE_RAGGED(LA)                                                                  \
+  void launch_pass_ragged_la##LA(bool inv, bool last, const PassArgs* segs, const uint32_t* bs, \
+                                 const uint32_t* sb, uint32_t nb, hipStream_t s);
+// Kernel arguments of pass p of the call c when every pass is a regular
+// tile pass (false for the tiny and single-pass premultiplied forms, which
+// ntt_launch handles with extra kernels).
+bool pass_args_regular(const Domain& d, const NttCall& c, int p, PassArgs& a);
+PL_DECLARE_RAGGED(0)
+PL_DECLARE_RAGGED(1)
+PL_DECLARE_RAGGED(2)
+PL_DECLARE_RAGGED(3)
+PL_DECLARE_RAGGED(4)
 void launch_pass_la0(bool inv, bool last, const PassArgs& a, hipStream_t s);
 void launch_pass_la1(bool inv, bool last, const PassArgs& a, hipStream_t s);
 void launch_pass_la2(bool inv, bool last, const PassArgs& a, hipStream_t s);

@@ -72,6 +72,8 @@ hipError_t ntt_launch(const Domain& d, const NttCall& c, hipStream_t s);
 hipError_t ntt_launch_one(const Domain& d, const NttCall& c, int p, const fr* in, uint64_t in_pstride,
                           const fr* premul, fr* out, hipStream_t s);
 hipError_t domain_build(Domain& d, int lg, hipStream_t s);
+// maps a HIP error to a pl_status and records its text for pl_last_hip_error
+int hip_status(hipError_t e);
 hipError_t params_fill(fr* out, hipStream_t s);
 hipError_t ntt_naive(const Domain& dom, const fr* in, fr* out, hipStream_t s);
 hipError_t lde_naive(const Domain& dom, const fr* in, fr* out, int lg_blowup, hipStream_t s);
@@ -84,6 +86,7 @@ hipError_t fill_zero(fr* p, uint64_t n, hipStream_t s);
 hipError_t grand_product(fr* inout, fr* incl_out, uint64_t n, fr* scratch, hipStream_t s);
 hipError_t div_linear(fr* inout, uint64_t n, const fr* c, fr* rem, fr* scratch, hipStream_t s);
 hipError_t evaluate(const fr* coeffs, uint64_t n, const fr* x, fr* y, fr* scratch, hipStream_t s);
+hipError_t eval_div(const fr* in, fr* q, uint64_t n, const fr* z, fr* y, fr* scratch, hipStream_t s);
 
 // element-wise ops, batch inversion, permutation ratio (ops.hip)
 uint64_t ops_scratch_elems(uint64_t n);

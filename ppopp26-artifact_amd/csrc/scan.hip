@@ -317,4 +317,12 @@ hipError_t evaluate(const fr* coeffs, uint64_t n, const fr* x, fr* y, fr* scratc
   return suffix_scan(coeffs, nullptr, n, pw, y, scratch + kPw, s);
 }
 
+// one scan for an evaluation + division pair: q(X) = (p(X) - p(z)) / (X - z)
+// (q[n-1] = 0) and y = p(z); the suffix s_i = a_i + z s_{i+1} gives both
+hipError_t eval_div(const fr* in, fr* q, uint64_t n, const fr* z, fr* y, fr* scratch, hipStream_t s) {
+  fr* pw = scratch;
+  hipLaunchKernelGGL(sfx_powers_kernel, dim3(1), dim3(64), 0, s, z, 0, pw);
+  return suffix_scan(in, q, n, pw, y, scratch + kPw, s);
+}
+
 }  // namespace pl

@@ -62,7 +62,21 @@ SIGNATURES = {
     "pl_perm_ratio": (_i, [ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp]),
     "pl_perm_grand_product": (_i, [ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp, _vp, _vp, _i, _vp, _vp, _vp,
                                    _vp]),
+    "pl_batch_create": (_i, [_vp, _i, _i, ctypes.POINTER(_vp)]),
+    "pl_batch_run": (_i, [_vp, _vp]),
+    "pl_batch_run_plain": (_i, [_vp, _vp]),
+    "pl_batch_info": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_size_t)]),
+    "pl_batch_destroy": (None, [_vp]),
 }
+
+BATCH_OPS = {"ntt": 0, "intt": 1, "lde": 2, "eval_div": 3}
+PL_BATCH_GRAPH = 1
+
+
+class BatchItem(ctypes.Structure):
+    """pl_batch_item"""
+    _fields_ = [("op", _i), ("lg", _i), ("lg_blowup", _i), ("d_in", _vp), ("d_out", _vp), ("d_z", _vp),
+                ("d_y", _vp)]
 
 VEC_OPS = {"add": 0, "sub": 1, "mul": 2, "div": 3, "add_scalar": 4, "sub_scalar": 5, "mul_scalar": 6,
            "div_scalar": 7, "neg": 8, "inv": 9, "to_mont": 10, "from_mont": 11}
@@ -235,6 +249,53 @@ def perm_ratio(w, s, beta, gamma, k, dom, out=None, grand_product=False, work=No
     check("pl_perm_ratio", fn(wp, sp, _ptr(beta), _ptr(gamma), ctypes.c_void_p(kk.ctypes.data), dom.lg, dom.h,
                               _ptr(out), _ptr(work), _stream(stream)))
     return out
+
+
+class Batch:
+    """A planned batch over caller-owned tensors.
+
+    items: list of dicts {op: "ntt"|"intt"|"lde"|"eval_div", lg, lg_blowup,
+    inp, out, z, y} (tensors on the current device).  The tensors are kept
+    referenced for the batch's lifetime."""
+
+    def __init__(self, items, graph=True):
+        arr = (BatchItem * len(items))()
+        self._keep = []
+        for i, it in enumerate(items):
+            arr[i].op = BATCH_OPS[it["op"]]
+            arr[i].lg = it["lg"]
+            arr[i].lg_blowup = it.get("lg_blowup", 0)
+            arr[i].d_in = it["inp"].data_ptr()
+            arr[i].d_out = it["out"].data_ptr()
+            arr[i].d_z = it["z"].data_ptr() if it.get("z") is not None else None
+            arr[i].d_y = it["y"].data_ptr() if it.get("y") is not None else None
+            self._keep.append(it)
+        self.h = ctypes.c_void_p()
+        check("pl_batch_create", lib().pl_batch_create(ctypes.cast(arr, _vp), len(items),
+                                                       PL_BATCH_GRAPH if graph else 0, ctypes.byref(self.h)))
+
+    def run(self, stream=None):
+        check("pl_batch_run", lib().pl_batch_run(self.h, _stream(stream)))
+
+    def run_plain(self, stream=None):
+        check("pl_batch_run_plain", lib().pl_batch_run_plain(self.h, _stream(stream)))
+
+    def info(self):
+        n = ctypes.c_int()
+        b = ctypes.c_size_t()
+        check("pl_batch_info", lib().pl_batch_info(self.h, ctypes.byref(n), ctypes.byref(b)))
+        return n.value, b.value
+
+    def close(self):
+        if self.h:
+            lib().pl_batch_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
 
 
 def ntt_pass(x, out, dom, p, npoly=1, inverse=False, stream=None):

@@ -81,3 +81,32 @@ def test_argument_errors_without_device():
     assert L.pl_ntt(None, None, 12, 0, None, None, None) == -1
     assert L.pl_grand_product(None, 10, None, None, None) == -1
     assert L.pl_div_linear(None, 0, None, None, None, None) == -1
+
+
+def test_batch_argument_errors_without_device():
+    import plffi
+    if not os.path.exists(plffi.LIB_PATH):
+        pytest.skip("libpl.so not built")
+    L = plffi.lib()
+    h = ctypes.c_void_p()
+    assert L.pl_batch_create(None, 1, 1, ctypes.byref(h)) == -1
+    items = (plffi.BatchItem * 2)()
+    for it in items:
+        it.op, it.lg, it.d_in, it.d_out = 0, 12, 256, 512
+    assert L.pl_batch_create(ctypes.cast(items, ctypes.c_void_p), 2, 2, ctypes.byref(h)) == -1  # flags
+    assert L.pl_batch_create(ctypes.cast(items, ctypes.c_void_p), 0, 1, ctypes.byref(h)) == -1
+    items[1].lg = 29
+    assert L.pl_batch_create(ctypes.cast(items, ctypes.c_void_p), 2, 1, ctypes.byref(h)) == -3
+    items[1].lg, items[1].op = 12, 7
+    assert L.pl_batch_create(ctypes.cast(items, ctypes.c_void_p), 2, 1, ctypes.byref(h)) == -1
+    items[1].op, items[1].d_out = 3, 256  # eval_div: out aliases in, no point
+    assert L.pl_batch_create(ctypes.cast(items, ctypes.c_void_p), 2, 1, ctypes.byref(h)) == -1
+    items[1].op, items[1].lg_blowup, items[1].d_out = 2, 0, 512
+    assert L.pl_batch_create(ctypes.cast(items, ctypes.c_void_p), 2, 1, ctypes.byref(h)) == -1
+    items[1].d_in = 260  # misaligned
+    items[1].lg_blowup = 2
+    assert L.pl_batch_create(ctypes.cast(items, ctypes.c_void_p), 2, 1, ctypes.byref(h)) == -1
+    assert not h.value
+    assert L.pl_batch_info(None, None, None) == -1
+    assert L.pl_batch_run(None, None) == -1
+    L.pl_batch_destroy(None)
