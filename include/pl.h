@@ -201,6 +201,79 @@ int pl_batch_run_plain(pl_batch* b, pl_stream_t stream);
 int pl_batch_info(const pl_batch* b, int* n_launches, size_t* bytes);
 void pl_batch_destroy(pl_batch* b);
 
+/* ---- served form: a request loop for proof jobs ---------------------------
+ * (no counterpart in the reference, which runs one operator call at a time)
+ * A job is one proof's eight columns e_0..e_7 of N = 2^lg evaluations (device
+ * buffers the caller keeps valid until the job is done), beta, gamma and an
+ * opening point zeta (device elements).  It computes, bit-identical to the
+ * per-call entry points:
+ *   coeffs[j] = iNTT(e_j)                                     8 x N elements
+ *   lde[j]    = LDE(coeffs[j]) over 7^(2^lg_blowup) <w_M>     8 x M, M = N 2^lg_blowup
+ *   z         = pl_perm_grand_product(w = e_0..e_2, s = e_3..e_5, beta, gamma,
+ *               k = (1, 7, 13))                               N
+ *   y[j]      = coeffs[j](zeta)                               8
+ *   q[j]      = (coeffs[j] - y[j]) / (X - zeta), q[j][N-1] = 0  8 x N
+ * into a server-owned block that stays in HBM until pl_server_release.
+ * The server takes an arena of hbm_bytes at creation and admits queued jobs
+ * in arrival order when their output and work blocks fit in it.  Batched
+ * mode merges consecutive small jobs into one group, runs a group's
+ * transforms as ragged launches and overlaps up to `slots` groups on the
+ * device; plain mode runs one job at a time as per-call launch sequences.
+ * pl_server_wait blocks until the job is done and reports the server-clock
+ * times (seconds since creation) of its arrival (submit), admission and
+ * completion.  A job that can never fit fails with PL_ERR_NOMEM. */
+enum pl_serve_mode { PL_SERVE_BATCHED = 0, PL_SERVE_PLAIN = 1 };
+typedef struct pl_server_config {
+  size_t hbm_bytes;      /* arena size; 0 = free device memory less 24 GiB for domain tables */
+  int slots;             /* groups running at once, 1..8; 0 = 3 (plain: 1) */
+  int mode;              /* pl_serve_mode */
+  int max_group_jobs;    /* jobs merged into one group; 0 = 16 */
+  uint64_t group_elems;  /* merge while a group's 8 N sum stays <= this; 0 = 2^22 */
+} pl_server_config;
+typedef struct pl_job_desc {
+  int lg;         /* 1..27 */
+  int lg_blowup;  /* 1..4, lg + lg_blowup <= 28 */
+  const void* d_evals[8];
+  const void* d_beta;
+  const void* d_gamma;
+  const void* d_zeta;
+} pl_job_desc;
+typedef struct pl_job_result {
+  int status;
+  double arrival_s, admit_s, done_s;
+  const void* d_coeffs; /* [8][N] */
+  const void* d_lde;    /* [8][M] */
+  const void* d_z;      /* [N] */
+  const void* d_y;      /* [8] */
+  const void* d_q;      /* [8][N] */
+} pl_job_result;
+typedef struct pl_server_stats {
+  uint64_t submitted, completed, groups, queued;
+  size_t arena_bytes, arena_peak, arena_used, domain_bytes;
+} pl_server_stats;
+typedef struct pl_server pl_server;
+int pl_server_create(const pl_server_config* cfg, pl_server** out);
+int pl_server_submit(pl_server* s, const pl_job_desc* job, uint64_t* job_id);
+int pl_server_wait(pl_server* s, uint64_t job_id, pl_job_result* result);
+int pl_server_release(pl_server* s, uint64_t job_id);
+int pl_server_get_stats(pl_server* s, pl_server_stats* stats);
+double pl_server_now(const pl_server* s);
+/* output block bytes of one job (0 for unsupported sizes) */
+size_t pl_job_bytes(int lg, int lg_blowup);
+void pl_server_destroy(pl_server* s);
+
+/* ---- synthetic inputs and digests -----------------------------------------
+ * (test and benchmark infrastructure; no counterpart in the reference)
+ * pl_fill_uniform writes n elements uniform in [0, r) drawn from the seeded
+ * SplitMix64 stream (4 outputs per candidate as little-endian limbs, top bit
+ * cleared, rejected while >= r): the same values as the host generator on
+ * the same seed.  d_work: NULL or pl_uniform_work_bytes(n) bytes.
+ * pl_digest writes to d_out (8 bytes) sum_k (w_k ^ k*0x9E3779B97F4A7C15) *
+ * (2k+1) mod 2^64 over the 64-bit words w_k of n elements. */
+size_t pl_uniform_work_bytes(uint64_t n);
+int pl_fill_uniform(void* d_out, uint64_t n, uint64_t seed, void* d_work, pl_stream_t stream);
+int pl_digest(const void* d_data, uint64_t n, void* d_out, pl_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -55,7 +55,7 @@ int pl_domain_create(int lg, pl_domain** out) {
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return PL_ERR_NO_DEVICE;
   pl_domain* d = new (std::nothrow) pl_domain();
   if (!d) return PL_ERR_NOMEM;
-  hipGetDevice(&d->d.device);
+  (void)hipGetDevice(&d->d.device);
   hipError_t e = pl::domain_build(d->d, lg, nullptr);
   if (e != hipSuccess) {
     pl::domain_free(d->d);
@@ -111,7 +111,7 @@ int pl_ntt_batch(const void* d_in, void* d_out, int lg, uint32_t npoly, int inve
   c.npoly = npoly;
   c.inverse = inverse != 0;
   hipError_t e = pl::ntt_launch(d->d, c, s);
-  if (own) hipFreeAsync(tmp, s);
+  if (own) (void)hipFreeAsync(tmp, s);
   return e == hipSuccess ? PL_OK : hip_fail(e);
 }
 
@@ -317,6 +317,28 @@ int pl_perm_grand_product(const void* const* d_w, const void* const* d_s, const 
   int st = pl_perm_ratio(d_w, d_s, d_beta, d_gamma, h_k, lg, d, d_z, d_work, stream);
   if (st != PL_OK) return st;
   return pl_grand_product(d_z, 1ull << lg, nullptr, d_work, stream);
+}
+
+size_t pl_uniform_work_bytes(uint64_t n) { return (size_t)pl::uniform_work_bytes(n); }
+
+int pl_fill_uniform(void* d_out, uint64_t n, uint64_t seed, void* d_work, pl_stream_t stream) {
+  if ((!d_out && n) || n >= (1ull << 31) || !aligned16(d_out) || ((uintptr_t)d_work & 3)) return PL_ERR_ARG;
+  if (n == 0) return PL_OK;
+  hipStream_t s = (hipStream_t)stream;
+  void* w = d_work;
+  if (!w) {
+    hipError_t e = hipMallocAsync(&w, pl_uniform_work_bytes(n), s);
+    if (e != hipSuccess) return hip_fail(e);
+  }
+  hipError_t e = pl::fill_uniform((pl::fr*)d_out, n, seed, (uint32_t*)w, s);
+  if (!d_work) (void)hipFreeAsync(w, s);
+  return e == hipSuccess ? PL_OK : hip_fail(e);
+}
+
+int pl_digest(const void* d_data, uint64_t n, void* d_out, pl_stream_t stream) {
+  if ((!d_data && n) || !d_out || ((uintptr_t)d_data & 7) || ((uintptr_t)d_out & 7)) return PL_ERR_ARG;
+  hipError_t e = pl::digest((const pl::fr*)d_data, n, (uint64_t*)d_out, (hipStream_t)stream);
+  return e == hipSuccess ? PL_OK : hip_fail(e);
 }
 
 int pl_ntt_naive(const void* d_in, void* d_out, int lg, const pl_domain* d, pl_stream_t stream) {

@@ -114,7 +114,7 @@ hipError_t fill_pow_table(fr* out, uint64_t n, const host::fr64& base, const hos
   if (e != hipSuccess) return e;
   e = fill_pow(out, n, 0, base, &scale, win, s);
   hipError_t e2 = hipStreamSynchronize(s);
-  hipFree(win);
+  (void)hipFree(win);
   return e != hipSuccess ? e : e2;
 }
 
@@ -162,7 +162,7 @@ hipError_t domain_build(Domain& d, int lg, hipStream_t s) {
   }
   e = hipStreamSynchronize(s);
 done:
-  hipFree(win);
+  (void)hipFree(win);
   return e;
 }
 
@@ -185,7 +185,7 @@ hipError_t lde_domain_build(LdeDomain& d, int lg_n, int lg_blowup, const host::f
     st = host::mul(st, wM);
   }
   if (e == hipSuccess) e = hipStreamSynchronize(s);
-  hipFree(win);
+  (void)hipFree(win);
   return e;
 }
 
@@ -205,12 +205,12 @@ hipError_t params_fill(fr* out, hipStream_t s) {
 
 void lde_domain_free(LdeDomain& d) {
   domain_free(d.ntt);
-  if (d.cp) hipFree(d.cp);
+  if (d.cp) (void)hipFree(d.cp);
   d.cp = nullptr;
 }
 
 void domain_free(Domain& d) {
-  if (d.block) hipFree(d.block);
+  if (d.block) (void)hipFree(d.block);
   d.block = nullptr;
 }
 

@@ -317,17 +317,24 @@ __global__ void __launch_bounds__(kThreads, PL_NTT_WAVES) ntt_pass_kernel(PassAr
 }
 
 // Ragged form: one launch runs the tiles of many polynomials of different
-// sizes (one PassArgs per segment).  block_seg maps a block to its segment,
-// seg_block0 gives the segment's first block.
+// sizes (one PassArgs per segment).  seg_block0[s] is segment s's first
+// block, ascending; a block finds its segment by binary search (uniform
+// scalar loads, a few steps for the tens of segments of a batch).
 template <int LA, bool INV, bool LAST>
 __global__ void __launch_bounds__(kThreads, PL_NTT_WAVES)
-    ntt_pass_ragged_kernel(const PassArgs* __restrict__ segs, const uint32_t* __restrict__ block_seg,
-                           const uint32_t* __restrict__ seg_block0) {
-  const uint32_t seg = block_seg[blockIdx.x];
-  // read the arguments in place (uniform loads); a private copy would put
-  // the dynamically indexed lgrad[] in scratch
-  const PassArgs& a = segs[seg];
-  ntt_pass_body<LA, INV, LAST>(a, blockIdx.x - seg_block0[seg]);
+    ntt_pass_ragged_kernel(const PassArgs* __restrict__ segs, const uint32_t* __restrict__ seg_block0,
+                           uint32_t nsegs) {
+  const uint32_t b = blockIdx.x;
+  uint32_t lo = 0, hi = nsegs - 1;
+  while (lo < hi) {
+    const uint32_t mid = (lo + hi + 1) >> 1;
+    if (seg_block0[mid] <= b) lo = mid;
+    else hi = mid - 1;
+  }
+  // read the arguments in place; a private copy would put the dynamically
+  // indexed lgrad[] in scratch
+  const PassArgs& a = segs[lo];
+  ntt_pass_body<LA, INV, LAST>(a, b - seg_block0[lo]);
 }
 
 // Host launcher for one radix: LA = log2(column size) - 4.
@@ -339,10 +346,10 @@ static void launch_pass(const PassArgs& a, hipStream_t s) {
 }
 
 template <int LA, bool INV, bool LAST>
-static void launch_pass_ragged(const PassArgs* segs, const uint32_t* block_seg, const uint32_t* seg_block0,
-                               uint32_t nblocks, hipStream_t s) {
-  hipLaunchKernelGGL((ntt_pass_ragged_kernel<LA, INV, LAST>), dim3(nblocks), dim3(kThreads), 0, s, segs, block_seg,
-                     seg_block0);
+static void launch_pass_ragged(const PassArgs* segs, const uint32_t* seg_block0, uint32_t nsegs, uint32_t nblocks,
+                               hipStream_t s) {
+  hipLaunchKernelGGL((ntt_pass_ragged_kernel<LA, INV, LAST>), dim3(nblocks), dim3(kThreads), 0, s, segs, seg_block0,
+                     nsegs);
 }
 
 #define PL_DEFINE_PASS_LAUNCHER(LA)                                        \
@@ -355,20 +362,20 @@ static void launch_pass_ragged(const PassArgs* segs, const uint32_t* block_seg, 
       else launch_pass<LA, false, false>(a, s);                            \
     }                                                                      \
   }                                                                        \
-  void launch_pass_ragged_la##LA(bool inv, bool last, const PassArgs* segs, const uint32_t* bs, \
-                                 const uint32_t* sb, uint32_t nb, hipStream_t s) {            \
+  void launch_pass_ragged_la##LA(bool inv, bool last, const PassArgs* segs, const uint32_t* sb, \
+                                 uint32_t ns, uint32_t nb, hipStream_t s) {                   \
     if (inv) {                                                             \
-      if (last) launch_pass_ragged<LA, true, true>(segs, bs, sb, nb, s);   \
-      else launch_pass_ragged<LA, true, false>(segs, bs, sb, nb, s);       \
+      if (last) launch_pass_ragged<LA, true, true>(segs, sb, ns, nb, s);   \
+      else launch_pass_ragged<LA, true, false>(segs, sb, ns, nb, s);       \
     } else {                                                               \
-      if (last) launch_pass_ragged<LA, false, true>(segs, bs, sb, nb, s);  \
-      else launch_pass_ragged<LA, false, false>(segs, bs, sb, nb, s);      \
+      if (last) launch_pass_ragged<LA, false, true>(segs, sb, ns, nb, s);  \
+      else launch_pass_ragged<LA, false, false>(segs, sb, ns, nb, s);      \
     }                                                                      \
   }
 
 #define PL_DECLARE_RAGGED(LA)                                                                  \
-  void launch_pass_ragged_la##LA(bool inv, bool last, const PassArgs* segs, const uint32_t* bs, \
-                                 const uint32_t* sb, uint32_t nb, hipStream_t s);
+  void launch_pass_ragged_la##LA(bool inv, bool last, const PassArgs* segs, const uint32_t* sb, \
+                                 uint32_t ns, uint32_t nb, hipStream_t s);
 // Kernel arguments of pass p of the call c when every pass is a regular
 // tile pass (false for the tiny and single-pass premultiplied forms, which
 // ntt_launch handles with extra kernels).

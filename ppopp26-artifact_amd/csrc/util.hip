@@ -1,0 +1,168 @@
+// Synthetic inputs and output digests on the device.
+//
+// fill_uniform reproduces the seeded host stream exactly: candidate j is the
+// SplitMix64 outputs number 4j+1 .. 4j+4 of the seed (little-endian limbs,
+// top bit of the highest limb cleared), accepted when < r; the accepted
+// candidates in order are the output.  SplitMix64's k-th output is a pure
+// function of seed + k * golden, so candidates are evaluated in parallel and
+// the acceptances compacted with a block count / scan / write sequence.
+// n + n/8 + 4096 candidates are evaluated: with acceptance probability
+// r / 2^255 ~ 0.906 the expected surplus is >= 0.019 n + 3700, many standard
+// deviations (0.3 sqrt(n)) above zero for every n.
+//
+// digest folds a vector into 64 bits, position-weighted so that permuted,
+// shifted or changed elements change it: sum_k (w_k ^ (k * golden)) * (2k+1)
+// mod 2^64 over the 64-bit words w_k.  Integer sums are order-independent,
+// so the value is deterministic.
+#include "pl_internal.hpp"
+
+namespace pl {
+
+constexpr int kUniThreads = 256;
+constexpr int kUniPer = 8;  // candidates per thread
+constexpr uint64_t kUniBlock = (uint64_t)kUniThreads * kUniPer;
+constexpr uint64_t kGolden = 0x9E3779B97F4A7C15ull;
+
+__device__ __forceinline__ uint64_t splitmix_at(uint64_t seed, uint64_t k) {
+  uint64_t z = seed + k * kGolden;
+  z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+  z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+  return z ^ (z >> 31);
+}
+
+__device__ __forceinline__ bool candidate(uint64_t seed, uint64_t j, uint64_t (&l)[4]) {
+  for (int i = 0; i < 4; i++) l[i] = splitmix_at(seed, 4 * j + 1 + i);
+  l[3] &= 0x7fffffffffffffffull;
+  const uint64_t r[4] = {0xffffffff00000001ull, 0x53bda402fffe5bfeull, 0x3339d80809a1d805ull, 0x73eda753299d7d48ull};
+  for (int i = 3; i >= 0; i--) {
+    if (l[i] < r[i]) return true;
+    if (l[i] > r[i]) return false;
+  }
+  return false;  // == r
+}
+
+// per block: number of accepted candidates
+__global__ void __launch_bounds__(kUniThreads) uni_count_kernel(uint64_t seed, uint64_t ncand, uint32_t* counts) {
+  __shared__ uint32_t red[kUniThreads / 64];
+  const uint64_t j0 = blockIdx.x * kUniBlock + (uint64_t)threadIdx.x * kUniPer;
+  uint32_t c = 0;
+  for (int i = 0; i < kUniPer; i++) {
+    uint64_t l[4];
+    if (j0 + i < ncand && candidate(seed, j0 + i, l)) c++;
+  }
+  for (int o = 32; o > 0; o >>= 1) c += __shfl_down(c, o, 64);
+  if ((threadIdx.x & 63) == 0) red[threadIdx.x >> 6] = c;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint32_t t = 0;
+    for (int w = 0; w < kUniThreads / 64; w++) t += red[w];
+    counts[blockIdx.x] = t;
+  }
+}
+
+// exclusive scan of the block counts in place (one block)
+__global__ void __launch_bounds__(1024) uni_scan_kernel(uint32_t* counts, uint32_t nb) {
+  __shared__ uint64_t part[1024];
+  const uint32_t per = (nb + 1023) / 1024;
+  const uint32_t b0 = threadIdx.x * per;
+  uint64_t s = 0;
+  for (uint32_t i = b0; i < b0 + per && i < nb; i++) s += counts[i];
+  part[threadIdx.x] = s;
+  __syncthreads();
+  if (threadIdx.x == 0) {
+    uint64_t run = 0;
+    for (int t = 0; t < 1024; t++) {
+      uint64_t v = part[t];
+      part[t] = run;
+      run += v;
+    }
+  }
+  __syncthreads();
+  uint64_t run = part[threadIdx.x];
+  for (uint32_t i = b0; i < b0 + per && i < nb; i++) {
+    uint32_t v = counts[i];
+    counts[i] = (uint32_t)run;  // n < 2^32 elements
+    run += v;
+  }
+}
+
+__global__ void __launch_bounds__(kUniThreads) uni_write_kernel(uint64_t seed, uint64_t ncand, const uint32_t* base,
+                                                                uint64_t* out, uint64_t n) {
+  __shared__ uint32_t wsum[kUniThreads / 64];
+  const uint64_t j0 = blockIdx.x * kUniBlock + (uint64_t)threadIdx.x * kUniPer;
+  uint32_t acc = 0;
+  uint64_t keep[kUniPer][4];
+  bool ok[kUniPer];
+  for (int i = 0; i < kUniPer; i++) {
+    ok[i] = j0 + i < ncand && candidate(seed, j0 + i, keep[i]);
+    acc += ok[i] ? 1 : 0;
+  }
+  // exclusive prefix of acc over the block: wave inclusive scan, then waves
+  const int lane = threadIdx.x & 63, wv = threadIdx.x >> 6;
+  uint32_t inc = acc;
+  for (int o = 1; o < 64; o <<= 1) {
+    uint32_t v = __shfl_up(inc, o, 64);
+    if (lane >= o) inc += v;
+  }
+  if (lane == 63) wsum[wv] = inc;
+  __syncthreads();
+  uint32_t before = 0;
+  for (int w = 0; w < wv; w++) before += wsum[w];
+  uint64_t pos = (uint64_t)base[blockIdx.x] + before + inc - acc;
+  for (int i = 0; i < kUniPer; i++) {
+    if (!ok[i]) continue;
+    if (pos < n) {
+      uint64_t* o = out + 4 * pos;
+      o[0] = keep[i][0];
+      o[1] = keep[i][1];
+      o[2] = keep[i][2];
+      o[3] = keep[i][3];
+    }
+    pos++;
+  }
+}
+
+uint64_t uniform_work_bytes(uint64_t n) {
+  const uint64_t ncand = n + n / 8 + 4096;
+  return ((ncand + kUniBlock - 1) / kUniBlock) * sizeof(uint32_t);
+}
+
+hipError_t fill_uniform(fr* out, uint64_t n, uint64_t seed, uint32_t* work, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const uint64_t ncand = n + n / 8 + 4096;
+  const uint32_t nb = (uint32_t)((ncand + kUniBlock - 1) / kUniBlock);
+  hipLaunchKernelGGL(uni_count_kernel, dim3(nb), dim3(kUniThreads), 0, s, seed, ncand, work);
+  hipLaunchKernelGGL(uni_scan_kernel, dim3(1), dim3(1024), 0, s, work, nb);
+  hipLaunchKernelGGL(uni_write_kernel, dim3(nb), dim3(kUniThreads), 0, s, seed, ncand, (const uint32_t*)work,
+                     (uint64_t*)out, n);
+  return hipGetLastError();
+}
+
+// ---- digest -----------------------------------------------------------------
+
+__global__ void digest_zero_kernel(unsigned long long* out) {
+  if (threadIdx.x == 0) *out = 0;
+}
+
+__global__ void __launch_bounds__(256) digest_kernel(const uint64_t* w, uint64_t nw, unsigned long long* out) {
+  uint64_t acc = 0;
+  const uint64_t stride = (uint64_t)gridDim.x * blockDim.x;
+  for (uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x; k < nw; k += stride)
+    acc += (w[k] ^ (k * kGolden)) * (2 * k + 1);
+  for (int o = 32; o > 0; o >>= 1) acc += __shfl_down(acc, o, 64);
+  if ((threadIdx.x & 63) == 0) atomicAdd(out, (unsigned long long)acc);
+}
+
+hipError_t digest(const fr* d, uint64_t n, uint64_t* out, hipStream_t s) {
+  hipLaunchKernelGGL(digest_zero_kernel, dim3(1), dim3(64), 0, s, (unsigned long long*)out);
+  if (n) {
+    const uint64_t nw = 4 * n;
+    uint64_t blocks = (nw + 255) / 256;
+    if (blocks > 4096) blocks = 4096;
+    hipLaunchKernelGGL(digest_kernel, dim3((unsigned)blocks), dim3(256), 0, s, (const uint64_t*)d, nw,
+                       (unsigned long long*)out);
+  }
+  return hipGetLastError();
+}
+
+}  // namespace pl

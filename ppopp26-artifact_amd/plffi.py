@@ -67,7 +67,46 @@ SIGNATURES = {
     "pl_batch_run_plain": (_i, [_vp, _vp]),
     "pl_batch_info": (_i, [_vp, ctypes.POINTER(_i), ctypes.POINTER(ctypes.c_size_t)]),
     "pl_batch_destroy": (None, [_vp]),
+    "pl_server_create": (_i, [_vp, ctypes.POINTER(_vp)]),
+    "pl_server_submit": (_i, [_vp, _vp, ctypes.POINTER(_u64)]),
+    "pl_server_wait": (_i, [_vp, _u64, _vp]),
+    "pl_server_release": (_i, [_vp, _u64]),
+    "pl_server_get_stats": (_i, [_vp, _vp]),
+    "pl_server_now": (ctypes.c_double, [_vp]),
+    "pl_job_bytes": (ctypes.c_size_t, [_i, _i]),
+    "pl_server_destroy": (None, [_vp]),
+    "pl_uniform_work_bytes": (ctypes.c_size_t, [_u64]),
+    "pl_fill_uniform": (_i, [_vp, _u64, _u64, _vp, _vp]),
+    "pl_digest": (_i, [_vp, _u64, _vp, _vp]),
 }
+
+SERVE_BATCHED, SERVE_PLAIN = 0, 1
+
+
+class ServerConfig(ctypes.Structure):
+    """pl_server_config"""
+    _fields_ = [("hbm_bytes", ctypes.c_size_t), ("slots", _i), ("mode", _i), ("max_group_jobs", _i),
+                ("group_elems", _u64)]
+
+
+class JobDesc(ctypes.Structure):
+    """pl_job_desc"""
+    _fields_ = [("lg", _i), ("lg_blowup", _i), ("d_evals", _vp * 8), ("d_beta", _vp), ("d_gamma", _vp),
+                ("d_zeta", _vp)]
+
+
+class JobResult(ctypes.Structure):
+    """pl_job_result"""
+    _fields_ = [("status", _i), ("arrival_s", ctypes.c_double), ("admit_s", ctypes.c_double),
+                ("done_s", ctypes.c_double), ("d_coeffs", _vp), ("d_lde", _vp), ("d_z", _vp), ("d_y", _vp),
+                ("d_q", _vp)]
+
+
+class ServerStats(ctypes.Structure):
+    """pl_server_stats"""
+    _fields_ = [("submitted", _u64), ("completed", _u64), ("groups", _u64), ("queued", _u64),
+                ("arena_bytes", ctypes.c_size_t), ("arena_peak", ctypes.c_size_t),
+                ("arena_used", ctypes.c_size_t), ("domain_bytes", ctypes.c_size_t)]
 
 BATCH_OPS = {"ntt": 0, "intt": 1, "lde": 2, "eval_div": 3}
 PL_BATCH_GRAPH = 1
@@ -296,6 +335,92 @@ class Batch:
             self.close()
         except Exception:
             pass
+
+
+class Server:
+    """Request loop for proof jobs (pl_server_*).  Inputs are device tensors
+    kept referenced until the job is waited for."""
+
+    def __init__(self, hbm_bytes=0, slots=0, mode=SERVE_BATCHED, max_group_jobs=0, group_elems=0):
+        cfg = ServerConfig(hbm_bytes, slots, mode, max_group_jobs, group_elems)
+        self.h = ctypes.c_void_p()
+        self._keep = {}
+        check("pl_server_create", lib().pl_server_create(ctypes.byref(cfg), ctypes.byref(self.h)))
+
+    def submit(self, lg, lg_blowup, evals, beta, gamma, zeta):
+        d = JobDesc()
+        d.lg, d.lg_blowup = lg, lg_blowup
+        for c in range(8):
+            d.d_evals[c] = evals[c].data_ptr()
+        d.d_beta, d.d_gamma, d.d_zeta = beta.data_ptr(), gamma.data_ptr(), zeta.data_ptr()
+        jid = ctypes.c_uint64()
+        check("pl_server_submit", lib().pl_server_submit(self.h, ctypes.byref(d), ctypes.byref(jid)))
+        self._keep[jid.value] = (evals, beta, gamma, zeta)
+        return jid.value
+
+    def wait(self, jid):
+        r = JobResult()
+        st = lib().pl_server_wait(self.h, jid, ctypes.byref(r))
+        self._keep.pop(jid, None)
+        if st != PL_OK:
+            raise PlError("pl_server_wait", st)
+        return r
+
+    def release(self, jid):
+        check("pl_server_release", lib().pl_server_release(self.h, jid))
+
+    def now(self):
+        return lib().pl_server_now(self.h)
+
+    def stats(self):
+        st = ServerStats()
+        check("pl_server_get_stats", lib().pl_server_get_stats(self.h, ctypes.byref(st)))
+        return {k: getattr(st, k) for k, _ in ServerStats._fields_}
+
+    def close(self):
+        if self.h:
+            lib().pl_server_destroy(self.h)
+            self.h = ctypes.c_void_p()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def job_bytes(lg, lg_blowup):
+    return lib().pl_job_bytes(lg, lg_blowup)
+
+
+def fill_uniform(out, seed, stream=None):
+    """out: device tensor (n, 4) int64; the seeded stream of synth.uniform."""
+    check("pl_fill_uniform", lib().pl_fill_uniform(_ptr(out), out.shape[0], seed & 0xFFFFFFFFFFFFFFFF, None,
+                                                   _stream(stream)))
+    return out
+
+
+def digest_ptr(ptr, n, stream=None):
+    """64-bit digest of n elements at a device address (see synth.digest)."""
+    import torch
+    o = torch.zeros(1, dtype=torch.int64, device="cuda")
+    check("pl_digest", lib().pl_digest(ctypes.c_void_p(ptr), n, _ptr(o), _stream(stream)))
+    return int(o.cpu().numpy().view("uint64")[0])
+
+
+def digest(t, stream=None):
+    return digest_ptr(t.data_ptr(), t.shape[0], stream)
+
+
+def copy_from_ptr(ptr, n, stream=None):
+    """Device copy of n elements at a raw device address into a new tensor
+    (out = a + 0 through the element-wise operator)."""
+    import torch
+    out = torch.empty((n, 4), dtype=torch.int64, device="cuda")
+    zero = torch.zeros((1, 4), dtype=torch.int64, device="cuda")
+    check("pl_vec_op", lib().pl_vec_op(VEC_OPS["add_scalar"], ctypes.c_void_p(ptr), _ptr(zero), _ptr(out), n, None,
+                                       _stream(stream)))
+    return out
 
 
 def ntt_pass(x, out, dom, p, npoly=1, inverse=False, stream=None):

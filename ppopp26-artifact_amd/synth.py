@@ -67,6 +67,28 @@ def uniform(n, s, chunk=1 << 20):
     return out
 
 
+def digest(a):
+    """The device digest (pl_digest) on the host: sum_k (w_k ^ k*golden) *
+    (2k+1) mod 2^64 over the 64-bit words of a (n, 4) uint64 array."""
+    w = np.ascontiguousarray(a, dtype=np.uint64).ravel()
+    k = np.arange(len(w), dtype=np.uint64)
+    with np.errstate(over="ignore"):
+        t = (w ^ (k * _GOLDEN)) * (np.uint64(2) * k + np.uint64(1))
+        return int(np.sum(t, dtype=np.uint64))
+
+
+def served_jobs(rate_index, n_jobs, lg_lo=14, lg_hi=24):
+    """[(lg, lg_blowup, t_arrival_unit)] for the served workload: lg uniform
+    in lg_lo..lg_hi, lg_blowup 2 or 4, Poisson arrivals with unit rate (scale
+    the times by 1/rate)."""
+    rng = np.random.default_rng(seed(OP_SERVE, 0, rate_index))
+    lgs = rng.integers(lg_lo, lg_hi + 1, size=n_jobs)
+    lbs = np.where(rng.random(n_jobs) < 0.5, 2, 4)
+    t = np.cumsum(rng.exponential(1.0, size=n_jobs))
+    t -= t[0]
+    return [(int(a), int(b), float(c)) for a, b, c in zip(lgs, lbs, t)]
+
+
 def ragged_batch(s=None):
     """[(op, lg)] for the ragged proof batch, op in ntt / lde / eval_div."""
     rng = np.random.default_rng(seed(OP_BATCH, 0) if s is None else s)
