@@ -1,32 +1,48 @@
 #!/usr/bin/env python3
-"""Benchmark of the hot path: device-resident NTT of 2^24 BLS12-381 Fr
-elements (the reference's `NTT --bench-split 24` workload), plus the coset
-LDE 2^24 -> 2^26 reported beside it.
+"""Benchmark of the hot path on one MI355X.
 
-    python bench.py [--gpus 1] [--steps K] [--warmup W] [--lg 24] [--no-cpu]
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--lg 24]
+                    [--no-cpu] [--no-lde] [--no-batch] [--no-serve] [--serve-jobs J]
 
-Prints ONE JSON line.  `value` = Fr elements transformed per second with the
-input already in HBM (K back-to-back NTTs bracketed by HIP events on the
-launch stream).  `roofline` is for the dominant kernel (the intermediate NTT
-pass), timed per launch with HIP events on its stream; `cpu_baseline` is the
-oracle's radix-2 NTT on one host core over the same 2^24 workload.
+Prints ONE JSON line.  `value` = Fr elements per second of the device-resident
+NTT of 2^24 elements (the reference's `NTT --bench-split 24` workload with the
+input already in HBM): K back-to-back NTTs bracketed by HIP events on the
+launch stream.  Beside it, in the same line:
+
+  lde      coset LDE 2^24 -> 2^26 (lambda = 2, shift 7^4), device-resident
+  batched  the ragged proof batch (64 polynomials, lg 12..20: 40 NTT, 16 LDE
+           lambda = 2, 8 evaluation + division), one hipGraph launch versus one
+           launch sequence per polynomial; per-run latency p50 / p99
+  served   proof jobs (8 columns of 2^lg, lg 14..24, lambda 2 or 4: iNTT ->
+           LDE -> permutation grand product -> evaluation -> division) through
+           the request loop, batched versus plain form: throughput with every
+           job queued at once, and p50 / p99 latency under Poisson arrivals at
+           three rates; outputs compared by digest between the two forms
+  roofline the dominant kernel (NTT pass 0), per-launch HIP events on its
+           stream, against the measured v_mad_u64_u32 peak
+  cpu_baseline  the oracle's radix-2 NTT 2^24 on one host core
 """
 import argparse
 import json
 import os
 import platform
+import queue
 import sys
+import threading
 import time
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, os.path.join(ROOT, "ppopp26-artifact_amd"))
-sys.path.insert(0, os.path.join(ROOT, "oracle"))
 
-# MI355X: 256 CUs x 64 lanes x 2.4 GHz peak engine clock; v_mad_u64_u32 issues
-# at the full VALU rate on gfx950 (one 32x32->64 multiply-add per lane-cycle)
-PEAK_MAD = 256 * 64 * 2.4e9
+# MI355X: 256 CUs x 64 lanes x 2.4 GHz peak engine clock; v_mad_u64_u32 is a
+# full-rate VALU op on gfx950 (one 32x32->64 multiply-add per lane-cycle)
+PEAK_MAD_SPEC = 256 * 64 * 2.4e9
 PEAK_HBM = 8.0e12
 MAD_PER_MULMOD = 128  # 8-limb Montgomery product: 2 * 8^2 limb multiply-adds
+# HBM bytes per launch of the dominant kernel (NTT pass 0 at 2^24), from
+# rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs scaled by a plain copy of
+# known size (profiles/ntt_pass_2p24_hbm_traffic.txt); None when not profiled
+TRAFFIC_PASS0_2P24 = None
 
 
 def parse():
@@ -37,6 +53,9 @@ def parse():
     ap.add_argument("--lg", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-lde", action="store_true")
+    ap.add_argument("--no-batch", action="store_true")
+    ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--serve-jobs", type=int, default=40, help="jobs per served run")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain device copy of 2^lg * 32 bytes (HBM counter calibration)")
     return ap.parse_args()
@@ -54,23 +73,302 @@ def time_events(fn, reps, stream):
     return e0.elapsed_time(e1) / reps  # ms
 
 
+def time_each(fn, reps, stream):
+    """per-run milliseconds (event pair around each run)"""
+    import torch
+    out = []
+    for _ in range(reps):
+        e0 = torch.cuda.Event(enable_timing=True)
+        e1 = torch.cuda.Event(enable_timing=True)
+        e0.record(stream)
+        fn()
+        e1.record(stream)
+        e1.synchronize()
+        out.append(e0.elapsed_time(e1))
+    return out
+
+
+def progress(msg):
+    print("[bench] " + msg, file=sys.stderr, flush=True)
+
+
+def pct(v, q):
+    import numpy as np
+    return float(np.percentile(np.asarray(v), q)) if len(v) else None
+
+
+def dev_uniform(pl, n, seed):
+    import torch
+    t = torch.empty((n, 4), dtype=torch.int64, device="cuda")
+    return pl.fill_uniform(t, seed)
+
+
+# ---------------------------------------------------------------------------
+def bench_ntt(pl, synth, lg, steps, warmup, stream, peak):
+    import torch
+    n = 1 << lg
+    dom = pl.Domain(lg)
+    x = dev_uniform(pl, n, synth.seed(synth.OP_NTT, lg))
+    y = torch.empty_like(x)
+    work = torch.empty_like(x)
+    # correctness gate on the measured buffers: iNTT(NTT(x)) == x
+    pl.ntt(x, dom, out=y, work=work)
+    z = pl.ntt(y, dom, inverse=True, work=work)
+    torch.cuda.synchronize()
+    verified = bool(torch.equal(z, x))
+    del z
+    step = lambda: pl.ntt(x, dom, out=y, work=work)  # noqa: E731
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    ms = time_events(step, steps, stream)
+    npass = pl.ntt_num_passes(lg)
+    pass_ms = []
+    for p in range(npass):
+        src = x if p == 0 else work
+        dst = work if p < npass - 1 else y
+        for _ in range(2):
+            pl.ntt_pass(src, dst, dom, p)
+        pass_ms.append(time_events(lambda: pl.ntt_pass(src, dst, dom, p), steps, stream))
+    # the library's plan: ceil(lg/8) passes whose sizes differ by <= 1, larger first
+    base, extra = lg // npass, lg % npass
+    m_pass = [base + (1 if p < extra else 0) for p in range(npass)]
+    dom_ms = pass_ms[0]
+    mad = (n // 2) * m_pass[0] * MAD_PER_MULMOD
+    achieved = mad / (dom_ms * 1e-3)
+    byts = 64 * n
+    roof = {
+        "bound": "valu",
+        "achieved": round(achieved / 1e12, 3),
+        "peak": round(peak / 1e12, 3),
+        "unit": "Tmad/s",
+        "frac": round(achieved / peak, 4),
+        "traffic": TRAFFIC_PASS0_2P24 if lg == 24 else None,
+        "peak_source": "measured v_mad_u64_u32 rate (pl_probe_mad_rate); spec full rate %.2f" % (
+            PEAK_MAD_SPEC / 1e12),
+        "kernel": "ntt_pass_kernel<%d,false,false> (pass 0 of %d, 2^%d-point columns)" % (m_pass[0] - 4, npass,
+                                                                                          m_pass[0]),
+        "alg_per_launch": "%d mulmod = (N/2)*%d, x128 u32 multiply-adds; %d bytes = 64 N" % (
+            (n // 2) * m_pass[0], m_pass[0], byts),
+        "avg_launch_ms": round(dom_ms, 4),
+        "pass_ms": [round(v, 4) for v in pass_ms],
+        "hbm_gbps_alg": round(byts / (dom_ms * 1e-3) / 1e9, 1),
+        "hbm_frac_alg": round(byts / (dom_ms * 1e-3) / PEAK_HBM, 4),
+        "whole_ntt_frac": round(((n // 2) * lg * MAD_PER_MULMOD / peak) / (ms * 1e-3), 4),
+    }
+    del x, y, work
+    return {"value": n / (ms * 1e-3), "ms": ms, "verified": verified, "npass": npass}, roof
+
+
+def bench_lde(pl, synth, lg, lb, steps, warmup, stream, peak):
+    import torch
+    n, m = 1 << lg, 1 << (lg + lb)
+    dom = pl.LdeDomain(lg, lb)
+    c = dev_uniform(pl, n, synth.seed(synth.OP_LDE, lg, lb))
+    out = torch.empty((m, 4), dtype=torch.int64, device="cuda")
+    work = torch.empty((m, 4), dtype=torch.int64, device="cuda")
+    step = lambda: pl.lde(c, dom, out=out, work=work)  # noqa: E731
+    for _ in range(warmup):
+        step()
+    torch.cuda.synchronize()
+    ms = time_events(step, steps, stream)
+    mulmod = (m // 2) * lg + m
+    res = {
+        "workload": "coset LDE 2^%d -> 2^%d (lambda=%d, shift 7^%d), device-resident" % (lg, lg + lb, lb, 1 << lb),
+        "ms": round(ms, 4),
+        "value": round(n / (ms * 1e-3), 1),
+        "unit": "Fr elems/s (input N)",
+        "output_elems_per_s": round(m / (ms * 1e-3), 1),
+        "valu_frac": round((mulmod * MAD_PER_MULMOD / peak) / (ms * 1e-3), 4),
+        "alg": "%d mulmod = (M/2)*%d + M" % (mulmod, lg),
+    }
+    del c, out, work
+    return res
+
+
+def bench_batch(pl, synth, steps, stream):
+    import torch
+    spec = [(op, lg, 2 if op == "lde" else 0) for op, lg in synth.ragged_batch()]
+    items, n_in, n_out = [], 0, 0
+    for i, (op, lg, lb) in enumerate(spec):
+        n = 1 << lg
+        it = {"op": op, "lg": lg, "lg_blowup": lb, "inp": dev_uniform(pl, n, synth.seed(synth.OP_BATCH, lg, i))}
+        it["out"] = torch.empty(((n << lb), 4), dtype=torch.int64, device="cuda")
+        n_in += n
+        n_out += n << lb
+        if op == "eval_div":
+            it["z"] = dev_uniform(pl, 1, synth.seed(synth.OP_BATCH, lg, 1000 + i))
+            it["y"] = torch.empty((1, 4), dtype=torch.int64, device="cuda")
+        items.append(it)
+    b = pl.Batch(items, graph=True)
+    nk, nbytes = b.info()
+    # bit-exactness gate: batched == per-polynomial form, by digest
+    b.run()
+    torch.cuda.synchronize()
+    d_b = [pl.digest(it["out"]) for it in items]
+    b.run_plain()
+    torch.cuda.synchronize()
+    d_p = [pl.digest(it["out"]) for it in items]
+    for _ in range(3):
+        b.run()
+        b.run_plain()
+    torch.cuda.synchronize()
+    t_b = time_each(b.run, steps, stream)
+    t_p = time_each(b.run_plain, steps, stream)
+    mb, mp = pct(t_b, 50), pct(t_p, 50)
+    b.close()
+    return {
+        "workload": "ragged proof batch: 64 polys lg 12..20 (40 NTT, 16 LDE lambda=2, 8 eval+div)",
+        "value": round(n_in / (mb * 1e-3), 1),
+        "unit": "Fr elems/s (inputs)",
+        "plain_value": round(n_in / (mp * 1e-3), 1),
+        "speedup_vs_plain": round(mp / mb, 3),
+        "p50_ms": round(mb, 4), "p99_ms": round(pct(t_b, 99), 4),
+        "plain_p50_ms": round(mp, 4), "plain_p99_ms": round(pct(t_p, 99), 4),
+        "input_elems": n_in, "output_elems": n_out,
+        "graph_kernel_nodes": nk, "plan_bytes": nbytes,
+        "bit_exact_vs_plain": d_b == d_p,
+    }
+
+
+def run_server(pl, mode, jobs, times, pool, scal, verify):
+    """Submit jobs at their arrival times (seconds, None = all at once);
+    returns per-job (arrival, admit, done) and digests."""
+    import torch
+    srv = pl.Server(mode=mode)
+    res = [None] * len(jobs)
+    q = queue.Queue()
+    err = []
+
+    def releaser():
+        s = torch.cuda.Stream()
+        try:
+            for i in range(len(jobs)):
+                jid = q.get()
+                r = srv.wait(jid)
+                d = None
+                if verify:
+                    lg, lb = jobs[i]
+                    n, m = 1 << lg, 1 << (lg + lb)
+                    d = (pl.digest_ptr(r.d_coeffs, 8 * n, s), pl.digest_ptr(r.d_lde, 8 * m, s),
+                         pl.digest_ptr(r.d_z, n, s), pl.digest_ptr(r.d_y, 8, s), pl.digest_ptr(r.d_q, 8 * n, s))
+                res[i] = (r.arrival_s, r.admit_s, r.done_s, d)
+                srv.release(jid)
+        except Exception as e:  # surfaced by the caller
+            err.append(e)
+
+    th = threading.Thread(target=releaser)
+    th.start()
+    t0 = srv.now()
+    for i, (lg, lb) in enumerate(jobs):
+        if times is not None:
+            while True:
+                d = t0 + times[i] - srv.now()
+                if d <= 0:
+                    break
+                time.sleep(min(d, 0.001))
+        beta, gamma, zeta = scal[i]
+        q.put(srv.submit(lg, lb, pool[lg], beta, gamma, zeta))
+    th.join()
+    st = srv.stats()
+    srv.close()
+    if err:
+        raise err[0]
+    return res, st
+
+
+def summarize(jobs, res):
+    lat = [(r[2] - r[0]) * 1e3 for r in res]
+    span = max(r[2] for r in res) - min(r[0] for r in res)
+    elems = sum(8 << lg for lg, _ in jobs)
+    return {"p50_ms": round(pct(lat, 50), 2), "p99_ms": round(pct(lat, 99), 2),
+            "elems_per_s": round(elems / span, 1), "span_s": round(span, 3)}
+
+
+def bench_served(pl, synth, n_jobs):
+    import torch
+    # resident inputs: 8 columns per size (jobs of one size share columns,
+    # each job has its own beta, gamma, zeta)
+    pool = {lg: [dev_uniform(pl, 1 << lg, synth.seed(synth.OP_SERVE, lg, c)) for c in range(8)]
+            for lg in range(14, 25)}
+    rates_rel = [0.5, 0.8, 1.1]
+    wl = [synth.served_jobs(ri, n_jobs) for ri in range(3)]
+    scal = [[tuple(dev_uniform(pl, 1, synth.seed(synth.OP_SERVE, 0, 10000 * (ri + 1) + 3 * i + k))
+                   for k in range(3)) for i in range(n_jobs)] for ri in range(3)]
+    # warm-up: one job of every size and blowup (domains and coset tables are
+    # built on first use and stay resident)
+    warm = sorted(set((lg, lb) for w in wl for lg, lb, _ in w))
+    run_server(pl, pl.SERVE_BATCHED, warm, None, pool, [scal[0][0]] * len(warm), False)
+    torch.cuda.synchronize()
+    progress("served: warm-up done (%d sizes)" % len(warm))
+    jobs0 = [(lg, lb) for lg, lb, _ in wl[0]]
+    # capacity: every job queued at once; outputs verified between the forms
+    sat_p, _ = run_server(pl, pl.SERVE_PLAIN, jobs0, None, pool, scal[0], True)
+    sat_b, st_b = run_server(pl, pl.SERVE_BATCHED, jobs0, None, pool, scal[0], True)
+    exact = all(a[3] == b[3] for a, b in zip(sat_p, sat_b))
+    progress("served: saturation runs done, bit-exact %s" % exact)
+    cap_p = summarize(jobs0, sat_p)
+    cap_b = summarize(jobs0, sat_b)
+    plain_jobs_per_s = len(jobs0) / cap_p["span_s"]
+    rates = []
+    for ri, rel in enumerate(rates_rel):
+        jobs = [(lg, lb) for lg, lb, _ in wl[ri]]
+        rate = rel * plain_jobs_per_s
+        times = [t / rate for _, _, t in wl[ri]]
+        rp, _ = run_server(pl, pl.SERVE_PLAIN, jobs, times, pool, scal[ri], False)
+        rb, _ = run_server(pl, pl.SERVE_BATCHED, jobs, times, pool, scal[ri], False)
+        sp, sb = summarize(jobs, rp), summarize(jobs, rb)
+        progress("served: rate %.1f jobs/s: p99 batched %.1f ms, plain %.1f ms" % (rate, sb["p99_ms"], sp["p99_ms"]))
+        rates.append({"rate_jobs_per_s": round(rate, 2), "load_vs_plain_capacity": rel,
+                      "batched_p50_ms": sb["p50_ms"], "batched_p99_ms": sb["p99_ms"],
+                      "plain_p50_ms": sp["p50_ms"], "plain_p99_ms": sp["p99_ms"],
+                      "batched_elems_per_s": sb["elems_per_s"], "plain_elems_per_s": sp["elems_per_s"]})
+    del pool
+    torch.cuda.synchronize()
+    return {
+        "workload": "proof jobs: 8 cols x 2^lg (lg 14..24), iNTT -> LDE (lambda 2|4) -> perm GP -> eval -> div; "
+                    "%d jobs per run" % n_jobs,
+        "value": cap_b["elems_per_s"], "unit": "Fr elems/s (job inputs, all jobs queued at once)",
+        "plain_value": cap_p["elems_per_s"],
+        "speedup_vs_plain": round(cap_b["elems_per_s"] / cap_p["elems_per_s"], 3),
+        "saturated_p50_ms": cap_b["p50_ms"], "saturated_p99_ms": cap_b["p99_ms"],
+        "plain_saturated_p50_ms": cap_p["p50_ms"], "plain_saturated_p99_ms": cap_p["p99_ms"],
+        "poisson": rates,
+        "bit_exact_vs_plain": exact,
+        "groups": st_b["groups"], "arena_peak_gib": round(st_b["arena_peak"] / 2**30, 2),
+        "arena_gib": round(st_b["arena_bytes"] / 2**30, 1), "domain_cache_gib": round(st_b["domain_bytes"] / 2**30, 2),
+    }
+
+
+def cpu_baseline(lg):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import pyoracle as po  # the CPU baseline leg only
+    n = 1 << lg
+    xh = po.uniform(n, po.seed(po.OP_NTT, lg))
+    t0 = time.perf_counter()
+    po.ntt(xh, lg)
+    dt = time.perf_counter() - t0
+    return {"value": round(n / dt, 1), "unit": "Fr elems/s", "cores": 1, "kind": "port",
+            "sample": "one full NTT 2^%d by the oracle's radix-2 restatement (%.1f s), %s" % (
+                lg, dt, platform.processor() or platform.machine())}
+
+
 def main():
     args = parse()
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (no multi-GPU path); run with --gpus 1",
               file=sys.stderr)
         sys.exit(2)
-    import numpy as np
     import torch
 
-    import plffi
-    import pyoracle as po  # input generation and the CPU baseline only
+    import plffi as pl
+    import synth
 
     torch.cuda.init()
     stream = torch.cuda.current_stream()
     lg = args.lg
-    n = 1 << lg
     if args.copy_probe:
+        n = 1 << lg
         a = torch.ones((n, 4), dtype=torch.int64, device="cuda")
         b = torch.empty_like(a)
         for _ in range(3):
@@ -78,88 +376,43 @@ def main():
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_read": n * 32, "copy_probe_bytes_written": n * 32, "launches": 3}))
         return
-    dom = plffi.Domain(lg)
-    x = plffi.to_dev(po.uniform(n, po.seed(po.OP_NTT, lg)))
-    y = torch.empty_like(x)
-    work = torch.empty((n, 4), dtype=torch.int64, device="cuda")
-
-    # correctness gate on the measured buffers: iNTT(NTT(x)) == x
-    plffi.ntt(x, dom, out=y, work=work)
-    z = plffi.ntt(y, dom, inverse=True, work=work)
-    torch.cuda.synchronize()
-    verified = bool(torch.equal(z, x))
-
-    step = lambda: plffi.ntt(x, dom, out=y, work=work)  # noqa: E731
-    for _ in range(args.warmup):
-        step()
-    torch.cuda.synchronize()
-    ms = time_events(step, args.steps, stream)
-    torch.cuda.synchronize()
-    value = n / (ms * 1e-3)
-
-    # dominant kernel: intermediate pass (pass 0), one launch per rep
-    npass = plffi.ntt_num_passes(lg)
-    pass_ms = []
-    for p in range(npass):
-        src = x if p == 0 else work
-        dst = work if p < npass - 1 else y
-        for _ in range(2):
-            plffi.ntt_pass(src, dst, dom, p)
-        pass_ms.append(time_events(lambda: plffi.ntt_pass(src, dst, dom, p), args.steps, stream))
-    m_pass = [0] * npass
-    rem = lg
-    # same split as the library's plan: ceil(lg/8) passes, sizes differ by <= 1
-    base, extra = lg // npass, lg % npass
-    for p in range(npass):
-        m_pass[p] = base + (1 if p < extra else 0)
-    dom_ms = pass_ms[0]
-    mad_per_launch = (n // 2) * m_pass[0] * MAD_PER_MULMOD
-    achieved = mad_per_launch / (dom_ms * 1e-3)
-    bytes_per_launch = 64 * n
-    roofline = {
-        "bound": "valu",
-        "achieved": round(achieved / 1e12, 3),
-        "peak": round(PEAK_MAD / 1e12, 3),
-        "unit": "Tmad/s",
-        "frac": round(achieved / PEAK_MAD, 4),
-        "traffic": None,
-        "kernel": "ntt_pass_kernel<4,false,false> (pass 0 of %d, 2^%d-point columns)" % (npass, m_pass[0]),
-        "alg_per_launch": "%d mulmod = (N/2)*%d, x128 u32 multiply-adds" % ((n // 2) * m_pass[0], m_pass[0]),
-        "avg_launch_ms": round(dom_ms, 4),
-        "pass_ms": [round(v, 4) for v in pass_ms],
-        "hbm_gbps_alg": round(bytes_per_launch / (dom_ms * 1e-3) / 1e9, 1),
-        "hbm_frac_alg": round(bytes_per_launch / (dom_ms * 1e-3) / PEAK_HBM, 4),
-        "whole_ntt_frac": round(((n // 2) * lg * MAD_PER_MULMOD / PEAK_MAD) / (ms * 1e-3), 4),
-    }
-
+    peak = pl.probe_mad_rate() * 1e12
+    progress("measured v_mad_u64_u32 peak %.2f T/s" % (peak / 1e12))
+    ntt, roof = bench_ntt(pl, synth, lg, args.steps, args.warmup, stream, peak)
+    progress("ntt 2^%d: %.3f ms" % (lg, ntt["ms"]))
+    torch.cuda.empty_cache()
     result = {
         "metric": "Fr elems/s for 2^24 NTT & LDE; ragged-batch p50/p99 vs one-call-per-launch",
-        "value": round(value, 1),
+        "value": round(ntt["value"], 1),
         "unit": "Fr elems/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": round(ms, 4),
+        "ms_per_step": round(ntt["ms"], 4),
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
         "dtype": "u32",
-        "data": "synthetic (SplitMix64 uniform Fr, Montgomery form)",
+        "data": "synthetic (SplitMix64 uniform Fr, Montgomery form, generated on the device)",
         "config": {"workload": "NTT 2^%d, natural order in/out, device-resident" % lg, "lg_n": lg,
-                   "passes": npass, "verified_intt_roundtrip": verified},
-        "roofline": roofline,
+                   "passes": ntt["npass"], "verified_intt_roundtrip": ntt["verified"]},
+        "roofline": roof,
+        "measured_mad_peak_tmad_s": round(peak / 1e12, 3),
     }
-
+    if not args.no_lde:
+        result["lde"] = bench_lde(pl, synth, lg, 2, args.steps, args.warmup, stream, peak)
+        progress("lde: %.3f ms" % result["lde"]["ms"])
+        torch.cuda.empty_cache()
+    if not args.no_batch:
+        result["batched"] = bench_batch(pl, synth, args.steps, stream)
+        progress("batched: %.3f ms vs plain %.3f ms" % (result["batched"]["p50_ms"], result["batched"]["plain_p50_ms"]))
+        torch.cuda.empty_cache()
+    if not args.no_serve:
+        result["served"] = bench_served(pl, synth, args.serve_jobs)
+        torch.cuda.empty_cache()
     if not args.no_cpu:
-        xh = po.uniform(n, po.seed(po.OP_NTT, lg))
-        t0 = time.perf_counter()
-        po.ntt(xh, lg)
-        dt = time.perf_counter() - t0
-        result["cpu_baseline"] = {
-            "value": round(n / dt, 1), "unit": "Fr elems/s", "cores": 1, "kind": "port",
-            "sample": "one full NTT 2^%d by the oracle's radix-2 restatement (%.1f s), %s" % (
-                lg, dt, platform.processor() or platform.machine()),
-        }
+        progress("cpu baseline (one core, ~15 s)")
+        result["cpu_baseline"] = cpu_baseline(lg)
     print(json.dumps(result), flush=True)
 
 

@@ -273,6 +273,11 @@ void pl_server_destroy(pl_server* s);
 size_t pl_uniform_work_bytes(uint64_t n);
 int pl_fill_uniform(void* d_out, uint64_t n, uint64_t seed, void* d_work, pl_stream_t stream);
 int pl_digest(const void* d_data, uint64_t n, void* d_out, pl_stream_t stream);
+/* Measured peak of 32x32->64-bit multiply-adds (v_mad_u64_u32) on the
+ * current device, in 10^12 per second: independent chains in every lane of
+ * a grid filling every CU (the roofline the Montgomery kernels are held to).
+ * Synchronizes the stream. */
+int pl_probe_mad_rate(double* tmad_per_s, pl_stream_t stream);
 
 #ifdef __cplusplus
 }

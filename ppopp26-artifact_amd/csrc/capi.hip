@@ -341,6 +341,14 @@ int pl_digest(const void* d_data, uint64_t n, void* d_out, pl_stream_t stream) {
   return e == hipSuccess ? PL_OK : hip_fail(e);
 }
 
+int pl_probe_mad_rate(double* tmad_per_s, pl_stream_t stream) {
+  if (!tmad_per_s) return PL_ERR_ARG;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return PL_ERR_NO_DEVICE;
+  hipError_t e = pl::mad_probe(tmad_per_s, (hipStream_t)stream);
+  return e == hipSuccess ? PL_OK : hip_fail(e);
+}
+
 int pl_ntt_naive(const void* d_in, void* d_out, int lg, const pl_domain* d, pl_stream_t stream) {
   if (!d || !d_in || !d_out || d_in == d_out || d->d.lg != lg) return PL_ERR_ARG;
   if (!aligned16(d_in) || !aligned16(d_out)) return PL_ERR_ARG;

@@ -99,6 +99,7 @@ hipError_t perm_ratio(const fr* const w[3], const fr* const sg[3], const fr* bet
 uint64_t uniform_work_bytes(uint64_t n);
 hipError_t fill_uniform(fr* out, uint64_t n, uint64_t seed, uint32_t* work, hipStream_t s);
 hipError_t digest(const fr* d, uint64_t n, uint64_t* out, hipStream_t s);
+hipError_t mad_probe(double* tmad_per_s, hipStream_t s);
 
 // element-wise helper kernels
 hipError_t fill_pow_table(fr* out, uint64_t n, const host::fr64& base, const host::fr64& scale, hipStream_t s);

@@ -166,3 +166,58 @@ hipError_t digest(const fr* d, uint64_t n, uint64_t* out, hipStream_t s) {
 }
 
 }  // namespace pl
+
+// ---- multiply-add rate probe ---------------------------------------------
+// Independent chains of 32x32->64 multiply-adds (v_mad_u64_u32) in every
+// lane of a grid that fills all CUs several times over: the integer
+// multiply-add peak the Montgomery kernels are measured against.
+namespace pl {
+
+constexpr int kProbeChains = 8;
+
+__global__ void __launch_bounds__(256) mad_probe_kernel(uint32_t iters, uint32_t seed, uint64_t* sink) {
+  uint32_t a[kProbeChains];
+  uint64_t c[kProbeChains];
+  for (int i = 0; i < kProbeChains; i++) {
+    a[i] = seed * (threadIdx.x + 1) + i;
+    c[i] = a[i] ^ blockIdx.x;
+  }
+  const uint32_t b = seed | 1;
+  for (uint32_t it = 0; it < iters; it++) {
+    // exactly one v_mad_u64_u32 per statement: c = a * b + c
+#pragma unroll
+    for (int i = 0; i < kProbeChains; i++) {
+      uint64_t cc;
+      asm volatile("v_mad_u64_u32 %0, %1, %2, %3, %0" : "+v"(c[i]), "=s"(cc) : "v"(a[i]), "s"(b));
+    }
+  }
+  uint64_t s = 0;
+  for (int i = 0; i < kProbeChains; i++) s ^= c[i];
+  if (s == 0x123456789abcdefull) sink[0] = s;  // keeps the chains live
+}
+
+hipError_t mad_probe(double* tmad_per_s, hipStream_t s) {
+  uint64_t* sink = nullptr;
+  hipEvent_t e0 = nullptr, e1 = nullptr;
+  hipError_t e = hipMalloc(&sink, 64);
+  if (e == hipSuccess) e = hipEventCreate(&e0);
+  if (e == hipSuccess) e = hipEventCreate(&e1);
+  const uint32_t iters = 4096, blocks = 256 * 16;
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(mad_probe_kernel, dim3(blocks), dim3(256), 0, s, iters, 0x9e3779b9u, sink);  // warm-up
+    (void)hipEventRecord(e0, s);
+    hipLaunchKernelGGL(mad_probe_kernel, dim3(blocks), dim3(256), 0, s, iters, 0x85ebca6bu, sink);
+    (void)hipEventRecord(e1, s);
+    e = hipEventSynchronize(e1);
+  }
+  float ms = 0;
+  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+  if (e == hipSuccess)
+    *tmad_per_s = (double)blocks * 256 * iters * kProbeChains / (ms * 1e-3) / 1e12;
+  if (e0) (void)hipEventDestroy(e0);
+  if (e1) (void)hipEventDestroy(e1);
+  if (sink) (void)hipFree(sink);
+  return e;
+}
+
+}  // namespace pl

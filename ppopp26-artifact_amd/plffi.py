@@ -78,6 +78,7 @@ SIGNATURES = {
     "pl_uniform_work_bytes": (ctypes.c_size_t, [_u64]),
     "pl_fill_uniform": (_i, [_vp, _u64, _u64, _vp, _vp]),
     "pl_digest": (_i, [_vp, _u64, _vp, _vp]),
+    "pl_probe_mad_rate": (_i, [ctypes.POINTER(ctypes.c_double), _vp]),
 }
 
 SERVE_BATCHED, SERVE_PLAIN = 0, 1
@@ -387,6 +388,13 @@ class Server:
             self.close()
         except Exception:
             pass
+
+
+def probe_mad_rate(stream=None):
+    """Measured v_mad_u64_u32 peak of the device, 10^12 per second."""
+    v = ctypes.c_double()
+    check("pl_probe_mad_rate", lib().pl_probe_mad_rate(ctypes.byref(v), _stream(stream)))
+    return v.value
 
 
 def job_bytes(lg, lg_blowup):
