@@ -184,12 +184,21 @@ __global__ void __launch_bounds__(256) mad_probe_kernel(uint32_t iters, uint32_t
   }
   const uint32_t b = seed | 1;
   for (uint32_t it = 0; it < iters; it++) {
-    // exactly one v_mad_u64_u32 per statement: c = a * b + c
-#pragma unroll
-    for (int i = 0; i < kProbeChains; i++) {
-      uint64_t cc;
-      asm volatile("v_mad_u64_u32 %0, %1, %2, %3, %0" : "+v"(c[i]), "=s"(cc) : "v"(a[i]), "s"(b));
-    }
+    // eight independent c = a * b + c multiply-adds in one block (no
+    // compiler-inserted wait states between them); the carry-out lane masks
+    // go to four SGPR pairs in rotation
+    asm volatile(
+        "v_mad_u64_u32 %0, s[40:41], %8, %16, %0\n\t"
+        "v_mad_u64_u32 %1, s[42:43], %9, %16, %1\n\t"
+        "v_mad_u64_u32 %2, s[44:45], %10, %16, %2\n\t"
+        "v_mad_u64_u32 %3, s[46:47], %11, %16, %3\n\t"
+        "v_mad_u64_u32 %4, s[40:41], %12, %16, %4\n\t"
+        "v_mad_u64_u32 %5, s[42:43], %13, %16, %5\n\t"
+        "v_mad_u64_u32 %6, s[44:45], %14, %16, %6\n\t"
+        "v_mad_u64_u32 %7, s[46:47], %15, %16, %7"
+        : "+v"(c[0]), "+v"(c[1]), "+v"(c[2]), "+v"(c[3]), "+v"(c[4]), "+v"(c[5]), "+v"(c[6]), "+v"(c[7])
+        : "v"(a[0]), "v"(a[1]), "v"(a[2]), "v"(a[3]), "v"(a[4]), "v"(a[5]), "v"(a[6]), "v"(a[7]), "s"(b)
+        : "s40", "s41", "s42", "s43", "s44", "s45", "s46", "s47");
   }
   uint64_t s = 0;
   for (int i = 0; i < kProbeChains; i++) s ^= c[i];
@@ -202,18 +211,25 @@ hipError_t mad_probe(double* tmad_per_s, hipStream_t s) {
   hipError_t e = hipMalloc(&sink, 64);
   if (e == hipSuccess) e = hipEventCreate(&e0);
   if (e == hipSuccess) e = hipEventCreate(&e1);
-  const uint32_t iters = 4096, blocks = 256 * 16;
-  if (e == hipSuccess) {
-    hipLaunchKernelGGL(mad_probe_kernel, dim3(blocks), dim3(256), 0, s, iters, 0x9e3779b9u, sink);  // warm-up
+  const uint32_t iters = 2048, blocks = 256 * 16, reps = 5;
+  double best = 0;
+  for (int trial = 0; trial < 3 && e == hipSuccess; trial++) {
+    // warm-up launches bring the clocks up; then the average of `reps`
+    for (int w = 0; w < 2; w++)
+      hipLaunchKernelGGL(mad_probe_kernel, dim3(blocks), dim3(256), 0, s, iters, 0x9e3779b9u + w, sink);
     (void)hipEventRecord(e0, s);
-    hipLaunchKernelGGL(mad_probe_kernel, dim3(blocks), dim3(256), 0, s, iters, 0x85ebca6bu, sink);
+    for (uint32_t r = 0; r < reps; r++)
+      hipLaunchKernelGGL(mad_probe_kernel, dim3(blocks), dim3(256), 0, s, iters, 0x85ebca6bu + r, sink);
     (void)hipEventRecord(e1, s);
     e = hipEventSynchronize(e1);
+    float ms = 0;
+    if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
+    if (e == hipSuccess) {
+      const double rate = (double)reps * blocks * 256 * iters * kProbeChains / (ms * 1e-3) / 1e12;
+      if (rate > best) best = rate;
+    }
   }
-  float ms = 0;
-  if (e == hipSuccess) e = hipEventElapsedTime(&ms, e0, e1);
-  if (e == hipSuccess)
-    *tmad_per_s = (double)blocks * 256 * iters * kProbeChains / (ms * 1e-3) / 1e12;
+  if (e == hipSuccess) *tmad_per_s = best;
   if (e0) (void)hipEventDestroy(e0);
   if (e1) (void)hipEventDestroy(e1);
   if (sink) (void)hipFree(sink);

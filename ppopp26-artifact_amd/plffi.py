@@ -411,9 +411,12 @@ def fill_uniform(out, seed, stream=None):
 def digest_ptr(ptr, n, stream=None):
     """64-bit digest of n elements at a device address (see synth.digest)."""
     import torch
-    o = torch.zeros(1, dtype=torch.int64, device="cuda")
-    check("pl_digest", lib().pl_digest(ctypes.c_void_p(ptr), n, _ptr(o), _stream(stream)))
-    return int(o.cpu().numpy().view("uint64")[0])
+    s = stream if stream is not None else torch.cuda.current_stream()
+    with torch.cuda.stream(s):
+        o = torch.zeros(1, dtype=torch.int64, device="cuda")
+        check("pl_digest", lib().pl_digest(ctypes.c_void_p(ptr), n, _ptr(o), _stream(s)))
+        s.synchronize()
+        return int(o.cpu().numpy().view("uint64")[0])
 
 
 def digest(t, stream=None):
