@@ -41,8 +41,9 @@ PEAK_HBM = 8.0e12
 MAD_PER_MULMOD = 128  # 8-limb Montgomery product: 2 * 8^2 limb multiply-adds
 # HBM bytes per launch of the dominant kernel (NTT pass 0 at 2^24), from
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs scaled by a plain copy of
-# known size (profiles/ntt_pass_2p24_hbm_traffic.txt); None when not profiled
-TRAFFIC_PASS0_2P24 = None
+# known size (profiles/ntt_pass_2p24_hbm_traffic.txt): the data (2^24 x 32 B
+# read and written) plus the pass's 2^24-entry twiddle table read once
+TRAFFIC_PASS0_2P24 = 1656578784
 
 
 def parse():
@@ -56,6 +57,9 @@ def parse():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=40, help="jobs per served run")
+    ap.add_argument("--plans", default=os.path.join(ROOT, "ppopp26-artifact_amd", "tune", "mi355x_plans.txt"),
+                    help="autotuned pass-split cache (measured and written when absent)")
+    ap.add_argument("--no-tune", action="store_true", help="default pass splits")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain device copy of 2^lg * 32 bytes (HBM counter calibration)")
     return ap.parse_args()
@@ -378,6 +382,13 @@ def main():
         return
     peak = pl.probe_mad_rate() * 1e12
     progress("measured v_mad_u64_u32 peak %.2f T/s" % (peak / 1e12))
+    plans = None
+    if not args.no_tune:
+        t0 = time.time()
+        os.makedirs(os.path.dirname(args.plans), exist_ok=True)
+        pl.autotune(4, 24, args.plans, reps=5)
+        plans = {lg: pl.plan_get(lg) for lg in range(12, 25)}
+        progress("pass splits from %s (%.1f s)" % (args.plans, time.time() - t0))
     ntt, roof = bench_ntt(pl, synth, lg, args.steps, args.warmup, stream, peak)
     progress("ntt 2^%d: %.3f ms" % (lg, ntt["ms"]))
     torch.cuda.empty_cache()
@@ -398,6 +409,7 @@ def main():
                    "passes": ntt["npass"], "verified_intt_roundtrip": ntt["verified"]},
         "roofline": roof,
         "measured_mad_peak_tmad_s": round(peak / 1e12, 3),
+        "pass_splits": {str(k): v for k, v in plans.items()} if plans else "default",
     }
     if not args.no_lde:
         result["lde"] = bench_lde(pl, synth, lg, 2, args.steps, args.warmup, stream, peak)

@@ -122,6 +122,27 @@ int pl_eval(const void* d_poly, uint64_t n, const void* d_x, void* d_y, void* d_
  * discards it).  c is a device element and is not modified. */
 int pl_div_linear(void* d_inout, uint64_t n, const void* d_c, void* d_rem, void* d_work, pl_stream_t stream);
 
+/* ---- pass-split autotuner ----------------------------------------------------
+ * (replaces the reference's manual segment sweep, NTT/LDE --bench-sweep,
+ *  ntt.cu:1494-1500, lde.cu:1834-1846)
+ * A size-2^lg transform runs as P passes of 2^m[p]-point columns (4 <= m <= 8,
+ * sum = lg); every split gives the same exact result.  pl_autotune times the
+ * candidate splits of each lg in [lg_lo, lg_hi] on the current device (a
+ * batch of max(1, 2^22 / N) polynomials per call), keeps the fastest and
+ * records the choices in cache_path ("<lg> <P> <m0..> <ms>" lines); sizes
+ * already in the cache are not re-measured unless PL_TUNE_REMEASURE.  The
+ * chosen splits apply to domains created afterwards (process-wide).
+ * pl_plan_get reports the split in use (P = 0 below lg 4: one tiny kernel);
+ * pl_plan_set sets one (P = 0 restores the default); pl_plan_load applies a
+ * cache file and returns the number of sizes read; pl_plan_candidates writes
+ * up to max_cand splits (8 ints each, zero-padded) and returns their count. */
+enum pl_tune_flags { PL_TUNE_REMEASURE = 1 };
+int pl_autotune(int lg_lo, int lg_hi, const char* cache_path, int reps, int flags);
+int pl_plan_get(int lg, int* m, int* P);
+int pl_plan_set(int lg, const int* m, int P);
+int pl_plan_load(const char* path);
+int pl_plan_candidates(int lg, int* out, int max_cand);
+
 /* ---- baselines ------------------------------------------------------------
  * [standard4_ntt, ntt.cu:1039-1138; naive_lde, lde.cu:1367-1490]
  * Workload baselines for the CLIs: bit reversal plus one radix-2 launch per
@@ -224,7 +245,7 @@ void pl_batch_destroy(pl_batch* b);
  * completion.  A job that can never fit fails with PL_ERR_NOMEM. */
 enum pl_serve_mode { PL_SERVE_BATCHED = 0, PL_SERVE_PLAIN = 1 };
 typedef struct pl_server_config {
-  size_t hbm_bytes;      /* arena size; 0 = free device memory less 24 GiB for domain tables */
+  size_t hbm_bytes;      /* arena size; 0 = free device memory less 32 GiB for domain tables */
   int slots;             /* groups running at once, 1..8; 0 = 3 (plain: 1) */
   int mode;              /* pl_serve_mode */
   int max_group_jobs;    /* jobs merged into one group; 0 = 16 */

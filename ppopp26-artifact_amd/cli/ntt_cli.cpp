@@ -135,7 +135,8 @@ static void usage(const char* exe) {
             << "  " << exe << " --bench-baseline <lg_N>\n"
             << "  " << exe << " --check-split <lg_N>\n"
             << "  " << exe << " --check-baseline <lg_N>\n"
-            << "  " << exe << " --bench-sweep <lg_N> [reps]\n";
+            << "  " << exe << " --bench-sweep <lg_N> [reps]\n"
+            << "  " << exe << " --autotune <lg_lo> <lg_hi> [cache_file]\n";
 }
 
 int main(int argc, char** argv) {
@@ -155,6 +156,24 @@ int main(int argc, char** argv) {
     if (cmd == "--bench-split") {
       NttScaleConfig c = ntt_scale_config(lg_N);
       return run_bench(c, c.ours_lg_k, c.ours_lg_k, c.reps, false);
+    }
+    if (cmd == "--autotune") {
+      // pass-split autotuner (replaces the manual segment sweep)
+      if (argc < 4) {
+        usage(argv[0]);
+        return 1;
+      }
+      const int lo = std::atoi(argv[2]), hi = std::atoi(argv[3]);
+      const char* cache = argc >= 5 ? argv[4] : nullptr;
+      cli::check(pl_autotune(lo, hi, cache, 5, 0), "pl_autotune");
+      for (int lg = lo; lg <= hi; lg++) {
+        int m[8], P = 0;
+        cli::check(pl_plan_get(lg, m, &P), "pl_plan_get");
+        std::cout << "[autotune] lg_N = " << lg << ": " << P << " passes of";
+        for (int i = 0; i < P; i++) std::cout << " 2^" << m[i];
+        std::cout << std::endl;
+      }
+      return 0;
     }
     if (cmd == "--bench-sweep") {
       NttScaleConfig c = ntt_scale_config(lg_N);

@@ -125,7 +125,7 @@ int pl_ntt_pass(const void* d_in, void* d_out, int lg, uint32_t npoly, int pass,
                 pl_stream_t stream) {
   if (!d || !d_in || !d_out || npoly == 0 || d->d.lg != lg) return PL_ERR_ARG;
   if (!aligned16(d_in) || !aligned16(d_out)) return PL_ERR_ARG;
-  const int np = pl_ntt_num_passes(lg);
+  const int np = d->d.plan.P == 0 ? 1 : d->d.plan.P;  // the domain's own split
   if (pass < 0 || pass >= np) return PL_ERR_ARG;
   // the last pass scatters across the whole polynomial: not in place
   if (pass == np - 1 && np > 1 && d_in == d_out) return PL_ERR_ARG;

@@ -1,5 +1,7 @@
 // NTT host side: plans, domain tables, pass sequencing; small-size kernel
 // and table builders.  The pass kernel itself is in ntt_pass.hpp.
+#include <mutex>
+
 #include "ntt_pass.hpp"
 
 namespace pl {
@@ -74,24 +76,62 @@ static inline fr to_dev(const host::fr64& h) {
 
 static inline unsigned grid_for(uint64_t n, unsigned block) { return (unsigned)((n + block - 1) / block); }
 
-NttPlan make_plan(int lg) {
+NttPlan plan_from_splits(int lg, const int* m, int P) {
   NttPlan p;
   p.lg = lg;
-  if (lg < 4) {
-    p.P = 0;
-    return p;
-  }
-  int P = (lg + 7) / 8;
-  int base = lg / P, rem = lg % P;
+  p.P = P;
   int lgM = lg;
   for (int i = 0; i < P; i++) {
-    p.m[i] = base + (i < rem ? 1 : 0);
+    p.m[i] = m[i];
     p.lgM[i] = lgM;
-    p.lgS[i] = lgM - p.m[i];
-    lgM -= p.m[i];
+    p.lgS[i] = lgM - m[i];
+    lgM -= m[i];
   }
-  p.P = P;
   return p;
+}
+
+// Pass splits chosen by the autotuner (P == 0: the default split below).
+static std::mutex g_plan_mu;
+static int g_plan_P[29];
+static int g_plan_m[29][PL_MAX_PASSES];
+
+bool plan_valid(int lg, const int* m, int P) {
+  if (lg < 4 || lg > 28 || P < 1 || P > PL_MAX_PASSES) return false;
+  int sum = 0;
+  for (int i = 0; i < P; i++) {
+    if (m[i] < 4 || m[i] > 8) return false;
+    sum += m[i];
+  }
+  return sum == lg;
+}
+
+bool plan_set(int lg, const int* m, int P) {
+  if (lg < 0 || lg > 28) return false;
+  if (P != 0 && !plan_valid(lg, m, P)) return false;
+  std::lock_guard<std::mutex> g(g_plan_mu);
+  g_plan_P[lg] = P;
+  for (int i = 0; i < P; i++) g_plan_m[lg][i] = m[i];
+  return true;
+}
+
+// default: ceil(lg/8) passes of 4..8 points, sizes differing by <= 1, larger first
+NttPlan default_plan(int lg) {
+  NttPlan p;
+  p.lg = lg;
+  if (lg < 4) return p;
+  const int P = (lg + 7) / 8;
+  int m[PL_MAX_PASSES];
+  const int base = lg / P, rem = lg % P;
+  for (int i = 0; i < P; i++) m[i] = base + (i < rem ? 1 : 0);
+  return plan_from_splits(lg, m, P);
+}
+
+NttPlan make_plan(int lg) {
+  if (lg >= 4 && lg <= 28) {
+    std::lock_guard<std::mutex> g(g_plan_mu);
+    if (g_plan_P[lg]) return plan_from_splits(lg, g_plan_m[lg], g_plan_P[lg]);
+  }
+  return default_plan(lg);
 }
 
 // out[i] = base^e(i) * scale for the n entries of a table (see pow_window_kernel)
@@ -166,27 +206,34 @@ done:
   return e;
 }
 
+// cp[t << lg_n | i] = (shift * w_M^t)^i, M = 2^(lg_n + lg_blowup)
+hipError_t coset_table_build(fr** cp, size_t* bytes, int lg_n, int lg_blowup, const host::fr64& shift,
+                             hipStream_t s) {
+  const uint64_t n = 1ull << lg_n;
+  const uint64_t B = 1ull << lg_blowup;
+  *bytes = (n * B) * sizeof(fr);
+  hipError_t e = hipMalloc((void**)cp, *bytes);
+  if (e != hipSuccess) return e;
+  fr* win = nullptr;
+  if ((e = hipMalloc((void**)&win, 2 * 16384 * sizeof(fr))) != hipSuccess) return e;
+  const host::fr64 wM = host::root_of_unity(lg_n + lg_blowup);
+  host::fr64 st = shift;
+  for (uint64_t t = 0; t < B; t++) {
+    if ((e = fill_pow(*cp + t * n, n, 0, st, nullptr, win, s)) != hipSuccess) break;
+    st = host::mul(st, wM);
+  }
+  if (e == hipSuccess) e = hipStreamSynchronize(s);
+  (void)hipFree(win);
+  return e;
+}
+
 hipError_t lde_domain_build(LdeDomain& d, int lg_n, int lg_blowup, const host::fr64& shift, hipStream_t s) {
   d.lg_n = lg_n;
   d.lg_blowup = lg_blowup;
   d.shift = shift;
   hipError_t e = domain_build(d.ntt, lg_n, s);
   if (e != hipSuccess) return e;
-  const uint64_t n = 1ull << lg_n;
-  const uint64_t B = 1ull << lg_blowup;
-  d.bytes = (n * B) * sizeof(fr);
-  if ((e = hipMalloc((void**)&d.cp, d.bytes)) != hipSuccess) return e;
-  fr* win = nullptr;
-  if ((e = hipMalloc((void**)&win, 2 * 16384 * sizeof(fr))) != hipSuccess) return e;
-  const host::fr64 wM = host::root_of_unity(lg_n + lg_blowup);
-  host::fr64 st = shift;
-  for (uint64_t t = 0; t < B; t++) {
-    if ((e = fill_pow(d.cp + t * n, n, 0, st, nullptr, win, s)) != hipSuccess) break;
-    st = host::mul(st, wM);
-  }
-  if (e == hipSuccess) e = hipStreamSynchronize(s);
-  (void)hipFree(win);
-  return e;
+  return coset_table_build(&d.cp, &d.bytes, lg_n, lg_blowup, shift, s);
 }
 
 // The reference's windowed power tables (parameters.cuh:7-33; the first

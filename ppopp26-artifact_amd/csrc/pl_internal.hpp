@@ -24,7 +24,12 @@ struct NttPlan {
   int lgS[PL_MAX_PASSES] = {};
 };
 
-NttPlan make_plan(int lg);
+NttPlan make_plan(int lg);         // the tuned split when one is set, else default_plan
+NttPlan default_plan(int lg);
+NttPlan plan_from_splits(int lg, const int* m, int P);
+bool plan_valid(int lg, const int* m, int P);
+// set the split used by domains built from now on (P == 0: back to the default)
+bool plan_set(int lg, const int* m, int P);
 
 // Device-resident tables for NTT / iNTT of one size (the reference's Ntt /
 // Intt Params, PLONK/utils/function.cu:278, 328; parameters.cuh:142-222).
@@ -53,6 +58,9 @@ struct LdeDomain {
 };
 
 hipError_t lde_domain_build(LdeDomain& d, int lg_n, int lg_blowup, const host::fr64& shift, hipStream_t s);
+// the coset factor table alone (*bytes = its size)
+hipError_t coset_table_build(fr** cp, size_t* bytes, int lg_n, int lg_blowup, const host::fr64& shift,
+                             hipStream_t s);
 void lde_domain_free(LdeDomain& d);
 
 // Launch geometry for one call: a batch of `npoly` polynomials of size 2^lg.

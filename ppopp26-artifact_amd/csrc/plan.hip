@@ -13,7 +13,11 @@ namespace pl {
 // ends
 static std::mutex g_cache_mu;
 static std::map<std::pair<int, int>, Domain*> g_domains;
-static std::map<std::tuple<int, int, int>, LdeDomain*> g_lde;
+struct CosetTable {
+  fr* cp = nullptr;
+  size_t bytes = 0;
+};
+static std::map<std::tuple<int, int, int>, CosetTable> g_coset;
 
 static int current_device() {
   int dev = 0;
@@ -40,31 +44,29 @@ Domain* cached_domain(int lg, hipError_t* err) {
   return d;
 }
 
-LdeDomain* cached_lde_domain(int lg_n, int lg_blowup, hipError_t* err) {
+const fr* cached_coset_table(int lg_n, int lg_blowup, hipError_t* err) {
   std::lock_guard<std::mutex> g(g_cache_mu);
   const auto key = std::make_tuple(current_device(), lg_n, lg_blowup);
-  auto it = g_lde.find(key);
-  if (it != g_lde.end()) return it->second;
-  LdeDomain* d = new LdeDomain();
+  auto it = g_coset.find(key);
+  if (it != g_coset.end()) return it->second.cp;
+  CosetTable t;
   // the reference's coset factor 7^(2^lambda) (lde.cu:642-645, 659)
   host::fr64 shift = host::pow_u64(host::from_u64(7), 1ull << lg_blowup);
-  hipError_t e = lde_domain_build(*d, lg_n, lg_blowup, shift, nullptr);
-  if (e == hipSuccess) e = hipStreamSynchronize(nullptr);
+  hipError_t e = coset_table_build(&t.cp, &t.bytes, lg_n, lg_blowup, shift, nullptr);
   if (e != hipSuccess) {
-    lde_domain_free(*d);
-    delete d;
+    if (t.cp) (void)hipFree(t.cp);
     if (err) *err = e;
     return nullptr;
   }
-  g_lde[key] = d;
-  return d;
+  g_coset[key] = t;
+  return t.cp;
 }
 
 size_t cached_domain_bytes() {
   std::lock_guard<std::mutex> g(g_cache_mu);
   size_t b = 0;
   for (auto& kv : g_domains) b += kv.second->bytes;
-  for (auto& kv : g_lde) b += kv.second->bytes + kv.second->ntt.bytes;
+  for (auto& kv : g_coset) b += kv.second.bytes;
   return b;
 }
 
@@ -99,12 +101,12 @@ hipError_t phase_plan(PhasePlan& ph, const std::vector<Item>& items, Bump& bump,
     c.in = it.in;
     c.out = it.out;
     if (it.op == IT_LDE) {
-      const LdeDomain* ld = cached_lde_domain(it.lg, it.lg_blowup, &e);
-      if (!ld) return e;
-      d = &ld->ntt;
+      d = cached_domain(it.lg, &e);
+      const fr* cp = d ? cached_coset_table(it.lg, it.lg_blowup, &e) : nullptr;
+      if (!cp) return e;
       c.in_pstride = 0;  // every coset reads the same coefficients
       c.npoly = 1u << it.lg_blowup;
-      c.premul = ld->cp;
+      c.premul = cp;
       c.pm_mask = c.npoly - 1;
       c.lgI = it.lg_blowup;
     } else {

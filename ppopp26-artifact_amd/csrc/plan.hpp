@@ -18,7 +18,9 @@ namespace pl {
 
 // domains shared by every batch and server of the process, keyed by device
 Domain* cached_domain(int lg, hipError_t* err);
-LdeDomain* cached_lde_domain(int lg_n, int lg_blowup, hipError_t* err);
+// coset factor table of the reference's LDE (shift 7^(2^lg_blowup)); the
+// transform itself uses cached_domain(lg_n)
+const fr* cached_coset_table(int lg_n, int lg_blowup, hipError_t* err);
 size_t cached_domain_bytes();
 
 enum ItemOp { IT_NTT = 0, IT_INTT = 1, IT_LDE = 2, IT_EVAL_DIV = 3, IT_PERM_GP = 4 };

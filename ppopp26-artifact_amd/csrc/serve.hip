@@ -472,8 +472,9 @@ int pl_server_create(const pl_server_config* cfg, pl_server** out) {
   if (c.hbm_bytes == 0) {
     size_t fr_ = 0, tot = 0;
     if (hipMemGetInfo(&fr_, &tot) != hipSuccess) return PL_ERR_HIP;
-    // leave room for the domain caches (coset tables up to 2^28 elements)
-    const size_t reserve = (size_t)24 << 30;
+    // leave room for the domain caches: coset tables (2^(lg + lg_blowup)
+    // elements per size), NTT twiddles (~2 N per size) and windows
+    const size_t reserve = (size_t)32 << 30;
     c.hbm_bytes = fr_ > 2 * reserve ? fr_ - reserve : fr_ / 2;
   }
   pl_server* S = new (std::nothrow) pl_server();

@@ -79,6 +79,11 @@ SIGNATURES = {
     "pl_fill_uniform": (_i, [_vp, _u64, _u64, _vp, _vp]),
     "pl_digest": (_i, [_vp, _u64, _vp, _vp]),
     "pl_probe_mad_rate": (_i, [ctypes.POINTER(ctypes.c_double), _vp]),
+    "pl_autotune": (_i, [_i, _i, ctypes.c_char_p, _i, _i]),
+    "pl_plan_get": (_i, [_i, ctypes.POINTER(_i), ctypes.POINTER(_i)]),
+    "pl_plan_set": (_i, [_i, ctypes.POINTER(_i), _i]),
+    "pl_plan_load": (_i, [ctypes.c_char_p]),
+    "pl_plan_candidates": (_i, [_i, ctypes.POINTER(_i), _i]),
 }
 
 SERVE_BATCHED, SERVE_PLAIN = 0, 1
@@ -395,6 +400,30 @@ def probe_mad_rate(stream=None):
     v = ctypes.c_double()
     check("pl_probe_mad_rate", lib().pl_probe_mad_rate(ctypes.byref(v), _stream(stream)))
     return v.value
+
+
+def autotune(lg_lo, lg_hi, cache_path=None, reps=5, remeasure=False):
+    check("pl_autotune", lib().pl_autotune(lg_lo, lg_hi, cache_path.encode() if cache_path else None, reps,
+                                           1 if remeasure else 0))
+
+
+def plan_get(lg):
+    m = (_i * 8)()
+    P = _i()
+    check("pl_plan_get", lib().pl_plan_get(lg, m, ctypes.byref(P)))
+    return [m[i] for i in range(P.value)]
+
+
+def plan_set(lg, m):
+    arr = (_i * 8)(*m) if m else None
+    check("pl_plan_set", lib().pl_plan_set(lg, arr, len(m) if m else 0))
+
+
+def plan_candidates(lg):
+    n = lib().pl_plan_candidates(lg, None, 0)
+    buf = (_i * (8 * max(n, 1)))()
+    lib().pl_plan_candidates(lg, buf, n)
+    return [[buf[8 * i + k] for k in range(8) if buf[8 * i + k]] for i in range(n)]
 
 
 def job_bytes(lg, lg_blowup):
