@@ -80,3 +80,16 @@ def test_miss_ops_cli():
     assert p.returncode == 0
     for tag in ("[naive_gp]", "[ours_gp]", "[ours_perm_gp]", "[ours_eval]", "[naive_div]", "[ours_div]"):
         assert p.stdout.count(tag + " execution time:") == 3, tag
+
+
+def test_ntt_cli_autotune(tmp_path):
+    cache = str(tmp_path / "plans.txt")
+    p = run(["NTT", "--autotune", "10", "12", cache])
+    assert p.returncode == 0
+    lines = [l for l in p.stdout.splitlines() if l.startswith("[autotune]")]
+    assert len(lines) == 3
+    rows = [l.split() for l in open(cache) if not l.startswith("#")]
+    assert [int(r[0]) for r in rows] == [10, 11, 12]
+    for r in rows:
+        m = [int(v) for v in r[2:2 + int(r[1])]]
+        assert sum(m) == int(r[0]) and all(4 <= v <= 8 for v in m)
