@@ -387,19 +387,23 @@ static bool try_admit(pl_server& S) {
     if (cand.size() == 1) break;
     cand.pop_back();  // try a smaller group
   }
-  // the head does not fit now; if nothing is running it never will
-  bool running = false;
-  for (auto& s : S.slots) running |= s.busy;
-  if (!running) {
-    Job* j = S.queue.front();
-    j->status = PL_ERR_NOMEM;
-    j->done = true;
-    j->t_done = S.now();
-    S.queue.pop_front();
-    S.cv_done.notify_all();
-    return true;
-  }
-  return false;
+  // The head does not fit now.  It waits for running groups to finish and
+  // for clients to release finished jobs, unless it could not fit even in
+  // an empty arena (or cannot be planned at all): then it fails.
+  Job* j = S.queue.front();
+  std::vector<PhasePlan> dry;
+  size_t blk = 0;
+  hipError_t e = group_plan(S, {j}, nullptr, nullptr, dry, &blk);
+  int st = PL_OK;
+  if (e != hipSuccess) st = hip_status(e);
+  else if (Arena::round(j->out_bytes) + Arena::round(blk) > S.arena.cap) st = PL_ERR_NOMEM;
+  if (st == PL_OK) return false;
+  j->status = st;
+  j->done = true;
+  j->t_done = S.now();
+  S.queue.pop_front();
+  S.cv_done.notify_all();
+  return true;
 }
 
 static void worker_main(pl_server* Sp) {

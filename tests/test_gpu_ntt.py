@@ -87,3 +87,41 @@ def test_reference_input_prefix(pl, golden):
     dom = pl.Domain(12)
     y = pl.to_host(pl.ntt(pl.to_dev(raw), dom))
     np.testing.assert_array_equal(y, po.ntt(raw, 12))
+
+
+def test_ntt_roundtrip_2p26(pl):
+    # the reference's largest NTT scale: round trip, and X[k] = x(w^k) at three
+    # k by Horner on the CPU
+    import torch
+    lg = 26
+    n = 1 << lg
+    dom = pl.Domain(lg)
+    x = po.uniform(n, po.seed(po.OP_NTT, lg))
+    xd = pl.to_dev(x)
+    print("ntt roundtrip lg=26", flush=True)
+    y = pl.ntt(xd, dom)
+    assert torch.equal(pl.ntt(y, dom, inverse=True), xd)
+    w = po.root_of_unity(lg)
+    for k in (1, 12345, n - 1):
+        np.testing.assert_array_equal(pl.to_host(y[k:k + 1]), po.eval_(x, po.pow_(w, k)))
+
+
+def test_ntt_roundtrip_2p28(pl):
+    # the largest supported size (8 GiB per vector): round trip, and
+    # X[k] = x(w^k) through the device evaluation (itself checked against the
+    # oracle up to 2^24)
+    import torch
+    lg = 28
+    n = 1 << lg
+    dom = pl.Domain(lg)
+    xd = torch.empty((n, 4), dtype=torch.int64, device="cuda")
+    pl.fill_uniform(xd, po.seed(po.OP_NTT, lg))
+    print("ntt roundtrip lg=28", flush=True)
+    y = pl.ntt(xd, dom)
+    assert torch.equal(pl.ntt(y, dom, inverse=True), xd)
+    w = po.root_of_unity(lg)
+    for k in (1, n // 2 + 3):
+        pt = pl.to_dev(po.pow_(w, k))
+        np.testing.assert_array_equal(pl.to_host(pl.evaluate(xd, pt)), pl.to_host(y[k:k + 1]))
+    del xd, y
+    torch.cuda.empty_cache()

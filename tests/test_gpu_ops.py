@@ -49,27 +49,29 @@ def test_lde_in_place(pl):
     np.testing.assert_array_equal(pl.to_host(buf), po.lde(c, lg_n, lb, po.ref_lde_shift(lb)))
 
 
-@pytest.mark.parametrize("lg_n", [22, 24])
-def test_lde_large_properties(pl, lg_n):
-    # lambda=2 at the benchmark sizes: out[j*4 + t] is the size-N coset NTT
-    # with shift 7^4 * w_M^t; the inverse NTT of one coset column times
-    # shift_t^-i must give back c.  Check t = 0 and t = 3 fully.
+@pytest.mark.parametrize("lg_n,lb", [(22, 2), (24, 2), (24, 3), (24, 4)])
+def test_lde_large_properties(pl, lg_n, lb):
+    # the benchmark sizes: out[j*B + t] is the size-N coset NTT with shift
+    # 7^B * w_M^t (B = 2^lb); the inverse NTT of one coset column gives
+    # c_i * shift_t^i.  Check the first and the last coset at five positions.
     import torch
-    lb = 2
     n = 1 << lg_n
+    B = 1 << lb
     c = po.uniform(n, po.seed(po.OP_LDE, lg_n, lb))
     cd = pl.to_dev(c)
     print("lde large", lg_n, flush=True)
     out = pl.lde(cd, pl.LdeDomain(lg_n, lb))
     dom = pl.Domain(lg_n)
-    for t in (0, 3):
-        col = out[t::4].contiguous()
+    for t in (0, B - 1):
+        col = out[t::B].contiguous()
         back = pl.to_host(pl.ntt(col, dom, inverse=True))
         # back[i] = c_i * s_t^i ; compare at a few positions against the oracle
         s = po.mul(po.ref_lde_shift(lb), po.pow_(po.root_of_unity(lg_n + lb), t))
         for i in (0, 1, 2, n // 3, n - 1):
             np.testing.assert_array_equal(back[i:i + 1], po.mul(c[i:i + 1], po.pow_(s, i)))
+    del out
     torch.cuda.synchronize()
+    torch.cuda.empty_cache()
 
 
 def test_grand_product_golden(pl, golden):

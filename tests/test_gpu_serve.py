@@ -147,3 +147,37 @@ def test_admission_by_footprint(pl):
     srv.close()
     ref = oracle_job(jobs[0])
     assert rs[0][2]["q"] == synth.digest(ref["q"])
+
+
+def test_waits_for_release(pl):
+    # the arena holds one job's outputs: the next job must wait until the
+    # client releases the first (not fail), also when no group is running
+    import threading
+    import time
+    lg, lb = 12, 2
+    one = pl.job_bytes(lg, lb)
+    srv = pl.Server(hbm_bytes=int(one * 1.5) + (8 << 20), slots=1)
+    jobs = [make_job(pl, lg, lb, 8000 + i) for i in range(2)]
+    ids = [srv.submit(lg, lb, j["dev"], j["beta"], j["gamma"], j["zeta"]) for j in jobs]
+    r0 = srv.wait(ids[0])
+    got = {}
+
+    def waiter():
+        try:
+            got["r"] = srv.wait(ids[1])
+        except Exception as e:  # reported below
+            got["e"] = e
+
+    th = threading.Thread(target=waiter)
+    th.start()
+    time.sleep(0.3)            # the server is idle, the arena full of job 0
+    assert th.is_alive() and "e" not in got
+    d0 = digests(pl, r0, lg, lb)
+    srv.release(ids[0])
+    th.join(timeout=60)
+    assert "r" in got, got.get("e")
+    assert got["r"].admit_s >= r0.done_s
+    srv.release(ids[1])
+    srv.close()
+    ref = oracle_job(jobs[0])
+    assert d0["lde"] == synth.digest(ref["lde"])
