@@ -239,7 +239,9 @@ void pl_batch_destroy(pl_batch* b);
  * in arrival order when their output and work blocks fit in it.  Batched
  * mode merges consecutive small jobs into one group, runs a group's
  * transforms as ragged launches and overlaps up to `slots` groups on the
- * device; plain mode runs one job at a time as per-call launch sequences.
+ * device, at most one of them holding a large job (2^22 or more elements per
+ * column; small jobs may start ahead of a large one waiting for that); plain
+ * mode runs one job at a time as per-call launch sequences.
  * pl_server_wait blocks until the job is done and reports the server-clock
  * times (seconds since creation) of its arrival (submit), admission and
  * completion.  A job that can never fit fails with PL_ERR_NOMEM. */

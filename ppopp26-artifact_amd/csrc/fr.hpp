@@ -14,9 +14,10 @@
 //          product scanning (FIPS) over 32-bit limbs.  Each limb product
 //          is one v_mad_u64_u32 (64-bit accumulate, carry-out to an SGPR
 //          lane mask) plus one v_addc_co_u32 folding that carry into a
-//          third accumulator word.  r = 1 (mod 2^32), so the Montgomery
-//          digit is m = -t (mod 2^32) and m*r0 = m needs no multiply.
-//                                                                (reference mont_t.cuh:396-419)
+//          third accumulator word, issued in blocks of three products so
+//          that every carry is read two instructions after it is written.
+//          r = 1 (mod 2^32), so the Montgomery digit is m = -t (mod 2^32)
+//          and m*r0 = m needs no multiply.                        (reference mont_t.cuh:396-419)
 // v_mad_u64_u32 runs at the full VALU rate on gfx950, so the carry handling
 // is as expensive as the multiplies; FIPS keeps it to one op per product.
 #pragma once
@@ -100,29 +101,93 @@ __device__ __forceinline__ bool fr_eq(const fr& a, const fr& b) {
 // mask in an SGPR pair.
 typedef uint64_t lanemask;
 
-// acc(64) += x*y with carry-out of the 64-bit sum
-__device__ __forceinline__ uint64_t mad_co(uint32_t x, uint32_t y, uint64_t acc, lanemask& co) {
-  uint64_t d;
-  asm("v_mad_u64_u32 %0, %1, %2, %3, %4" : "=v"(d), "=s"(co) : "v"(x), "v"(y), "v"(acc));
-  return d;
+// ---- multiply-accumulate blocks --------------------------------------------
+// The column accumulator of the product is (acc: 64 bits, ext: 32 bits).
+// Each limb product is one v_mad_u64_u32 into acc whose carry-out (a lane
+// mask) is folded into ext.  A carry read must come two VALU instructions
+// after its write, so products are issued in blocks of three: three mads
+// (carries to three SGPR pairs), then three folds, each fold exactly two
+// instructions after its mad.  A block is one asm statement, so the wait
+// state the compiler adds after every asm statement is paid once per three
+// products.  Blocks of two and one put s_nop fillers where a third product
+// would be.  FIRST: the block opens a column (ext = first carry).  SY: the y
+// operands are wave-uniform (SGPRs).
+#define PL_MAD3 "v_mad_u64_u32 %0, %2, %6, %7, %0\n\tv_mad_u64_u32 %0, %3, %8, %9, %0\n\tv_mad_u64_u32 %0, %4, %10, %11, %0\n\t"
+#define PL_FOLD3_FIRST "v_cndmask_b32_e64 %1, 0, 1, %2\n\tv_addc_co_u32_e64 %1, %5, %1, 0, %3\n\tv_addc_co_u32_e64 %1, %5, %1, 0, %4"
+#define PL_FOLD3 "v_addc_co_u32_e64 %1, %5, %1, 0, %2\n\tv_addc_co_u32_e64 %1, %5, %1, 0, %3\n\tv_addc_co_u32_e64 %1, %5, %1, 0, %4"
+template <bool FIRST, bool SY>
+__device__ __forceinline__ void mac3(uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1, uint32_t x2, uint32_t y2,
+                                     uint64_t& acc, uint32_t& ext) {
+  lanemask c0, c1, c2, d;
+  if constexpr (FIRST && SY)
+    asm(PL_MAD3 PL_FOLD3_FIRST : "+v"(acc), "=&v"(ext), "=&s"(c0), "=&s"(c1), "=&s"(c2), "=&s"(d)
+        : "v"(x0), "s"(y0), "v"(x1), "s"(y1), "v"(x2), "s"(y2));
+  else if constexpr (FIRST)
+    asm(PL_MAD3 PL_FOLD3_FIRST : "+v"(acc), "=&v"(ext), "=&s"(c0), "=&s"(c1), "=&s"(c2), "=&s"(d)
+        : "v"(x0), "v"(y0), "v"(x1), "v"(y1), "v"(x2), "v"(y2));
+  else if constexpr (SY)
+    asm(PL_MAD3 PL_FOLD3 : "+v"(acc), "+v"(ext), "=&s"(c0), "=&s"(c1), "=&s"(c2), "=&s"(d)
+        : "v"(x0), "s"(y0), "v"(x1), "s"(y1), "v"(x2), "s"(y2));
+  else
+    asm(PL_MAD3 PL_FOLD3 : "+v"(acc), "+v"(ext), "=&s"(c0), "=&s"(c1), "=&s"(c2), "=&s"(d)
+        : "v"(x0), "v"(y0), "v"(x1), "v"(y1), "v"(x2), "v"(y2));
 }
-__device__ __forceinline__ uint64_t mad_co_s(uint32_t x, uint32_t y_uniform, uint64_t acc, lanemask& co) {
-  uint64_t d;
-  asm("v_mad_u64_u32 %0, %1, %2, %3, %4" : "=v"(d), "=s"(co) : "v"(x), "s"(y_uniform), "v"(acc));
-  return d;
+#define PL_MAD2 "v_mad_u64_u32 %0, %2, %5, %6, %0\n\tv_mad_u64_u32 %0, %3, %7, %8, %0\n\ts_nop 0\n\t"
+#define PL_FOLD2_FIRST "v_cndmask_b32_e64 %1, 0, 1, %2\n\tv_addc_co_u32_e64 %1, %4, %1, 0, %3"
+#define PL_FOLD2 "v_addc_co_u32_e64 %1, %4, %1, 0, %2\n\tv_addc_co_u32_e64 %1, %4, %1, 0, %3"
+template <bool FIRST, bool SY>
+__device__ __forceinline__ void mac2(uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1, uint64_t& acc,
+                                     uint32_t& ext) {
+  lanemask c0, c1, d;
+  if constexpr (FIRST && SY)
+    asm(PL_MAD2 PL_FOLD2_FIRST : "+v"(acc), "=&v"(ext), "=&s"(c0), "=&s"(c1), "=&s"(d)
+        : "v"(x0), "s"(y0), "v"(x1), "s"(y1));
+  else if constexpr (FIRST)
+    asm(PL_MAD2 PL_FOLD2_FIRST : "+v"(acc), "=&v"(ext), "=&s"(c0), "=&s"(c1), "=&s"(d)
+        : "v"(x0), "v"(y0), "v"(x1), "v"(y1));
+  else if constexpr (SY)
+    asm(PL_MAD2 PL_FOLD2 : "+v"(acc), "+v"(ext), "=&s"(c0), "=&s"(c1), "=&s"(d) : "v"(x0), "s"(y0), "v"(x1), "s"(y1));
+  else
+    asm(PL_MAD2 PL_FOLD2 : "+v"(acc), "+v"(ext), "=&s"(c0), "=&s"(c1), "=&s"(d) : "v"(x0), "v"(y0), "v"(x1), "v"(y1));
 }
-// ext + carry
-__device__ __forceinline__ uint32_t inc_c(uint32_t ext, lanemask c) {
-  uint32_t d;
-  lanemask dummy;
-  asm("v_addc_co_u32_e64 %0, %1, %2, 0, %3" : "=v"(d), "=s"(dummy) : "v"(ext), "s"(c));
-  return d;
+#define PL_MAD1 "v_mad_u64_u32 %0, %2, %4, %5, %0\n\ts_nop 1\n\t"
+template <bool FIRST, bool SY>
+__device__ __forceinline__ void mac1(uint32_t x0, uint32_t y0, uint64_t& acc, uint32_t& ext) {
+  lanemask c0, d;
+  if constexpr (FIRST && SY)
+    asm(PL_MAD1 "v_cndmask_b32_e64 %1, 0, 1, %2" : "+v"(acc), "=&v"(ext), "=&s"(c0), "=&s"(d) : "v"(x0), "s"(y0));
+  else if constexpr (FIRST)
+    asm(PL_MAD1 "v_cndmask_b32_e64 %1, 0, 1, %2" : "+v"(acc), "=&v"(ext), "=&s"(c0), "=&s"(d) : "v"(x0), "v"(y0));
+  else if constexpr (SY)
+    asm(PL_MAD1 "v_addc_co_u32_e64 %1, %3, %1, 0, %2" : "+v"(acc), "+v"(ext), "=&s"(c0), "=&s"(d) : "v"(x0), "s"(y0));
+  else
+    asm(PL_MAD1 "v_addc_co_u32_e64 %1, %3, %1, 0, %2" : "+v"(acc), "+v"(ext), "=&s"(c0), "=&s"(d) : "v"(x0), "v"(y0));
 }
-// 0/1 from a carry mask
-__device__ __forceinline__ uint32_t bit_c(lanemask c) {
-  uint32_t d;
-  asm("v_cndmask_b32_e64 %0, 0, 1, %1" : "=v"(d) : "s"(c));
-  return d;
+#undef PL_MAD3
+#undef PL_FOLD3_FIRST
+#undef PL_FOLD3
+#undef PL_MAD2
+#undef PL_FOLD2_FIRST
+#undef PL_FOLD2
+#undef PL_MAD1
+
+// L products x(i) * y(i), i < L, into (acc, ext) in blocks of three
+template <int L, bool FIRST, bool SY, typename FX, typename FY>
+__device__ __forceinline__ void mac_run(FX&& fx, FY&& fy, uint64_t& acc, uint32_t& ext) {
+  static_for<0, L / 3>([&](auto b_) {
+    constexpr int i = 3 * decltype(b_)::value;
+    mac3<FIRST && i == 0, SY>(fx(std::integral_constant<int, i>{}), fy(std::integral_constant<int, i>{}),
+                              fx(std::integral_constant<int, i + 1>{}), fy(std::integral_constant<int, i + 1>{}),
+                              fx(std::integral_constant<int, i + 2>{}), fy(std::integral_constant<int, i + 2>{}), acc,
+                              ext);
+  });
+  constexpr int R = L % 3, i = L - R;
+  constexpr bool RF = FIRST && i == 0;
+  if constexpr (R == 2)
+    mac2<RF, SY>(fx(std::integral_constant<int, i>{}), fy(std::integral_constant<int, i>{}),
+                 fx(std::integral_constant<int, i + 1>{}), fy(std::integral_constant<int, i + 1>{}), acc, ext);
+  else if constexpr (R == 1)
+    mac1<RF, SY>(fx(std::integral_constant<int, i>{}), fy(std::integral_constant<int, i>{}), acc, ext);
 }
 
 // ---- field ops -----------------------------------------------------------
@@ -175,48 +240,34 @@ __device__ __forceinline__ fr fr_mul_t(const fr& a, const fr& b) {
   uint32_t m[8];
   uint64_t acc = 0;
   uint32_t ext = 0;
-  bool ext_zero = true;   // compile-time after unrolling
-  lanemask c;
-#define PL_MAC(ACC_EXPR)                                   \
-  do {                                                     \
-    acc = (ACC_EXPR);                                      \
-    ext = ext_zero ? bit_c(c) : inc_c(ext, c);             \
-    ext_zero = false;                                      \
-  } while (0)
-#pragma unroll
-  for (int k = 0; k < 8; k++) {
-#pragma unroll
-    for (int i = 0; i < k; i++)            // m_i * r_{k-i}, r index >= 1
-      PL_MAC(mad_co_s(m[i], r_limb(k - i), acc, c));
-#pragma unroll
-    for (int i = 0; i <= k; i++)           // a_i * b_{k-i}
-      PL_MAC(UB ? mad_co_s(a.l[i], b.l[k - i], acc, c) : mad_co(a.l[i], b.l[k - i], acc, c));
+  static_for<0, 8>([&](auto k_) {
+    constexpr int k = decltype(k_)::value;
+    // m_i * r_{k-i} (i < k, r index >= 1), then a_i * b_{k-i} (i <= k)
+    mac_run<k, true, true>([&](auto i_) { return m[decltype(i_)::value]; },
+                           [&](auto i_) { return r_limb(k - decltype(i_)::value); }, acc, ext);
+    mac_run<k + 1, k == 0, UB>([&](auto i_) { return a.l[decltype(i_)::value]; },
+                               [&](auto i_) { return b.l[k - decltype(i_)::value]; }, acc, ext);
     // Montgomery digit: m_k = -lo (mod 2^32) since -r^-1 = -1 (mod 2^32).
     // lo + m_k*r0 = lo + m_k = 0 (mod 2^32) with carry (lo != 0).
-    uint32_t lo = (uint32_t)acc;
-    uint32_t hi = (uint32_t)(acc >> 32);
+    const uint32_t lo = (uint32_t)acc;
+    const uint32_t hi = (uint32_t)(acc >> 32);
     m[k] = 0u - lo;
     uint32_t c1, c2;
     (void)__builtin_addc(lo, m[k], 0u, &c1);
-    uint32_t nhi = __builtin_addc(hi, 0u, c1, &c2);
-    uint32_t next = ext_zero ? c2 : ext + c2;
-    acc = ((uint64_t)next << 32) | nhi;
-    ext_zero = true;
-  }
+    const uint32_t nhi = __builtin_addc(hi, 0u, c1, &c2);
+    acc = ((uint64_t)(ext + c2) << 32) | nhi;
+  });
   fr t;
-#pragma unroll
-  for (int k = 8; k < 15; k++) {
-#pragma unroll
-    for (int i = k - 7; i < 8; i++)
-      PL_MAC(mad_co_s(m[i], r_limb(k - i), acc, c));
-#pragma unroll
-    for (int i = k - 7; i < 8; i++)
-      PL_MAC(UB ? mad_co_s(a.l[i], b.l[k - i], acc, c) : mad_co(a.l[i], b.l[k - i], acc, c));
+  static_for<8, 15>([&](auto k_) {
+    constexpr int k = decltype(k_)::value;
+    constexpr int i0 = k - 7;  // i in [k-7, 8)
+    mac_run<15 - k, true, true>([&](auto i_) { return m[i0 + decltype(i_)::value]; },
+                                [&](auto i_) { return r_limb(k - i0 - decltype(i_)::value); }, acc, ext);
+    mac_run<15 - k, false, UB>([&](auto i_) { return a.l[i0 + decltype(i_)::value]; },
+                               [&](auto i_) { return b.l[k - i0 - decltype(i_)::value]; }, acc, ext);
     t.l[k - 8] = (uint32_t)acc;
     acc = ((uint64_t)ext << 32) | (uint32_t)(acc >> 32);
-    ext_zero = true;
-  }
-#undef PL_MAC
+  });
   t.l[7] = (uint32_t)acc;
   // result < 2r < 2^256, so the bits above limb 7 are zero here
   return fr_reduce_once(t);

@@ -17,7 +17,9 @@
 // and coset tables are cached per size for the life of the process.
 //
 // Scheduling (batched mode): a worker thread admits jobs whenever one of
-// the slots (a pair of streams) is free.  Consecutive small queued jobs are
+// the slots (a pair of streams) is free; at most one group holding a large
+// job (2^22 or more elements per column) runs at a time, and small jobs may
+// start ahead of large ones that wait only for that reason.  Consecutive small queued jobs are
 // merged into one group; a group runs as phases planned by the phase planner:
 // phase A = all iNTTs, phase B = all LDEs (ragged launches on the main
 // stream) beside the evaluation/division and grand-product scans (side
@@ -95,6 +97,7 @@ struct Job {
   fr *coeffs = nullptr, *lde = nullptr, *z = nullptr, *y = nullptr, *q = nullptr;
   uint64_t n() const { return 1ull << d.lg; }
   uint64_t m() const { return 1ull << (d.lg + d.lg_blowup); }
+  bool large() const { return n() >= (1ull << 22); }  // one column fills the chip
 };
 
 static size_t job_out_bytes(int lg, int lb) {
@@ -317,12 +320,25 @@ static bool try_admit(pl_server& S) {
     }
   if (!slot) return false;
   const bool plain = S.cfg.mode == PL_SERVE_PLAIN;
-  // candidate group: the head, then following small jobs
-  std::vector<Job*> cand{S.queue.front()};
+  // At most one group with a large job (one whose single transforms fill
+  // the chip) runs at a time: two of them side by side only stretch each
+  // other.  While one runs, small jobs may start ahead of queued large ones;
+  // otherwise admission is in arrival order.
+  bool large_running = false;
+  for (auto& sl : S.slots)
+    if (sl.busy)
+      for (Job* j : sl.jobs) large_running |= j->large();
+  size_t first = 0;
+  if (!plain && large_running) {
+    while (first < S.queue.size() && S.queue[first]->large()) first++;
+    if (first == S.queue.size()) return false;
+  }
+  // candidate group: that job, then following small jobs
+  std::vector<Job*> cand{S.queue[first]};
   auto small = [&](const Job* j) { return 8 * j->n() <= S.cfg.group_elems; };
   if (!plain && small(cand[0])) {
     uint64_t elems = 8 * cand[0]->n();
-    for (size_t i = 1; i < S.queue.size() && (int)cand.size() < S.cfg.max_group_jobs; i++) {
+    for (size_t i = first + 1; i < S.queue.size() && (int)cand.size() < S.cfg.max_group_jobs; i++) {
       Job* j = S.queue[i];
       if (!small(j) || elems + 8 * j->n() > S.cfg.group_elems) break;
       elems += 8 * j->n();
@@ -371,7 +387,7 @@ static bool try_admit(pl_server& S) {
         }
         S.arena.release(blk_off, blk_bytes);
         for (size_t i = 0; i < cand.size(); i++) S.arena.release(offs[i], cand[i]->out_bytes);
-        S.queue.erase(S.queue.begin(), S.queue.begin() + cand.size());
+        S.queue.erase(S.queue.begin() + first, S.queue.begin() + first + cand.size());
         S.cv_done.notify_all();
         return true;
       }
@@ -379,7 +395,7 @@ static bool try_admit(pl_server& S) {
       slot->jobs = cand;
       slot->blk_off = blk_off;
       slot->blk_bytes = blk_bytes;
-      S.queue.erase(S.queue.begin(), S.queue.begin() + cand.size());
+      S.queue.erase(S.queue.begin() + first, S.queue.begin() + first + cand.size());
       S.n_groups++;
       return true;
     }
@@ -387,6 +403,7 @@ static bool try_admit(pl_server& S) {
     if (cand.size() == 1) break;
     cand.pop_back();  // try a smaller group
   }
+  if (first != 0) return false;  // a small job ahead of large ones waits for memory
   // The head does not fit now.  It waits for running groups to finish and
   // for clients to release finished jobs, unless it could not fit even in
   // an empty arena (or cannot be planned at all): then it fails.

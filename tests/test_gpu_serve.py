@@ -156,7 +156,9 @@ def test_waits_for_release(pl):
     import time
     lg, lb = 12, 2
     one = pl.job_bytes(lg, lb)
-    srv = pl.Server(hbm_bytes=int(one * 1.5) + (8 << 20), slots=1)
+    # one job's outputs + its work block (~0.75 of the outputs) fit, two
+    # jobs' outputs do not
+    srv = pl.Server(hbm_bytes=int(one * 1.9), slots=1)
     jobs = [make_job(pl, lg, lb, 8000 + i) for i in range(2)]
     ids = [srv.submit(lg, lb, j["dev"], j["beta"], j["gamma"], j["zeta"]) for j in jobs]
     r0 = srv.wait(ids[0])
