@@ -101,105 +101,17 @@ __device__ __forceinline__ bool fr_eq(const fr& a, const fr& b) {
 // mask in an SGPR pair.
 typedef uint64_t lanemask;
 
-// ---- multiply-accumulate blocks --------------------------------------------
+// ---- multiply-accumulate columns ------------------------------------------
 // The column accumulator of the product is (acc: 64 bits, ext: 32 bits).
 // Each limb product is one v_mad_u64_u32 into acc whose carry-out (a lane
-// mask) is folded into ext.  A carry read must come two VALU instructions
-// after its write, so products are issued in blocks of three: three mads
-// (carries to three SGPR pairs), then three folds, each fold exactly two
-// instructions after its mad.  A block is one asm statement, so the wait
-// state the compiler adds after every asm statement is paid once per three
-// products.  Blocks of two and one put s_nop fillers where a third product
-// would be.  FIRST: the block opens a column (ext = first carry).  SY: the y
-// operands are wave-uniform (SGPRs).
-#define PL_MAD3 "v_mad_u64_u32 %0, %2, %6, %7, %0\n\tv_mad_u64_u32 %0, %3, %8, %9, %0\n\tv_mad_u64_u32 %0, %4, %10, %11, %0\n\t"
-#define PL_FOLD3_FIRST "v_cndmask_b32_e64 %1, 0, 1, %2\n\tv_addc_co_u32_e64 %1, %5, %1, 0, %3\n\tv_addc_co_u32_e64 %1, %5, %1, 0, %4"
-#define PL_FOLD3 "v_addc_co_u32_e64 %1, %5, %1, 0, %2\n\tv_addc_co_u32_e64 %1, %5, %1, 0, %3\n\tv_addc_co_u32_e64 %1, %5, %1, 0, %4"
-#define PL_MAC3(FOLD, EXT, K0, K1, K2)                                                          \
-  asm(PL_MAD3 FOLD : "+v"(acc), EXT(ext), "=&s"(c0), "=&s"(c1), "=&s"(c2), "=&s"(d)                \
-      : "v"(x0), K0(y0), "v"(x1), K1(y1), "v"(x2), K2(y2))
-#define PL_MAC3_ALL(FOLD, EXT)                                              \
-  if constexpr (S0 && S1 && S2) PL_MAC3(FOLD, EXT, "s", "s", "s");           \
-  else if constexpr (S0 && S1) PL_MAC3(FOLD, EXT, "s", "s", "v");            \
-  else if constexpr (S0 && S2) PL_MAC3(FOLD, EXT, "s", "v", "s");            \
-  else if constexpr (S0) PL_MAC3(FOLD, EXT, "s", "v", "v");                  \
-  else if constexpr (S1 && S2) PL_MAC3(FOLD, EXT, "v", "s", "s");            \
-  else if constexpr (S1) PL_MAC3(FOLD, EXT, "v", "s", "v");                  \
-  else if constexpr (S2) PL_MAC3(FOLD, EXT, "v", "v", "s");                  \
-  else PL_MAC3(FOLD, EXT, "v", "v", "v");
-// S0..S2: y operand i is wave-uniform (SGPR)
-template <bool FIRST, bool S0, bool S1, bool S2>
-__device__ __forceinline__ void mac3(uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1, uint32_t x2, uint32_t y2,
-                                     uint64_t& acc, uint32_t& ext) {
-  lanemask c0, c1, c2, d;
-  if constexpr (FIRST) {
-    PL_MAC3_ALL(PL_FOLD3_FIRST, "=&v")
-  } else {
-    PL_MAC3_ALL(PL_FOLD3, "+v")
-  }
-}
-#define PL_MAD2 "v_mad_u64_u32 %0, %2, %5, %6, %0\n\tv_mad_u64_u32 %0, %3, %7, %8, %0\n\ts_nop 0\n\t"
-#define PL_FOLD2_FIRST "v_cndmask_b32_e64 %1, 0, 1, %2\n\tv_addc_co_u32_e64 %1, %4, %1, 0, %3"
-#define PL_FOLD2 "v_addc_co_u32_e64 %1, %4, %1, 0, %2\n\tv_addc_co_u32_e64 %1, %4, %1, 0, %3"
-#define PL_MAC2(FOLD, EXT, K0, K1) \
-  asm(PL_MAD2 FOLD : "+v"(acc), EXT(ext), "=&s"(c0), "=&s"(c1), "=&s"(d) : "v"(x0), K0(y0), "v"(x1), K1(y1))
-#define PL_MAC2_ALL(FOLD, EXT)                                   \
-  if constexpr (S0 && S1) PL_MAC2(FOLD, EXT, "s", "s");          \
-  else if constexpr (S0) PL_MAC2(FOLD, EXT, "s", "v");           \
-  else if constexpr (S1) PL_MAC2(FOLD, EXT, "v", "s");           \
-  else PL_MAC2(FOLD, EXT, "v", "v");
-template <bool FIRST, bool S0, bool S1>
-__device__ __forceinline__ void mac2(uint32_t x0, uint32_t y0, uint32_t x1, uint32_t y1, uint64_t& acc,
-                                     uint32_t& ext) {
-  lanemask c0, c1, d;
-  if constexpr (FIRST) {
-    PL_MAC2_ALL(PL_FOLD2_FIRST, "=&v")
-  } else {
-    PL_MAC2_ALL(PL_FOLD2, "+v")
-  }
-}
-#define PL_MAD1 "v_mad_u64_u32 %0, %2, %4, %5, %0\n\ts_nop 1\n\t"
-#define PL_MAC1(FOLD, EXT, K0) asm(PL_MAD1 FOLD : "+v"(acc), EXT(ext), "=&s"(c0), "=&s"(d) : "v"(x0), K0(y0))
-template <bool FIRST, bool S0>
-__device__ __forceinline__ void mac1(uint32_t x0, uint32_t y0, uint64_t& acc, uint32_t& ext) {
-  lanemask c0, d;
-  if constexpr (FIRST && S0) PL_MAC1("v_cndmask_b32_e64 %1, 0, 1, %2", "=&v", "s");
-  else if constexpr (FIRST) PL_MAC1("v_cndmask_b32_e64 %1, 0, 1, %2", "=&v", "v");
-  else if constexpr (S0) PL_MAC1("v_addc_co_u32_e64 %1, %3, %1, 0, %2", "+v", "s");
-  else PL_MAC1("v_addc_co_u32_e64 %1, %3, %1, 0, %2", "+v", "v");
-}
-#undef PL_MAD3
-#undef PL_FOLD3_FIRST
-#undef PL_FOLD3
-#undef PL_MAC3
-#undef PL_MAC3_ALL
-#undef PL_MAD2
-#undef PL_FOLD2_FIRST
-#undef PL_FOLD2
-#undef PL_MAC2
-#undef PL_MAC2_ALL
-#undef PL_MAD1
-#undef PL_MAC1
-
-// L products x(i) * y(i), i < L, opening a column, into (acc, ext) in
-// blocks of three; sy(i) tells whether y(i) is wave-uniform
-template <int L, typename FX, typename FY, typename SYF>
-__device__ __forceinline__ void mac_column(FX&& fx, FY&& fy, SYF sy, uint64_t& acc, uint32_t& ext) {
-  using std::integral_constant;
-  static_for<0, L / 3>([&](auto b_) {
-    constexpr int i = 3 * decltype(b_)::value;
-    mac3<i == 0, sy(i), sy(i + 1), sy(i + 2)>(fx(integral_constant<int, i>{}), fy(integral_constant<int, i>{}),
-                                               fx(integral_constant<int, i + 1>{}), fy(integral_constant<int, i + 1>{}),
-                                               fx(integral_constant<int, i + 2>{}), fy(integral_constant<int, i + 2>{}),
-                                               acc, ext);
-  });
-  constexpr int R = L % 3, i = L - R;
-  if constexpr (R == 2)
-    mac2<i == 0, sy(i), sy(i + 1)>(fx(integral_constant<int, i>{}), fy(integral_constant<int, i>{}),
-                                   fx(integral_constant<int, i + 1>{}), fy(integral_constant<int, i + 1>{}), acc, ext);
-  else if constexpr (R == 1)
-    mac1<i == 0, sy(i)>(fx(integral_constant<int, i>{}), fy(integral_constant<int, i>{}), acc, ext);
-}
+// mask) is folded into ext.  A carry must be read at least two VALU
+// instructions after it is written; each column is one asm statement,
+// software-pipelined over three carry registers so that every fold lands
+// two or more instructions after its mad with no filler, and the wait state
+// the compiler places after every asm statement is paid once per column.
+// MontCol<K, UB>::run adds column K's products (m_i r_{K-i}, then a_i b_{K-i})
+// and opens the column (ext = first carry); UB: b is wave-uniform (SGPRs).
+#include "mont_columns.inc"
 
 // ---- field ops -----------------------------------------------------------
 
@@ -253,22 +165,7 @@ __device__ __forceinline__ fr fr_mul_t(const fr& a, const fr& b) {
   uint32_t ext = 0;
   static_for<0, 8>([&](auto k_) {
     constexpr int k = decltype(k_)::value;
-    // column k: m_i * r_{k-i} (i < k, r index >= 1), then a_i * b_{k-i} (i <= k)
-    struct SY {
-      constexpr bool operator()(int j) const { return j < k ? true : UB; }
-    };
-    mac_column<2 * k + 1>(
-        [&](auto j_) {
-          constexpr int j = decltype(j_)::value;
-          if constexpr (j < k) return m[j];
-          else return a.l[j - k];
-        },
-        [&](auto j_) {
-          constexpr int j = decltype(j_)::value;
-          if constexpr (j < k) return r_limb(k - j);
-          else return b.l[2 * k - j];
-        },
-        SY{}, acc, ext);
+    MontCol<k, UB>::run(acc, ext, m, a.l, b.l);
     // Montgomery digit: m_k = -lo (mod 2^32) since -r^-1 = -1 (mod 2^32);
     // lo + m_k*r0 = lo + m_k = 2^32 * (lo != 0), so the next column starts
     // at (ext:hi) + (lo != 0) -- a 64-bit add, no carry flags
@@ -279,22 +176,7 @@ __device__ __forceinline__ fr fr_mul_t(const fr& a, const fr& b) {
   fr t;
   static_for<8, 15>([&](auto k_) {
     constexpr int k = decltype(k_)::value;
-    constexpr int i0 = k - 7, L = 15 - k;  // i in [k-7, 8)
-    struct SY {
-      constexpr bool operator()(int j) const { return j < L ? true : UB; }
-    };
-    mac_column<2 * L>(
-        [&](auto j_) {
-          constexpr int j = decltype(j_)::value;
-          if constexpr (j < L) return m[i0 + j];
-          else return a.l[i0 + j - L];
-        },
-        [&](auto j_) {
-          constexpr int j = decltype(j_)::value;
-          if constexpr (j < L) return r_limb(k - i0 - j);
-          else return b.l[k - i0 - (j - L)];
-        },
-        SY{}, acc, ext);
+    MontCol<k, UB>::run(acc, ext, m, a.l, b.l);
     t.l[k - 8] = (uint32_t)acc;
     acc = ((uint64_t)ext << 32) | (uint32_t)(acc >> 32);
   });

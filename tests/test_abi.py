@@ -110,3 +110,28 @@ def test_batch_argument_errors_without_device():
     assert L.pl_batch_info(None, None, None) == -1
     assert L.pl_batch_run(None, None) == -1
     L.pl_batch_destroy(None)
+
+
+def test_generated_montgomery_columns_current():
+    # mont_columns.inc is generated; it must match its generator
+    gen = os.path.join(ROOT, "ppopp26-artifact_amd", "csrc", "gen_mont_columns.py")
+    inc = os.path.join(ROOT, "ppopp26-artifact_amd", "csrc", "mont_columns.inc")
+    out = subprocess.check_output(["python3", gen]).decode()
+    assert out == open(inc).read()
+    # every fold reads its carry two or more instructions after the mad wrote it
+    import re
+    for blk in re.findall(r'asm\("([^"]*)"', out):
+        ins = blk.split("\\n\\t")
+        written = {}
+        for pos, t in enumerate(ins):
+            if t.startswith("s_nop"):
+                continue
+            m = re.match(r"v_mad_u64_u32 %0, (%\d+),", t)
+            if m:
+                written[m.group(1)] = pos
+                continue
+            c = t.split(", ")[-1]
+            assert c in written, t
+            nops = sum(int(x.split()[1]) + 1 for x in ins[written[c] + 1:pos] if x.startswith("s_nop"))
+            valu = sum(1 for x in ins[written[c] + 1:pos] if not x.startswith("s_nop"))
+            assert valu + nops >= 2, blk
