@@ -58,6 +58,7 @@ def parse():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--serve-jobs", type=int, default=40, help="jobs per served run")
     ap.add_argument("--serve-slots", type=int, default=3, help="job groups running at once (batched form)")
+    ap.add_argument("--serve-large", type=int, default=25, help="log2 of the work that makes a job large")
     ap.add_argument("--plans", default=os.path.join(ROOT, "ppopp26-artifact_amd", "tune", "mi355x_plans.txt"),
                     help="autotuned pass-split cache (measured and written when absent)")
     ap.add_argument("--no-tune", action="store_true", help="default pass splits")
@@ -236,11 +237,11 @@ def bench_batch(pl, synth, steps, stream):
     }
 
 
-def run_server(pl, mode, jobs, times, pool, scal, verify, slots=0):
+def run_server(pl, mode, jobs, times, pool, scal, verify, slots=0, large_lg=0):
     """Submit jobs at their arrival times (seconds, None = all at once);
     returns per-job (arrival, admit, done) and digests."""
     import torch
-    srv = pl.Server(mode=mode, slots=slots)
+    srv = pl.Server(mode=mode, slots=slots, large_work=(1 << large_lg) if large_lg else 0)
     res = [None] * len(jobs)
     q = queue.Queue()
     err = []
@@ -290,7 +291,7 @@ def summarize(jobs, res):
             "elems_per_s": round(elems / span, 1), "span_s": round(span, 3)}
 
 
-def bench_served(pl, synth, n_jobs, slots):
+def bench_served(pl, synth, n_jobs, slots, large_lg):
     import torch
     # resident inputs: 8 columns per size (jobs of one size share columns,
     # each job has its own beta, gamma, zeta)
@@ -309,7 +310,7 @@ def bench_served(pl, synth, n_jobs, slots):
     jobs0 = [(lg, lb) for lg, lb, _ in wl[0]]
     # capacity: every job queued at once; outputs verified between the forms
     sat_p, _ = run_server(pl, pl.SERVE_PLAIN, jobs0, None, pool, scal[0], True)
-    sat_b, st_b = run_server(pl, pl.SERVE_BATCHED, jobs0, None, pool, scal[0], True, slots)
+    sat_b, st_b = run_server(pl, pl.SERVE_BATCHED, jobs0, None, pool, scal[0], True, slots, large_lg)
     exact = all(a[3] == b[3] for a, b in zip(sat_p, sat_b))
     progress("served: saturation runs done, bit-exact %s" % exact)
     cap_p = summarize(jobs0, sat_p)
@@ -321,7 +322,7 @@ def bench_served(pl, synth, n_jobs, slots):
         rate = rel * plain_jobs_per_s
         times = [t / rate for _, _, t in wl[ri]]
         rp, _ = run_server(pl, pl.SERVE_PLAIN, jobs, times, pool, scal[ri], False)
-        rb, _ = run_server(pl, pl.SERVE_BATCHED, jobs, times, pool, scal[ri], False, slots)
+        rb, _ = run_server(pl, pl.SERVE_BATCHED, jobs, times, pool, scal[ri], False, slots, large_lg)
         sp, sb = summarize(jobs, rp), summarize(jobs, rb)
         progress("served: rate %.1f jobs/s: p99 batched %.1f ms, plain %.1f ms" % (rate, sb["p99_ms"], sp["p99_ms"]))
         rates.append({"rate_jobs_per_s": round(rate, 2), "load_vs_plain_capacity": rel,
@@ -340,7 +341,7 @@ def bench_served(pl, synth, n_jobs, slots):
         "plain_saturated_p50_ms": cap_p["p50_ms"], "plain_saturated_p99_ms": cap_p["p99_ms"],
         "poisson": rates,
         "bit_exact_vs_plain": exact,
-        "slots": slots, "groups": st_b["groups"], "arena_peak_gib": round(st_b["arena_peak"] / 2**30, 2),
+        "slots": slots, "large_work_lg": large_lg, "groups": st_b["groups"], "arena_peak_gib": round(st_b["arena_peak"] / 2**30, 2),
         "arena_gib": round(st_b["arena_bytes"] / 2**30, 1), "domain_cache_gib": round(st_b["domain_bytes"] / 2**30, 2),
     }
 
@@ -421,7 +422,7 @@ def main():
         progress("batched: %.3f ms vs plain %.3f ms" % (result["batched"]["p50_ms"], result["batched"]["plain_p50_ms"]))
         torch.cuda.empty_cache()
     if not args.no_serve:
-        result["served"] = bench_served(pl, synth, args.serve_jobs, args.serve_slots)
+        result["served"] = bench_served(pl, synth, args.serve_jobs, args.serve_slots, args.serve_large)
         torch.cuda.empty_cache()
     if not args.no_cpu:
         progress("cpu baseline (one core, ~15 s)")

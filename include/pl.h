@@ -239,8 +239,9 @@ void pl_batch_destroy(pl_batch* b);
  * in arrival order when their output and work blocks fit in it.  Batched
  * mode merges consecutive small jobs into one group, runs a group's
  * transforms as ragged launches and overlaps up to `slots` groups on the
- * device, at most one of them holding a large job (2^22 or more elements per
- * column; small jobs may start ahead of a large one waiting for that); plain
+ * device, at most one of them holding a large job (transform work
+ * 8 N (1 + 2^lg_blowup) >= large_work; small jobs may start ahead of a large
+ * one waiting for that); plain
  * mode runs one job at a time as per-call launch sequences.
  * pl_server_wait blocks until the job is done and reports the server-clock
  * times (seconds since creation) of its arrival (submit), admission and
@@ -252,6 +253,7 @@ typedef struct pl_server_config {
   int mode;              /* pl_serve_mode */
   int max_group_jobs;    /* jobs merged into one group; 0 = 16 */
   uint64_t group_elems;  /* merge while a group's 8 N sum stays <= this; 0 = 2^22 */
+  uint64_t large_work;   /* a job is large when 8 N (1 + 2^lg_blowup) >= this; 0 = 2^25 */
 } pl_server_config;
 typedef struct pl_job_desc {
   int lg;         /* 1..27 */

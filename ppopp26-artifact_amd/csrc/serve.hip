@@ -18,8 +18,8 @@
 //
 // Scheduling (batched mode): a worker thread admits jobs whenever one of
 // the slots (a pair of streams) is free; at most one group holding a large
-// job (2^22 or more elements per column) runs at a time, and small jobs may
-// start ahead of large ones that wait only for that reason.  Consecutive small queued jobs are
+// job (transform work 8 N (1 + 2^lg_blowup) >= large_work) runs at a time,
+// and small jobs may start ahead of large ones that wait only for that.  Consecutive small queued jobs are
 // merged into one group; a group runs as phases planned by the phase planner:
 // phase A = all iNTTs, phase B = all LDEs (ragged launches on the main
 // stream) beside the evaluation/division and grand-product scans (side
@@ -97,7 +97,7 @@ struct Job {
   fr *coeffs = nullptr, *lde = nullptr, *z = nullptr, *y = nullptr, *q = nullptr;
   uint64_t n() const { return 1ull << d.lg; }
   uint64_t m() const { return 1ull << (d.lg + d.lg_blowup); }
-  bool large() const { return n() >= (1ull << 22); }  // one column fills the chip
+  uint64_t work() const { return 8 * (n() + m()); }  // transform elements
 };
 
 static size_t job_out_bytes(int lg, int lb) {
@@ -327,10 +327,10 @@ static bool try_admit(pl_server& S) {
   bool large_running = false;
   for (auto& sl : S.slots)
     if (sl.busy)
-      for (Job* j : sl.jobs) large_running |= j->large();
+      for (Job* j : sl.jobs) large_running |= j->work() >= S.cfg.large_work;
   size_t first = 0;
   if (!plain && large_running) {
-    while (first < S.queue.size() && S.queue[first]->large()) first++;
+    while (first < S.queue.size() && S.queue[first]->work() >= S.cfg.large_work) first++;
     if (first == S.queue.size()) return false;
   }
   // candidate group: that job, then following small jobs
@@ -488,6 +488,7 @@ int pl_server_create(const pl_server_config* cfg, pl_server** out) {
   if (c.mode == PL_SERVE_PLAIN) c.slots = 1;
   if (c.max_group_jobs == 0) c.max_group_jobs = 16;
   if (c.group_elems == 0) c.group_elems = 1ull << 22;
+  if (c.large_work == 0) c.large_work = 1ull << 25;
   int ndev = 0;
   if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return PL_ERR_NO_DEVICE;
   if (c.hbm_bytes == 0) {

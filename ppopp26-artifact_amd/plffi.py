@@ -92,7 +92,7 @@ SERVE_BATCHED, SERVE_PLAIN = 0, 1
 class ServerConfig(ctypes.Structure):
     """pl_server_config"""
     _fields_ = [("hbm_bytes", ctypes.c_size_t), ("slots", _i), ("mode", _i), ("max_group_jobs", _i),
-                ("group_elems", _u64)]
+                ("group_elems", _u64), ("large_work", _u64)]
 
 
 class JobDesc(ctypes.Structure):
@@ -347,8 +347,8 @@ class Server:
     """Request loop for proof jobs (pl_server_*).  Inputs are device tensors
     kept referenced until the job is waited for."""
 
-    def __init__(self, hbm_bytes=0, slots=0, mode=SERVE_BATCHED, max_group_jobs=0, group_elems=0):
-        cfg = ServerConfig(hbm_bytes, slots, mode, max_group_jobs, group_elems)
+    def __init__(self, hbm_bytes=0, slots=0, mode=SERVE_BATCHED, max_group_jobs=0, group_elems=0, large_work=0):
+        cfg = ServerConfig(hbm_bytes, slots, mode, max_group_jobs, group_elems, large_work)
         self.h = ctypes.c_void_p()
         self._keep = {}
         check("pl_server_create", lib().pl_server_create(ctypes.byref(cfg), ctypes.byref(self.h)))
