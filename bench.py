@@ -56,6 +56,7 @@ def parse():
     ap.add_argument("--no-lde", action="store_true")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
+    ap.add_argument("--no-miss", action="store_true", help="skip grand product / evaluation / division")
     ap.add_argument("--serve-jobs", type=int, default=40, help="jobs per served run")
     ap.add_argument("--serve-slots", type=int, default=3, help="job groups running at once (batched form)")
     ap.add_argument("--serve-large", type=int, default=25, help="log2 of the work that makes a job large")
@@ -237,6 +238,50 @@ def bench_batch(pl, synth, steps, stream):
     }
 
 
+def bench_miss_ops(pl, synth, steps, stream, peak):
+    """The miss_ops workload (grand product with the permutation ratio,
+    evaluation, division by X - z) at 2^20 and 2^24, device-resident."""
+    import torch
+    out = {}
+    # Montgomery forms of the coset constants 1, 7, 13, computed on the device
+    kraw = torch.zeros((3, 4), dtype=torch.int64, device="cuda")
+    kraw[0, 0], kraw[1, 0], kraw[2, 0] = 1, 7, 13
+    ks = pl.to_host(pl.vec_op("to_mont", kraw))
+    for lg in (20, 24):
+        n = 1 << lg
+        dom = pl.Domain(lg)
+        cols = [dev_uniform(pl, n, synth.seed(synth.OP_GP, lg, 10 + j)) for j in range(6)]
+        bg = dev_uniform(pl, 2, synth.seed(synth.OP_GP, lg, 20))
+        beta, gamma = bg[0:1], bg[1:2]
+        z = torch.empty_like(cols[0])
+        wv = torch.empty((pl.lib().pl_vec_work_bytes(n) + 31) // 32, 4, dtype=torch.int64, device="cuda")
+        gp = lambda: pl.perm_ratio(cols[:3], cols[3:], beta, gamma, ks, dom, out=z, grand_product=True,  # noqa: E731
+                                   work=wv)
+        poly = dev_uniform(pl, n, synth.seed(synth.OP_EVAL, lg))
+        pt = dev_uniform(pl, 1, synth.seed(synth.OP_EVAL, lg, 1))
+        y = torch.empty((1, 4), dtype=torch.int64, device="cuda")
+        sw = torch.empty((pl.lib().pl_scan_work_bytes(n) + 31) // 32, 4, dtype=torch.int64, device="cuda")
+        ev = lambda: pl.evaluate(poly, pt, y=y, work=sw)  # noqa: E731
+        q = poly.clone()
+        dv = lambda: pl.div_linear(q, pt, work=sw)  # noqa: E731 (in place; timing only)
+        res = {}
+        for name, fn, mm, byts in [("perm_grand_product", gp, 17 * n, 224 * n), ("eval", ev, n, 32 * n),
+                                   ("div", dv, n, 64 * n)]:
+            for _ in range(3):
+                fn()
+            torch.cuda.synchronize()
+            ms = time_events(fn, steps, stream)
+            t_roof = max(byts / PEAK_HBM, mm * MAD_PER_MULMOD / peak)
+            res[name] = {"ms": round(ms, 4), "elems_per_s": round(n / (ms * 1e-3), 1),
+                         "roofline_frac": round(t_roof / (ms * 1e-3), 4)}
+        out["2^%d" % lg] = res
+        del cols, z, wv, poly, q, sw
+        torch.cuda.empty_cache()
+    out["roofline"] = ("max(bytes / 8 TB/s, 128 MACs x mulmods / measured P_mac); mulmods 17 N (perm GP), "
+                       "N (eval), N (div); bytes 224 N, 32 N, 64 N")
+    return out
+
+
 def run_server(pl, mode, jobs, times, pool, scal, verify, slots=0, large_lg=0):
     """Submit jobs at their arrival times (seconds, None = all at once);
     returns per-job (arrival, admit, done) and digests."""
@@ -416,6 +461,10 @@ def main():
     if not args.no_lde:
         result["lde"] = bench_lde(pl, synth, lg, 2, args.steps, args.warmup, stream, peak)
         progress("lde: %.3f ms" % result["lde"]["ms"])
+        torch.cuda.empty_cache()
+    if not args.no_miss:
+        result["miss_ops"] = bench_miss_ops(pl, synth, args.steps, stream, peak)
+        progress("miss_ops: %s" % json.dumps(result["miss_ops"]["2^20"]))
         torch.cuda.empty_cache()
     if not args.no_batch:
         result["batched"] = bench_batch(pl, synth, args.steps, stream)

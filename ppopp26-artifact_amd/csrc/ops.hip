@@ -182,14 +182,23 @@ __device__ fr block_scan_prod(fr v, uint4 (*l)[kOpThreads], bool suffix) {
 // Phase 1: total[b] = product over block b of in[i] (zeros taken as one)
 __global__ void __launch_bounds__(kOpThreads) inv_reduce_kernel(const fr* in, uint64_t n, fr* total) {
   __shared__ uint4 l[2][kOpThreads];
-  const uint64_t base = ((uint64_t)blockIdx.x << kOpLgSeg) + (uint64_t)threadIdx.x * kOpPer;
-  fr acc = fr_one();
-  for (int k = 0; k < kOpPer; k++) {
-    if (base + k < n) {
-      fr v = fr_load(in + base + k);
-      if (!fr_is_zero(v)) acc = fr_mul(acc, v);
-    }
-  }
+  // order-free product: thread t takes elements b*4096 + 256 k + t (coalesced)
+  const uint64_t b0 = (uint64_t)blockIdx.x << kOpLgSeg;
+  fr v[kOpPer];
+  static_for<0, kOpPer>([&](auto k_) {
+    constexpr int k = decltype(k_)::value;
+    const uint64_t i = b0 + (uint64_t)k * kOpThreads + threadIdx.x;
+    fr x = i < n ? fr_load(in + i) : fr_one();
+    v[k] = fr_is_zero(x) ? fr_one() : x;
+  });
+  static_for<0, 4>([&](auto s_) {
+    constexpr int h = kOpPer >> (decltype(s_)::value + 1);
+    static_for<0, h>([&](auto k_) {
+      constexpr int k = decltype(k_)::value;
+      v[k] = fr_mul(v[k], v[k + h]);
+    });
+  });
+  fr acc = v[0];
   lds_put2(l, threadIdx.x, acc);
   __syncthreads();
   for (int s = kOpThreads / 2; s > 0; s >>= 1) {

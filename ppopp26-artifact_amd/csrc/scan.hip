@@ -19,9 +19,13 @@
 namespace pl {
 
 constexpr int kScanThreads = 256;
-constexpr int kPer = 16;
-constexpr int kSeg = kScanThreads * kPer;  // 4096 elements per block
-constexpr int kLgSeg = 12;
+#ifndef PL_SCAN_LGPER
+#define PL_SCAN_LGPER 4
+#endif
+constexpr int kLgPer = PL_SCAN_LGPER;        // elements per thread: 2^kLgPer
+constexpr int kPer = 1 << kLgPer;
+constexpr int kSeg = kScanThreads * kPer;    // elements per block
+constexpr int kLgSeg = 8 + kLgPer;
 
 static inline unsigned sgrid(uint64_t n) { return (unsigned)((n + kSeg - 1) / kSeg); }
 
@@ -61,13 +65,26 @@ __device__ fr block_excl_prod(fr v, uint4 (*l)[kScanThreads], fr* block_total) {
   return excl;
 }
 
-// partial[b] = product of block b's elements
+// partial[b] = product of block b's elements.  The product is order-free,
+// so thread t takes elements b*4096 + 256 k + t: every load is coalesced.
 __global__ void __launch_bounds__(kScanThreads) pfx_reduce_kernel(const fr* in, uint64_t n, fr* partial) {
   __shared__ uint4 l[2][kScanThreads];
-  const uint64_t base = ((uint64_t)blockIdx.x << kLgSeg) + (uint64_t)threadIdx.x * kPer;
-  fr acc = fr_one();
-  for (int k = 0; k < kPer; k++)
-    if (base + k < n) acc = fr_mul(acc, fr_load(in + base + k));
+  const uint64_t b0 = (uint64_t)blockIdx.x << kLgSeg;
+  fr v[kPer];
+  static_for<0, kPer>([&](auto k_) {
+    constexpr int k = decltype(k_)::value;
+    const uint64_t i = b0 + (uint64_t)k * kScanThreads + threadIdx.x;
+    v[k] = i < n ? fr_load(in + i) : fr_one();
+  });
+  // pairwise tree in registers (independent products)
+  static_for<0, kLgPer>([&](auto s_) {
+    constexpr int h = kPer >> (decltype(s_)::value + 1);
+    static_for<0, h>([&](auto k_) {
+      constexpr int k = decltype(k_)::value;
+      v[k] = fr_mul(v[k], v[k + h]);
+    });
+  });
+  fr acc = v[0];
   // tree product over the block
   lds_put(l, threadIdx.x, acc);
   __syncthreads();
@@ -145,7 +162,7 @@ static hipError_t prefix_product(const fr* in, fr* out_excl, fr* out_incl, uint6
 
 // ---------------------------------------------------------------------------
 // affine suffix scan  s_i = a_i + m * s_{i+1}
-// pw: device table of m^(2^j), j = 0..kLgSeg (m^4096 last)
+// pw: device table of m^(2^j), j = 0..kLgSeg (m^kSeg last)
 // ---------------------------------------------------------------------------
 
 // block suffix combine: V_t <- V_t + m^(16*2^s) V_{t+2^s}; returns the suffix
@@ -158,35 +175,50 @@ __device__ fr block_suffix(fr v, uint4 (*l)[kScanThreads], const fr* pw) {
     const int o = 1 << s;
     fr other = t + o < kScanThreads ? lds_get(l, t + o) : fr_zero();
     __syncthreads();
-    if (t + o < kScanThreads) v = fr_add(v, fr_mul(fr_load(pw + 4 + s), other));  // m^(16 * 2^s)
+    if (t + o < kScanThreads) v = fr_add(v, fr_mul(fr_load(pw + kLgPer + s), other));  // m^(kPer * 2^s)
     lds_put(l, t, v);
     __syncthreads();
   }
   return v;
 }
 
-// partial[b] = sum over block b of a_k m^(k - start)
+// partial[b] = sum over block b of a_k m^(k - start).  The sum is
+// order-free: thread t takes elements start + 256 j + t (coalesced loads),
+// H_t = sum_j a m^(256 j) by Horner in m^256, then m^t H_t; a plain sum over
+// the block finishes it.
 __global__ void __launch_bounds__(kScanThreads) sfx_reduce_kernel(const fr* a, uint64_t n, const fr* pw, fr* partial) {
   __shared__ uint4 l[2][kScanThreads];
-  const uint64_t base = ((uint64_t)blockIdx.x << kLgSeg) + (uint64_t)threadIdx.x * kPer;
-  const fr m = fr_load(pw);
-  fr h = fr_zero();
-  for (int k = kPer - 1; k >= 0; k--) {
-    h = fr_mul(h, m);
-    if (base + k < n) h = fr_add(h, fr_load(a + base + k));
-  }
-  // tree: V_t += m^(16*2^s) V_{t+2^s} for t = 0 mod 2^(s+1)
-  lds_put(l, threadIdx.x, h);
+  const int t = threadIdx.x;
+  const uint64_t b0 = (uint64_t)blockIdx.x << kLgSeg;
+  const fr M = fr_load(pw + 8);  // m^256
+  fr v[kPer];
+  static_for<0, kPer>([&](auto k_) {
+    constexpr int k = decltype(k_)::value;
+    const uint64_t i = b0 + (uint64_t)k * kScanThreads + t;
+    v[k] = i < n ? fr_load(a + i) : fr_zero();
+  });
+  fr h = v[kPer - 1];
+  static_for<0, kPer - 1>([&](auto k_) {
+    constexpr int k = kPer - 2 - decltype(k_)::value;
+    h = fr_add(fr_mul(h, M), v[k]);
+  });
+  // m^t from the powers m^(2^bit)
+  fr mt = fr_one();
+  static_for<0, 8>([&](auto b_) {
+    constexpr int b = decltype(b_)::value;
+    if ((t >> b) & 1) mt = fr_mul(mt, fr_load(pw + b));
+  });
+  h = fr_mul(h, mt);
+  lds_put(l, t, h);
   __syncthreads();
-  for (int s = 0; (1 << s) < kScanThreads; s++) {
-    const int o = 1 << s;
-    if ((threadIdx.x & (2 * o - 1)) == 0) {
-      h = fr_add(h, fr_mul(fr_load(pw + 4 + s), lds_get(l, threadIdx.x + o)));
-      lds_put(l, threadIdx.x, h);
+  for (int s = kScanThreads / 2; s > 0; s >>= 1) {
+    if (t < s) {
+      h = fr_add(h, lds_get(l, t + s));
+      lds_put(l, t, h);
     }
     __syncthreads();
   }
-  if (threadIdx.x == 0) fr_store(partial + blockIdx.x, h);
+  if (t == 0) fr_store(partial + blockIdx.x, h);
 }
 
 // out[i] = s_{i+1} within each block given carry[b] = s at the block's end
@@ -208,9 +240,9 @@ __global__ void __launch_bounds__(kScanThreads) sfx_down_kernel(const fr* a, fr*
     constexpr int k = kPer - 2 - decltype(k_)::value;
     h = fr_add(fr_mul(h, m), e[k]);
   });
-  // fold the block carry into the last thread: s(start of thread 255) = L + m^16 K
+  // fold the block carry into the last thread: s(start of thread 255) = L + m^kPer K
   fr K = carry ? fr_load(carry + blockIdx.x) : fr_zero();
-  if (t == kScanThreads - 1) h = fr_add(h, fr_mul(fr_load(pw + 4), K));
+  if (t == kScanThreads - 1) h = fr_add(h, fr_mul(fr_load(pw + kLgPer), K));
   fr v = block_suffix(h, l, pw);  // s at the start of this thread's run
   // carry-in at this thread's end = s at the start of thread t+1
   fr cin = t + 1 < kScanThreads ? lds_get(l, t + 1) : K;
@@ -245,7 +277,7 @@ __global__ void sfx_powers_kernel(const fr* src, int neg, fr* pw) {
   }
 }
 
-// pw_next[j] = pw[kLgSeg]^(2^j): powers of m^4096 for the next level
+// pw_next[j] = pw[kLgSeg]^(2^j): powers of m^kSeg for the next level
 __global__ void sfx_next_powers_kernel(const fr* pw, fr* pw_next) {
   if (threadIdx.x != 0) return;
   fr m = fr_load(pw + kLgSeg);
