@@ -1,0 +1,214 @@
+// Inverse of one field element by the optimized binary GCD (T. Pornin,
+// "Optimized Binary GCD for Modular Inversion", IACR ePrint 2020/972,
+// Algorithm 2, with k - 1 = 31).
+//
+// a = y, b = r, u = 1, v = 0 keep a = u y and b = v y (mod r).  Each of the
+// 17 outer steps runs 31 binary-GCD steps on 64-bit approximations of a and
+// b (their low 31 bits and top 33 bits), recording the update as a 2x2
+// integer matrix (f0 g0; f1 g1) with |f| + |g| <= 2^31, then applies it to the
+// full values: a <- (f0 a + g0 b) / 2^31, b <- (f1 a + g1 b) / 2^31 (exact;
+// negated with its row when negative), and u, v by the same combination
+// divided by 2^31 modulo r (a Montgomery step: r = 1 mod 2^32).  After
+// 17 * 31 >= 2 * 255 - 1 steps b = gcd = 1 and v = y^-1 mod r.
+//
+// Plain integer code for host and device (tested on the host against the
+// oracle's inversion); one thread runs it.
+#pragma once
+#include <stdint.h>
+#if defined(__HIPCC__)
+#include <hip/hip_runtime.h>
+#endif
+
+namespace pl {
+namespace inv {
+
+#if defined(__HIPCC__)
+#define PL_HD __host__ __device__ __forceinline__
+#else
+#define PL_HD inline
+#endif
+
+constexpr uint32_t kR[8] = {0x00000001u, 0xffffffffu, 0xfffe5bfeu, 0x53bda402u,
+                            0x09a1d805u, 0x3339d808u, 0x299d7d48u, 0x73eda753u};
+
+// 10-limb two's-complement accumulator (320 bits)
+struct S10 {
+  uint32_t w[10];
+};
+
+PL_HD int bitlen8(const uint32_t* x) {
+  for (int i = 7; i >= 0; i--)
+    if (x[i]) return 32 * i + 32 - __builtin_clz(x[i]);
+  return 0;
+}
+
+// bits [s, s + 33) of x (s >= 31), as an integer
+PL_HD uint64_t bits33(const uint32_t* x, int s) {
+  const int w = s >> 5, sh = s & 31;
+  const uint64_t x0 = w < 8 ? x[w] : 0, x1 = w + 1 < 8 ? x[w + 1] : 0, x2 = w + 2 < 8 ? x[w + 2] : 0;
+  uint64_t lo = x0 | (x1 << 32);
+  uint64_t v = sh ? ((lo >> sh) | (x2 << (64 - sh))) : lo;
+  return v & ((1ull << 33) - 1);
+}
+
+// acc += x * f  (x: 8 limbs unsigned, f signed, |f| <= 2^31)
+PL_HD void add_mul(S10& acc, const uint32_t* x, int64_t f) {
+  const bool neg = f < 0;
+  const uint64_t af = neg ? (uint64_t)(-f) : (uint64_t)f;
+  // p = x * |f| as 9 limbs
+  uint32_t p[10];
+  uint64_t c = 0;
+  for (int i = 0; i < 8; i++) {
+    uint64_t t = (uint64_t)x[i] * af + c;  // af < 2^32: no overflow
+    p[i] = (uint32_t)t;
+    c = t >> 32;
+  }
+  p[8] = (uint32_t)c;
+  p[9] = 0;
+  if (neg) {  // two's complement of p over 10 limbs
+    uint64_t b = 1;
+    for (int i = 0; i < 10; i++) {
+      uint64_t t = (uint64_t)(uint32_t)~p[i] + b;
+      p[i] = (uint32_t)t;
+      b = t >> 32;
+    }
+  }
+  uint64_t cc = 0;
+  for (int i = 0; i < 10; i++) {
+    uint64_t t = (uint64_t)acc.w[i] + p[i] + cc;
+    acc.w[i] = (uint32_t)t;
+    cc = t >> 32;
+  }
+}
+
+PL_HD bool s10_neg(const S10& a) { return (a.w[9] >> 31) != 0; }
+
+PL_HD void s10_negate(S10& a) {
+  uint64_t b = 1;
+  for (int i = 0; i < 10; i++) {
+    uint64_t t = (uint64_t)(uint32_t)~a.w[i] + b;
+    a.w[i] = (uint32_t)t;
+    b = t >> 32;
+  }
+}
+
+// x = a >> 31 (a non-negative, result < 2^256)
+PL_HD void s10_shr31_to8(const S10& a, uint32_t* x) {
+  for (int i = 0; i < 8; i++) x[i] = (a.w[i] >> 31) | (a.w[i + 1] << 1);
+}
+
+// u' = (u f0 + v g0) / 2^31 mod r, with u, v < r
+PL_HD void mod_update(uint32_t* out, const uint32_t* u, int64_t f, const uint32_t* v, int64_t g) {
+  S10 t = {};
+  add_mul(t, u, f);
+  add_mul(t, v, g);
+  // |t| < 2^32 r: make it non-negative by adding r * 2^32
+  if (s10_neg(t)) {
+    uint64_t c = 0;
+    for (int i = 0; i < 10; i++) {
+      uint64_t rr = (i >= 1 && i <= 8) ? kR[i - 1] : 0;
+      uint64_t s = (uint64_t)t.w[i] + rr + c;
+      t.w[i] = (uint32_t)s;
+      c = s >> 32;
+    }
+  }
+  // Montgomery step: q = -t mod 2^31 (r^-1 = 1 mod 2^31); t += q r; t /= 2^31
+  const uint32_t q = (0u - t.w[0]) & 0x7fffffffu;
+  {
+    uint64_t c = 0;
+    for (int i = 0; i < 10; i++) {
+      uint64_t s = (uint64_t)t.w[i] + c + (i < 8 ? (uint64_t)q * kR[i] : 0);
+      t.w[i] = (uint32_t)s;
+      c = s >> 32;
+    }
+  }
+  uint32_t x[9];
+  for (int i = 0; i < 8; i++) x[i] = (t.w[i] >> 31) | (t.w[i + 1] << 1);
+  x[8] = (t.w[8] >> 31) | (t.w[9] << 1);
+  // x < 5 r: subtract r while x >= r (at most four times)
+  for (int it = 0; it < 4; it++) {
+    uint32_t d[9];
+    int64_t bw = 0;
+    for (int i = 0; i < 9; i++) {
+      int64_t s = (int64_t)x[i] - (i < 8 ? kR[i] : 0) + bw;
+      d[i] = (uint32_t)s;
+      bw = s >> 32;  // 0 or -1
+    }
+    if (bw == 0)
+      for (int i = 0; i < 9; i++) x[i] = d[i];
+  }
+  for (int i = 0; i < 8; i++) out[i] = x[i];
+}
+
+// y^-1 mod r for 0 < y < r (y as 8 little-endian limbs); 0 for y = 0
+PL_HD void inverse(const uint32_t* y, uint32_t* out) {
+  uint32_t a[8], b[8], u[8], v[8];
+  bool zero = true;
+  for (int i = 0; i < 8; i++) {
+    a[i] = y[i];
+    b[i] = kR[i];
+    u[i] = 0;
+    v[i] = 0;
+    zero = zero && y[i] == 0;
+  }
+  if (zero) {
+    for (int i = 0; i < 8; i++) out[i] = 0;
+    return;
+  }
+  u[0] = 1;
+  for (int outer = 0; outer < 17; outer++) {
+    uint32_t ab[8];
+    for (int i = 0; i < 8; i++) ab[i] = a[i] | b[i];
+    int n = bitlen8(ab);
+    if (n < 64) n = 64;
+    uint64_t xa = (uint64_t)(a[0] & 0x7fffffffu) | (bits33(a, n - 33) << 31);
+    uint64_t xb = (uint64_t)(b[0] & 0x7fffffffu) | (bits33(b, n - 33) << 31);
+    int64_t f0 = 1, g0 = 0, f1 = 0, g1 = 1;
+    for (int j = 0; j < 31; j++) {
+      if (xa & 1) {
+        if (xa < xb) {
+          uint64_t tx = xa; xa = xb; xb = tx;
+          int64_t tf = f0; f0 = f1; f1 = tf;
+          int64_t tg = g0; g0 = g1; g1 = tg;
+        }
+        xa = (xa - xb) >> 1;
+        f0 -= f1;
+        g0 -= g1;
+      } else {
+        xa >>= 1;
+      }
+      f1 <<= 1;
+      g1 <<= 1;
+    }
+    S10 na = {}, nb = {};
+    add_mul(na, a, f0);
+    add_mul(na, b, g0);
+    add_mul(nb, a, f1);
+    add_mul(nb, b, g1);
+    if (s10_neg(na)) {
+      s10_negate(na);
+      f0 = -f0;
+      g0 = -g0;
+    }
+    if (s10_neg(nb)) {
+      s10_negate(nb);
+      f1 = -f1;
+      g1 = -g1;
+    }
+    s10_shr31_to8(na, a);
+    s10_shr31_to8(nb, b);
+    uint32_t nu[8], nv[8];
+    mod_update(nu, u, f0, v, g0);
+    mod_update(nv, u, f1, v, g1);
+    for (int i = 0; i < 8; i++) {
+      u[i] = nu[i];
+      v[i] = nv[i];
+    }
+  }
+  for (int i = 0; i < 8; i++) out[i] = v[i];
+}
+
+#undef PL_HD
+
+}  // namespace inv
+}  // namespace pl

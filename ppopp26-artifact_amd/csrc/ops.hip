@@ -14,6 +14,7 @@
 // canonical residue (variable time) and a Montgomery product with R^3 mod r
 // to return to Montgomery form.
 #include "pl_internal.hpp"
+#include "fr_inv.hpp"
 
 namespace pl {
 
@@ -39,81 +40,15 @@ __device__ __forceinline__ fr const_fr(uint32_t (*f)(int)) {
   return r;
 }
 
-// ---- single inversion: binary extended Euclid on the canonical value ------
-__device__ __forceinline__ bool big_is_one(const uint32_t* a) {
-  uint32_t o = a[0] ^ 1u;
-  for (int i = 1; i < 8; i++) o |= a[i];
-  return o == 0;
-}
-__device__ __forceinline__ bool big_geq(const uint32_t* a, const uint32_t* b) {
-  for (int i = 7; i >= 0; i--)
-    if (a[i] != b[i]) return a[i] > b[i];
-  return true;
-}
-__device__ __forceinline__ void big_sub(uint32_t* a, const uint32_t* b) {
-  uint32_t br = 0;
-  for (int i = 0; i < 8; i++) a[i] = __builtin_subc(a[i], b[i], br, &br);
-}
-__device__ __forceinline__ void big_shr1(uint32_t* a, uint32_t top) {
-  for (int i = 0; i < 7; i++) a[i] = (a[i] >> 1) | (a[i + 1] << 31);
-  a[7] = (a[7] >> 1) | (top << 31);
-}
-// x <- x/2 mod r (x < r)
-__device__ __forceinline__ void half_mod(uint32_t* x) {
-  if (x[0] & 1) {
-    uint32_t c = 0;
-    for (int i = 0; i < 8; i++) x[i] = __builtin_addc(x[i], r_limb(i), c, &c);
-    big_shr1(x, c);
-  } else {
-    big_shr1(x, 0);
-  }
-}
-// x <- x - y mod r
-__device__ __forceinline__ void sub_mod_raw(uint32_t* x, const uint32_t* y) {
-  uint32_t br = 0;
-  for (int i = 0; i < 8; i++) x[i] = __builtin_subc(x[i], y[i], br, &br);
-  if (br) {
-    uint32_t c = 0;
-    for (int i = 0; i < 8; i++) x[i] = __builtin_addc(x[i], r_limb(i), c, &c);
-  }
-}
-
-// Montgomery-form inverse of a (a = x R -> x^-1 R); zero maps to zero.
+// ---- single inversion ----------------------------------------------------
+// Montgomery-form inverse of a (a = x R -> x^-1 R); zero maps to zero.  The
+// integer inverse of the canonical value x R comes from the optimized binary
+// GCD in fr_inv.hpp (17 rounds of 31 steps, no data-dependent trip count).
 __device__ fr fr_inv_single(const fr& a_in) {
-  // canonical representative first, so that u = 0 cannot occur below
   const fr a = fr_reduce_once(a_in);
   if (fr_is_zero(a)) return fr_zero();
-  uint32_t u[8], v[8], x1[8], x2[8];
-  for (int i = 0; i < 8; i++) {
-    u[i] = a.l[i];
-    v[i] = r_limb(i);
-    x1[i] = 0;
-    x2[i] = 0;
-  }
-  x1[0] = 1;
-  // invariant: x1 * a = u, x2 * a = v (mod r); gcd(a, r) = 1 for 0 < a < r,
-  // and every round removes at least one bit from u or v, so 2*256 + 2
-  // rounds bound the loop (the bound only guards against bad input)
-  for (int round = 0; round < 2 * 256 + 2 && !big_is_one(u) && !big_is_one(v); round++) {
-    while ((u[0] & 1) == 0 && !fr_is_zero(*reinterpret_cast<const fr*>(u))) {
-      big_shr1(u, 0);
-      half_mod(x1);
-    }
-    while ((v[0] & 1) == 0 && !fr_is_zero(*reinterpret_cast<const fr*>(v))) {
-      big_shr1(v, 0);
-      half_mod(x2);
-    }
-    if (big_geq(u, v)) {
-      big_sub(u, v);
-      sub_mod_raw(x1, x2);
-    } else {
-      big_sub(v, u);
-      sub_mod_raw(x2, x1);
-    }
-  }
   fr r;
-  const uint32_t* src = big_is_one(u) ? x1 : x2;
-  for (int i = 0; i < 8; i++) r.l[i] = src[i];
+  inv::inverse(a.l, r.l);
   // r = (xR)^-1 = x^-1 R^-1 ; times R^3 under Montgomery: x^-1 R
   return fr_mul(r, const_fr(rrr_limb));
 }
