@@ -162,7 +162,7 @@ hipError_t domain_build(Domain& d, int lg, hipStream_t s) {
   d.lg = lg;
   d.plan = make_plan(lg);
   const NttPlan& P = d.plan;
-  size_t count = 2 * 256 + 2 * 16384;  // inner tables, w_N power windows
+  size_t count = 4 * 256 + 2 * 16384;  // inner tables (both forms), w_N power windows
   for (int p = 0; p + 1 < P.P; p++) count += 2 * (1ull << P.lgM[p]);
   d.bytes = count * sizeof(fr);
   hipError_t e = hipMalloc((void**)&d.block, d.bytes);
@@ -172,6 +172,8 @@ hipError_t domain_build(Domain& d, int lg, hipStream_t s) {
   d.pw1 = cur; cur += 16384;
   d.inner_fwd = cur; cur += 256;
   d.inner_inv = cur; cur += 256;
+  d.inner29_fwd = cur; cur += 256;
+  d.inner29_inv = cur; cur += 256;
   for (int p = 0; p + 1 < P.P; p++) {
     d.tw_fwd[p] = cur; cur += 1ull << P.lgM[p];
     d.tw_inv[p] = cur; cur += 1ull << P.lgM[p];
@@ -182,6 +184,8 @@ hipError_t domain_build(Domain& d, int lg, hipStream_t s) {
   host::fr64 w256 = host::root_of_unity(8);
   host::fr64 w256i = host::inv(w256);
   d.n_inv = host::domain_size_inverse(lg);
+  const host::fr64 c32 = host::from_u64(32);  // stored form -> w R' mod r
+  const host::fr64 n_inv32 = host::mul(d.n_inv, c32);
   {
     // powers of w_N in two 2^14 windows (permutation ratio, coset tables)
     fr b = to_dev(host::root_of_unity(lg)), one = to_dev(host::one());
@@ -192,13 +196,15 @@ hipError_t domain_build(Domain& d, int lg, hipStream_t s) {
   }
   if ((e = fill_pow(d.inner_fwd, 256, 0, w256, nullptr, win, s)) != hipSuccess) goto done;
   if ((e = fill_pow(d.inner_inv, 256, 0, w256i, nullptr, win, s)) != hipSuccess) goto done;
+  if ((e = fill_pow(d.inner29_fwd, 256, 0, w256, &c32, win, s)) != hipSuccess) goto done;
+  if ((e = fill_pow(d.inner29_inv, 256, 0, w256i, &c32, win, s)) != hipSuccess) goto done;
   for (int p = 0; p + 1 < P.P; p++) {
     host::fr64 w = host::root_of_unity(P.lgM[p]);
     host::fr64 wi = host::inv(w);
     uint64_t n = 1ull << P.lgM[p];
-    if ((e = fill_pow(d.tw_fwd[p], n, P.lgS[p], w, nullptr, win, s)) != hipSuccess) goto done;
+    if ((e = fill_pow(d.tw_fwd[p], n, P.lgS[p], w, &c32, win, s)) != hipSuccess) goto done;
     // the inverse transform's 2^-lg rides on the first pass's twiddles
-    if ((e = fill_pow(d.tw_inv[p], n, P.lgS[p], wi, p == 0 ? &d.n_inv : nullptr, win, s)) != hipSuccess) goto done;
+    if ((e = fill_pow(d.tw_inv[p], n, P.lgS[p], wi, p == 0 ? &n_inv32 : &c32, win, s)) != hipSuccess) goto done;
   }
   e = hipStreamSynchronize(s);
 done:
@@ -279,10 +285,12 @@ static void fill_pass_args(const Domain& d, const NttCall& c, int p, const fr* i
   const int lg = d.lg;
   memset(&a, 0, sizeof(a));
   a.inner = c.inverse ? d.inner_inv : d.inner_fwd;
+  a.inner29 = c.inverse ? d.inner29_inv : d.inner29_fwd;
   a.lgN = lg;
   a.lgI = c.lgI;
   a.pm_mask = c.pm_mask;
   a.scale = to_dev(d.n_inv);
+  a.scale29 = to_dev(host::mul(d.n_inv, host::from_u64(32)));
   a.in = in;
   a.out = out;
   a.in_pstride = in_pstride;

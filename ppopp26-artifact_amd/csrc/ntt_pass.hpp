@@ -24,36 +24,52 @@
 // C*32 >= 256 contiguous bytes.  The in-register DFTs are radix-2 DIF
 // networks whose twiddles (powers of w_16) are compile-time constants read
 // from SGPRs.
+//
+// Arithmetic: every product in a pass multiplies a data value by a constant
+// or table factor, so it uses the radix-2^29 product of fr29.hpp with the
+// factor held as w R' mod r (R' = 2^261): the DFT constants below, and the
+// domain's inner and pass twiddle tables, which are built in that form.  The
+// data stays in the stored form x R mod r, kept lazily in [0, 2r) between
+// operations and reduced once before each store, so a pass's output is
+// canonical and bit-identical to any exact implementation.  The optional
+// per-element factor table of the first pass (coset powers) is in the stored
+// form and uses the saturated product of fr.hpp.
 #pragma once
+#include "fr29.hpp"
 #include "pl_internal.hpp"
 
 namespace pl {
 
-// w_16^e and w_16^-e, e = 0..7, Montgomery form
-static __constant__ const uint32_t kW16[2][8][8] = {
-    {{0xfffffffeu, 0x00000001u, 0x00034802u, 0x5884b7fau, 0xecbc4ff5u, 0x998c4fefu, 0xacc5056fu, 0x1824b159u},
-     {0x88202b65u, 0x1acf64e4u, 0x819f004fu, 0x5b18b637u, 0x3107fdf5u, 0xee649cabu, 0xce9012a0u, 0x27a8543du},
-     {0x2c42eff8u, 0x6ecf4995u, 0x4d5dfa57u, 0x6ec1fa83u, 0x5656b15cu, 0x3ff5cf38u, 0x229e031bu, 0x54b80267u},
-     {0x45c09c96u, 0x5fb40c77u, 0x8b8b1b1bu, 0xfff59b78u, 0x6b7eed47u, 0x023ba68bu, 0x76f3e130u, 0x4e17f66fu},
-     {0xaa89cfb1u, 0xf3b05674u, 0x6006b9feu, 0x072f0140u, 0x25667a26u, 0xce9a0dbfu, 0x2d598374u, 0x4d2ce405u},
-     {0x0b0a1289u, 0x25c2a002u, 0x7d082188u, 0x971c717fu, 0xd81ac5d8u, 0x4739dccau, 0xeee341c1u, 0x4f607ba9u},
-     {0x3eb2a8ddu, 0x111f9842u, 0x6f38b840u, 0x1f772784u, 0x8e3ca13cu, 0xb28758a3u, 0xed9a48f0u, 0x5974f9f7u},
-     {0xdcbeed53u, 0x0ddd2e46u, 0x33dbdd55u, 0xdf977d7fu, 0xc1470bbau, 0xdda3b36au, 0xb67a22b2u, 0x4818fbf6u}},
-    {{0xfffffffeu, 0x00000001u, 0x00034802u, 0x5884b7fau, 0xecbc4ff5u, 0x998c4fefu, 0xacc5056fu, 0x1824b159u},
-     {0x234112aeu, 0xf222d1b8u, 0xcc227ea9u, 0x74262683u, 0x485acc4au, 0x5596249du, 0x73235a95u, 0x2bd4ab5cu},
-     {0xc14d5724u, 0xeee067bcu, 0x90c5a3beu, 0x34467c7eu, 0x7b6536c9u, 0x80b27f64u, 0x3c033457u, 0x1a78ad5bu},
-     {0xf4f5ed78u, 0xda3d5ffcu, 0x82f63a76u, 0xbca13283u, 0x3187122cu, 0xebfffb3du, 0x3aba3b86u, 0x248d2ba9u},
-     {0x55763050u, 0x0c4fa98au, 0x9ff7a200u, 0x4c8ea2c2u, 0xe43b5ddfu, 0x649fca48u, 0xfc43f9d3u, 0x26c0c34du},
-     {0xba3f636bu, 0xa04bf387u, 0x747340e3u, 0x53c8088au, 0x9e22eabdu, 0x30fe317cu, 0xb2a99c18u, 0x25d5b0e3u},
-     {0xd3bd1009u, 0x9130b669u, 0xb2a061a7u, 0xe4fba97fu, 0xb34b26a8u, 0xf34408cfu, 0x06ff7a2cu, 0x1f35a4ecu},
-     {0x77dfd49cu, 0xe5309b1au, 0x7e5f5bafu, 0xf8a4edcbu, 0xd899da0fu, 0x44d53b5cu, 0x5b0d6aa7u, 0x4c455315u}}};
+// w_16^e and w_16^-e, e = 0..7, as w R' mod r (R' = 2^261) in nine 29-bit
+// limbs: the constant operand of the radix-2^29 product (fr29.hpp)
+static __constant__ const uint32_t kW16r[2][8][9] = {
+    {{0x1fffffbau, 0x0000022fu, 0x1cb61180u, 0x0a4e5c00u, 0x0ee8b1a2u, 0x16e6aedfu, 0x1907f8bbu, 0x0853ddf7u, 0x004d043fu},
+     {0x04056c96u, 0x0f64e4d8u, 0x1c1c7b56u, 0x1cbda467u, 0x14e761dau, 0x128ae056u, 0x1d2f3144u, 0x0f463b6du, 0x006dc1feu},
+     {0x085dfee9u, 0x0f4995e4u, 0x1941c136u, 0x0b284957u, 0x1c313523u, 0x1f28f6a5u, 0x07ad921du, 0x0d72d344u, 0x002ca644u},
+     {0x181392abu, 0x140c77edu, 0x0175e03du, 0x09f5a4e3u, 0x0f090202u, 0x0c6252cbu, 0x1c504ed8u, 0x038e921bu, 0x00408014u},
+     {0x1139f60bu, 0x10567552u, 0x1e6cfcddu, 0x03679a01u, 0x08c51075u, 0x1f9df144u, 0x1eb67e0bu, 0x1a6828a4u, 0x00231dc6u},
+     {0x0142510bu, 0x02a002b3u, 0x09a9466eu, 0x1f776142u, 0x002a304fu, 0x10591c89u, 0x04f2cdf3u, 0x0d6e4f9eu, 0x006990bbu},
+     {0x16551b88u, 0x1f9842feu, 0x0f9a0688u, 0x1b208bceu, 0x1e704151u, 0x09d87033u, 0x1d460dfbu, 0x063990abu, 0x0050578fu},
+     {0x17ddaa4du, 0x1d2e4774u, 0x06b5ad2eu, 0x19075af7u, 0x06ef6bbdu, 0x125fb8efu, 0x07ff9cadu, 0x15571281u, 0x00687c13u}},
+    {{0x1fffffbau, 0x0000022fu, 0x1cb61180u, 0x0a4e5c00u, 0x0ee8b1a2u, 0x16e6aedfu, 0x1907f8bbu, 0x0853ddf7u, 0x004d043fu},
+     {0x082255b4u, 0x02d1b883u, 0x18e15291u, 0x0240ab08u, 0x1690e97eu, 0x19a44be1u, 0x0d213039u, 0x150e212eu, 0x000b7193u},
+     {0x09aae479u, 0x0067bcf9u, 0x0ffcf937u, 0x00277a31u, 0x1f1013eau, 0x022b949cu, 0x17dabeecu, 0x042ba303u, 0x00239618u},
+     {0x1ebdaef6u, 0x1d5ffd44u, 0x15edb951u, 0x1bd0a4bdu, 0x1d5624ebu, 0x1baae847u, 0x102dfef3u, 0x1cf6e411u, 0x000a5cebu},
+     {0x0ec609f6u, 0x0fa98aa5u, 0x012a02e2u, 0x17e06bfeu, 0x14bb44c6u, 0x0c66138cu, 0x166a4edbu, 0x0ffd0b0au, 0x0050cfe0u},
+     {0x07ec6d56u, 0x0bf3880au, 0x1e211f82u, 0x1152611cu, 0x0e775339u, 0x1fa1b205u, 0x18d07e0eu, 0x06d6a193u, 0x00336d93u},
+     {0x17a20118u, 0x10b66a13u, 0x06553e89u, 0x101fbca8u, 0x014f2018u, 0x0cdb0e2bu, 0x0d733ac9u, 0x1cf2606bu, 0x00474762u},
+     {0x1bfa936bu, 0x109b1b1fu, 0x037a8469u, 0x1e8a6198u, 0x0898f360u, 0x1979247au, 0x17f19ba2u, 0x1b1ef841u, 0x00062ba8u}}};
 
 template <bool INV>
-__device__ __forceinline__ fr w16c(int e) {
-  fr w;
+__device__ __forceinline__ void w16r(int e, uint32_t (&w)[9]) {
 #pragma unroll
-  for (int i = 0; i < 8; i++) w.l[i] = kW16[INV ? 1 : 0][e][i];
-  return w;
+  for (int i = 0; i < 9; i++) w[i] = kW16r[INV ? 1 : 0][e][i];
+}
+
+// an ordering point for the eight limbs of v (no instruction)
+__device__ __forceinline__ void pin(fr& v) {
+  asm volatile("" : "+v"(v.l[0]), "+v"(v.l[1]), "+v"(v.l[2]), "+v"(v.l[3]), "+v"(v.l[4]), "+v"(v.l[5]),
+               "+v"(v.l[6]), "+v"(v.l[7]));
 }
 
 __host__ __device__ constexpr int brev(int v, int bits) {
@@ -63,7 +79,9 @@ __host__ __device__ constexpr int brev(int v, int bits) {
 }
 
 // Radix-2 DIF DFT of size 2^LOGB on x[OFF + i], natural order in,
-// bit-reversed order out: afterwards x[OFF + brev(k)] = X[k].
+// bit-reversed order out: afterwards x[OFF + brev(k)] = X[k].  Values are kept
+// lazily in [0, 2r) (fr_add_lazy / fr_sub_lazy; the radix-2^29 product returns
+// below 1.03 r); the pass reduces them once before storing.
 template <int LOGB, bool INV, int OFF, int NX>
 __device__ __forceinline__ void dft_dif(fr (&x)[NX]) {
   constexpr int B = 1 << LOGB;
@@ -74,11 +92,23 @@ __device__ __forceinline__ void dft_dif(fr (&x)[NX]) {
       constexpr int s = decltype(blk_)::value * 2 * h;
       static_for<0, h>([&](auto i_) {
         constexpr int i = decltype(i_)::value;
+        // one butterfly at a time: the empty asm statements order the
+        // butterflies, so the live set stays at the 16 elements plus one
+        // butterfly's temporaries
         fr a = x[OFF + s + i];
         fr b = x[OFF + s + i + h];
-        x[OFF + s + i] = fr_add(a, b);
-        fr d = fr_sub(a, b);
-        if constexpr (i != 0) d = fr_mul_u(d, w16c<INV>((i * 8) / h));
+        pin(a);
+        pin(b);
+        fr sum = fr_add_lazy(a, b);
+        fr d = fr_sub_lazy(a, b);
+        if constexpr (i != 0) {
+          uint32_t w[9];
+          w16r<INV>((i * 8) / h, w);
+          d = fr_mul29<true>(d, w);
+        }
+        pin(sum);
+        pin(d);
+        x[OFF + s + i] = sum;
         x[OFF + s + i + h] = d;
       });
     });
@@ -89,8 +119,9 @@ struct PassArgs {
   const fr* in;
   fr* out;
   const fr* premul;     // !LAST: optional factor per input element [(poly & pm_mask) << lgN | pos]
-  const fr* tw;         // !LAST: pass twiddles, index j + S*k within the sub-problem
-  const fr* inner;      // w_256^e (or w_256^-e), e < 256
+  const fr* tw;         // !LAST: pass twiddles (w R' mod r), index j + S*k within the sub-problem
+  const fr* inner;      // w_256^e (or w_256^-e), e < 256, stored form (small-size kernel)
+  const fr* inner29;    // the same as w R' mod r (pass kernels)
   uint64_t in_pstride;  // element stride between input polys (0: shared input)
   uint64_t nlines;      // lines (columns) in this launch
   uint32_t pm_mask;
@@ -101,7 +132,8 @@ struct PassArgs {
   uint32_t ndig;        // LAST: number of earlier passes
   uint32_t do_scale;    // multiply every output by `scale`
   uint32_t lgrad[PL_MAX_PASSES];  // LAST: log2 radices of the earlier passes, pass 1 first
-  fr scale;
+  fr scale;             // stored form (small-size kernel)
+  fr scale29;           // scale R' mod r (pass kernels)
 };
 
 // Tile geometry (compile-time; see the file header):
@@ -205,10 +237,24 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, const uint64_t 
       x[r2] = fr_load(src + pos0 + (uint64_t)(rho1 + A * r2) * estride);
     });
     if (!LAST && a.premul) {
-      const fr* pm = a.premul + ((uint64_t)(poly & a.pm_mask) << a.lgN);
-      static_for<0, 16>([&](auto r2_) {
-        constexpr int r2 = decltype(r2_)::value;
-        x[r2] = fr_mul(x[r2], fr_load(pm + pos0 + (uint64_t)(rho1 + A * r2) * estride));
+      const fr* pm = a.premul + ((uint64_t)(poly & a.pm_mask) << a.lgN) + pos0;
+      // four factors at a time: each group's loads start after the previous
+      // group's products (the asm statements order them), so at most four
+      // factors are held in registers beside the 16 elements
+      static_for<0, 4>([&](auto g_) {
+        constexpr int g = decltype(g_)::value;
+        const fr* q = pm;
+        asm volatile("" : "+v"(q));
+        fr f[4];
+        static_for<0, 4>([&](auto r_) {
+          constexpr int r = decltype(r_)::value;
+          f[r] = fr_load(q + (uint64_t)(rho1 + A * (4 * g + r)) * estride);
+        });
+        static_for<0, 4>([&](auto r_) {
+          constexpr int r = decltype(r_)::value;
+          x[4 * g + r] = fr_mul(x[4 * g + r], f[r]);
+          pin(x[4 * g + r]);
+        });
       });
     }
   } else {
@@ -219,7 +265,8 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, const uint64_t 
     static_for<1, 16>([&](auto k2_) {
       constexpr int k2 = decltype(k2_)::value;
       const int e = rho1 * k2;  // < M'
-      x[brev(k2, 4)] = fr_mul(x[brev(k2, 4)], fr_load(a.inner + (e << (8 - m))));
+      pin(x[brev(k2, 4)]);
+      x[brev(k2, 4)] = fr_mul29(x[brev(k2, 4)], fr_load(a.inner29 + (e << (8 - m))));
     });
   }
   // Exchange through LDS in kPhases rounds of kW limbs (2 rounds of 16 bytes
@@ -230,8 +277,12 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, const uint64_t 
   // The line index is XOR-swizzled with rho1 so that both the line-major
   // (intermediate passes) and the rho-major (last pass) write mappings and
   // the line-major reads are free of LDS bank conflicts.
-  const int lc2 = t & (C - 1);
-  const int g = t >> LGC;
+  // a fresh copy of the thread index, so that sub-step 2's indices are not
+  // shared with sub-step 1's and kept live across it
+  int t2 = t;
+  asm volatile("" : "+v"(t2));
+  const int lc2 = t2 & (C - 1);
+  const int g = t2 >> LGC;
   fr (&y)[16] = x;
   static_for<0, kPhases>([&](auto h_) {
     constexpr int h = decltype(h_)::value;
@@ -284,8 +335,8 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, const uint64_t 
         constexpr int k1 = decltype(k1_)::value;
         const uint64_t k = (uint64_t)(k2 + 16 * k1);
         fr v = y[f * A + brev(k1, LA)];
-        if (a.do_scale) v = fr_mul_u(v, a.scale);
-        fr_store(dst + ((lp + (k << lgL)) << lgI), v);
+        if (a.do_scale) v = fr_mul29(v, a.scale29);
+        fr_store(dst + ((lp + (k << lgL)) << lgI), fr_reduce_once(v));
       });
     });
   } else {
@@ -302,9 +353,10 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, const uint64_t 
         constexpr int k1 = decltype(k1_)::value;
         const uint64_t k = (uint64_t)(k2 + 16 * k1);
         fr v = y[f * A + brev(k1, LA)];
-        v = fr_mul(v, fr_load(tw + (k << a.lgS)));
-        if (a.do_scale) v = fr_mul_u(v, a.scale);
-        fr_store(dst + (k << a.lgS), v);
+        pin(v);
+        v = fr_mul29(v, fr_load(tw + (k << a.lgS)));
+        if (a.do_scale) v = fr_mul29(v, a.scale29);
+        fr_store(dst + (k << a.lgS), fr_reduce_once(v));
       });
     });
   }

@@ -36,12 +36,16 @@ bool plan_set(int lg, const int* m, int P);
 struct Domain {
   int lg = 0;
   NttPlan plan;
+  // The pass kernels' factors are held as w R' mod r (R' = 2^261, the radix-2^29
+  // product's Montgomery constant; fr29.hpp), i.e. the stored form times 32.
   fr* tw_fwd[PL_MAX_PASSES] = {};  // pass twiddles, 2^lgM[p] entries, p < P-1
   fr* tw_inv[PL_MAX_PASSES] = {};  // inverse; pass 0 also carries 2^-lg when P >= 2
   fr* pw0 = nullptr;               // w_N^i = pw0[i & 16383] * pw1[i >> 14]
   fr* pw1 = nullptr;
-  fr* inner_fwd = nullptr;         // omega_256^e, e < 256
+  fr* inner_fwd = nullptr;         // omega_256^e, e < 256 (stored form; small-size kernel)
   fr* inner_inv = nullptr;
+  fr* inner29_fwd = nullptr;       // the same times 32 (pass kernels)
+  fr* inner29_inv = nullptr;
   fr* block = nullptr;             // single allocation holding all tables
   size_t bytes = 0;
   host::fr64 n_inv;

@@ -1,0 +1,131 @@
+"""The radix-2^29 Montgomery product of the NTT passes (csrc/fr29.hpp) and
+its constants, on the host.
+
+* fr29.hpp's plain-C product, built with g++, against Python integers:
+  mont29(x, w) = x w 2^-261 mod r and below 1.031 r for any 256-bit x and
+  w < r (the bound the lazy [0, 2r) butterflies rely on);
+* the DFT-16 constants in ntt_pass.hpp against the reference's own
+  16th roots of unity (tests/golden/ref_constants.json), times 2^261;
+* r and 2r as written in fr29.hpp;
+* mont29_columns.inc is what its generator writes, and every column block
+  adds each product term exactly once.
+
+The device asm blocks are exercised bit-for-bit by every GPU NTT/LDE test.
+"""
+import ctypes
+import json
+import os
+import random
+import re
+import subprocess
+
+import pytest
+
+R = 0x73EDA753299D7D483339D80809A1D80553BDA402FFFE5BFEFFFFFFFF00000001
+HERE = os.path.dirname(__file__)
+CSRC = os.path.join(HERE, "..", "ppopp26-artifact_amd", "csrc")
+M29 = (1 << 29) - 1
+
+
+def limbs(v, n, b):
+    return [(v >> (b * i)) & ((1 << b) - 1) for i in range(n)]
+
+
+@pytest.fixture(scope="module")
+def mul29(tmp_path_factory):
+    d = tmp_path_factory.mktemp("fr29")
+    src = d / "fr29_host.cpp"
+    src.write_text(
+        '#include "%s"\n'
+        'extern "C" void mul29(const uint32_t* x, const uint32_t* w9, uint32_t* y) {\n'
+        "  uint32_t a[9], t[9];\n"
+        "  pl::split29(x, a);\n"
+        "  pl::mont29(a, w9, t);\n"
+        "  pl::join29(t, y);\n"
+        "}\n" % os.path.abspath(os.path.join(CSRC, "fr29.hpp"))
+    )
+    lib = d / "libfr29.so"
+    subprocess.check_call(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", str(src), "-o", str(lib)])
+    so = ctypes.CDLL(str(lib))
+
+    def call(x, w):
+        X = (ctypes.c_uint32 * 8)(*limbs(x, 8, 32))
+        W = (ctypes.c_uint32 * 9)(*limbs(w, 9, 29))
+        Y = (ctypes.c_uint32 * 8)()
+        so.mul29(X, W, Y)
+        return sum(int(Y[i]) << (32 * i) for i in range(8))
+
+    return call
+
+
+def test_product_matches_integers(mul29):
+    rinv = pow(1 << 261, -1, R)
+    rng = random.Random(29)
+    cases = [(0, R - 1), (1, 1), (R - 1, R - 1), (2 * R - 1, R - 1), ((1 << 256) - 1, R - 1), ((1 << 256) - 1, 1)]
+    cases += [((1 << k) - 1, R - 1 - k) for k in range(1, 257)]
+    cases += [(rng.randrange(1 << 256), rng.randrange(R)) for _ in range(4000)]
+    cases += [(2 * R - 1 - rng.randrange(1 << 40), R - 1 - rng.randrange(1 << 40)) for _ in range(500)]
+    worst = 0
+    for x, w in cases:
+        y = mul29(x, w)
+        assert y % R == x * w * rinv % R, (hex(x), hex(w))
+        worst = max(worst, y)
+    assert worst < 1.031 * R
+
+
+def _table(text, name, rows, cols):
+    blk = text[text.index(name):]
+    blk = blk[: blk.index("}};") + 3]
+    nums = [int(v, 16) for v in re.findall(r"0x([0-9a-f]+)u", blk)]
+    assert len(nums) == 2 * rows * cols
+    return [[nums[(d * rows + e) * cols:(d * rows + e + 1) * cols] for e in range(rows)] for d in range(2)]
+
+
+def test_dft16_constants_are_reference_roots_times_2_261():
+    ref = json.load(open(os.path.join(HERE, "golden", "ref_constants.json")))
+    rinv = pow(1 << 256, -1, R)
+    w = int(ref["forward_roots_of_unity"][4], 16) * rinv % R
+    wi = int(ref["inverse_roots_of_unity"][4], 16) * rinv % R
+    assert pow(w, 16, R) == 1 and pow(w, 8, R) == R - 1 and w * wi % R == 1
+    tab = _table(open(os.path.join(CSRC, "ntt_pass.hpp")).read(), "kW16r[2][8][9]", 8, 9)
+    for d, base in ((0, w), (1, wi)):
+        for e in range(8):
+            assert all(v <= M29 for v in tab[d][e])
+            v = sum(l << (29 * i) for i, l in enumerate(tab[d][e]))
+            assert v == pow(base, e, R) * (1 << 261) % R, (d, e)
+
+
+def test_modulus_constants():
+    text = open(os.path.join(CSRC, "fr29.hpp")).read()
+    body = text[text.index("constexpr uint32_t r29(int i)"):]
+    r29 = [int(v, 16) for v in re.findall(r"0x([0-9a-f]+)u", body[: body.index(";")])]
+    assert r29 == limbs(R, 9, 29)
+    body = text[text.index("constexpr uint32_t r2_limb(int i)"):]
+    r2 = [int(v, 16) for v in re.findall(r"0x([0-9a-f]+)u", body[: body.index(";")])]
+    assert r2 == limbs(2 * R, 8, 32)
+
+
+def test_generated_columns_current_and_complete():
+    gen = os.path.join(CSRC, "gen_mont29_columns.py")
+    inc = os.path.join(CSRC, "mont29_columns.inc")
+    out = subprocess.check_output(["python3", gen]).decode()
+    assert out == open(inc).read()
+    # in each of the 36 blocks every product term of its column appears once
+    spec = re.findall(r"struct Mont29Col<(\d+), (true|false)> \{.*?asm\(\"([^\"]*)\"\s*: ([^\n]*)\s*: ([^;]*)\);",
+                      out, re.S)
+    assert len(spec) == 36
+    for k, ub, text, _outs, ins in spec:
+        k = int(k)
+        ops = ["acc", "d"] + [o.strip() for o in re.findall(r'"[vs]"\(([^)]*\)?)\)', ins)]
+        pairs = []
+        for t in text.split("\\n\\t"):
+            m = re.match(r"v_mad_u64_u32 %0, %1, %(\d+), %(\d+), (%0|0)", t)
+            if m:
+                pairs.append((ops[int(m.group(1))], ops[int(m.group(2))]))
+        want = set()
+        if k <= 16:
+            for i in range(max(0, k - 8), min(k, 8) + 1):
+                want.add(("x[%d]" % i, "w[%d]" % (k - i)))
+                if i < k:
+                    want.add(("m[%d]" % i, "r29(%d)" % (k - i)))
+        assert sorted(pairs) == sorted(want), k
