@@ -14,6 +14,7 @@
 // each block to one value, the block values are scanned by the same code one
 // level up (recursively), and a downsweep kernel rescans each block from its
 // carry.  About 3 mulmods per element in total, three memory sweeps.
+#include "fr29.hpp"
 #include "pl_internal.hpp"
 
 namespace pl {
@@ -162,8 +163,21 @@ static hipError_t prefix_product(const fr* in, fr* out_excl, fr* out_incl, uint6
 
 // ---------------------------------------------------------------------------
 // affine suffix scan  s_i = a_i + m * s_{i+1}
-// pw: device table of m^(2^j), j = 0..kLgSeg (m^kSeg last)
+// pw: device table of m^(2^j) R' mod r, j = 0..kLgSeg (m^kSeg last), R' = 2^261:
+// every product of the scan multiplies a value by a power of m, so it runs as
+// the radix-2^29 product (fr29.hpp) with the power in that form; results are
+// reduced to canonical form at once (the additions expect values below r).
 // ---------------------------------------------------------------------------
+
+// a * w R' / R' = a w (stored form), canonical; w9: the nine limbs of w R'
+__device__ __forceinline__ fr mulp(const fr& a, const uint32_t (&w9)[9]) { return fr_reduce_once(fr_mul29(a, w9)); }
+__device__ __forceinline__ fr mulp(const fr& a, const fr& w) { return fr_reduce_once(fr_mul29(a, w)); }
+__device__ __forceinline__ void split_fr(const fr& w, uint32_t (&w9)[9]) { split29(w.l, w9); }
+// R' mod r: the R'-form of one
+__host__ __device__ constexpr uint32_t rp_one_limb(int i) {
+  return i == 0 ? 0xffffffbau : i == 1 ? 0x00000045u : i == 2 ? 0x0072d846u : i == 3 ? 0x1a25272eu
+       : i == 4 ? 0x5dbeee8bu : i == 5 ? 0xfe2eedcdu : i == 6 ? 0x9eefbe41u : 0x4d043f42u;
+}
 
 // block suffix combine: V_t <- V_t + m^(16*2^s) V_{t+2^s}; returns the suffix
 // value starting at thread t (the block's carry folded in by the caller)
@@ -175,7 +189,7 @@ __device__ fr block_suffix(fr v, uint4 (*l)[kScanThreads], const fr* pw) {
     const int o = 1 << s;
     fr other = t + o < kScanThreads ? lds_get(l, t + o) : fr_zero();
     __syncthreads();
-    if (t + o < kScanThreads) v = fr_add(v, fr_mul(fr_load(pw + kLgPer + s), other));  // m^(kPer * 2^s)
+    if (t + o < kScanThreads) v = fr_add(v, mulp(other, fr_load(pw + kLgPer + s)));  // m^(kPer * 2^s)
     lds_put(l, t, v);
     __syncthreads();
   }
@@ -190,7 +204,8 @@ __global__ void __launch_bounds__(kScanThreads) sfx_reduce_kernel(const fr* a, u
   __shared__ uint4 l[2][kScanThreads];
   const int t = threadIdx.x;
   const uint64_t b0 = (uint64_t)blockIdx.x << kLgSeg;
-  const fr M = fr_load(pw + 8);  // m^256
+  uint32_t M[9];
+  split_fr(fr_load(pw + 8), M);  // m^256
   fr v[kPer];
   static_for<0, kPer>([&](auto k_) {
     constexpr int k = decltype(k_)::value;
@@ -200,15 +215,17 @@ __global__ void __launch_bounds__(kScanThreads) sfx_reduce_kernel(const fr* a, u
   fr h = v[kPer - 1];
   static_for<0, kPer - 1>([&](auto k_) {
     constexpr int k = kPer - 2 - decltype(k_)::value;
-    h = fr_add(fr_mul(h, M), v[k]);
+    h = fr_add(mulp(h, M), v[k]);
   });
-  // m^t from the powers m^(2^bit)
-  fr mt = fr_one();
+  // m^t R' from the powers m^(2^bit) R'
+  fr mt;
+#pragma unroll
+  for (int i = 0; i < 8; i++) mt.l[i] = rp_one_limb(i);
   static_for<0, 8>([&](auto b_) {
     constexpr int b = decltype(b_)::value;
-    if ((t >> b) & 1) mt = fr_mul(mt, fr_load(pw + b));
+    if ((t >> b) & 1) mt = mulp(mt, fr_load(pw + b));
   });
-  h = fr_mul(h, mt);
+  h = mulp(h, mt);
   lds_put(l, t, h);
   __syncthreads();
   for (int s = kScanThreads / 2; s > 0; s >>= 1) {
@@ -228,7 +245,8 @@ __global__ void __launch_bounds__(kScanThreads) sfx_down_kernel(const fr* a, fr*
   __shared__ uint4 l[2][kScanThreads];
   const int t = threadIdx.x;
   const uint64_t base = ((uint64_t)blockIdx.x << kLgSeg) + (uint64_t)t * kPer;
-  const fr m = fr_load(pw);
+  uint32_t m[9];
+  split_fr(fr_load(pw), m);
   fr e[kPer];
   static_for<0, kPer>([&](auto k_) {
     constexpr int k = decltype(k_)::value;
@@ -238,11 +256,11 @@ __global__ void __launch_bounds__(kScanThreads) sfx_down_kernel(const fr* a, fr*
   fr h = e[kPer - 1];
   static_for<0, kPer - 1>([&](auto k_) {
     constexpr int k = kPer - 2 - decltype(k_)::value;
-    h = fr_add(fr_mul(h, m), e[k]);
+    h = fr_add(mulp(h, m), e[k]);
   });
   // fold the block carry into the last thread: s(start of thread 255) = L + m^kPer K
   fr K = carry ? fr_load(carry + blockIdx.x) : fr_zero();
-  if (t == kScanThreads - 1) h = fr_add(h, fr_mul(fr_load(pw + kLgPer), K));
+  if (t == kScanThreads - 1) h = fr_add(h, mulp(K, fr_load(pw + kLgPer)));
   fr v = block_suffix(h, l, pw);  // s at the start of this thread's run
   // carry-in at this thread's end = s at the start of thread t+1
   fr cin = t + 1 < kScanThreads ? lds_get(l, t + 1) : K;
@@ -252,7 +270,7 @@ __global__ void __launch_bounds__(kScanThreads) sfx_down_kernel(const fr* a, fr*
   static_for<0, kPer>([&](auto k_) {
     constexpr int k = kPer - 1 - decltype(k_)::value;
     if (base + k < n) fr_store(out + base + k, sn);
-    sn = fr_add(e[k], fr_mul(m, sn));
+    sn = fr_add(e[k], mulp(sn, m));
   });
 }
 
@@ -266,24 +284,26 @@ hipError_t fill_zero(fr* p, uint64_t n, hipStream_t s) {
   return hipGetLastError();
 }
 
-// pw[j] = m^(2^j), j = 0..12; m = neg ? -(*src) : *src
+// pw[j] = m^(2^j) R' mod r, j = 0..12; m = neg ? -(*src) : *src.  32 m (five
+// doublings) is m R' in the stored form; squares stay in that form.
 __global__ void sfx_powers_kernel(const fr* src, int neg, fr* pw) {
   if (threadIdx.x != 0) return;
   fr m = fr_load(src);
   if (neg) m = fr_neg(m);
+  for (int d = 0; d < 5; d++) m = fr_add(m, m);
   for (int j = 0; j <= kLgSeg; j++) {
     fr_store(pw + j, m);
-    m = fr_sqr(m);
+    m = mulp(m, m);
   }
 }
 
-// pw_next[j] = pw[kLgSeg]^(2^j): powers of m^kSeg for the next level
+// pw_next[j] = pw[kLgSeg]^(2^j): powers of m^kSeg for the next level (R' form)
 __global__ void sfx_next_powers_kernel(const fr* pw, fr* pw_next) {
   if (threadIdx.x != 0) return;
   fr m = fr_load(pw + kLgSeg);
   for (int j = 0; j <= kLgSeg; j++) {
     fr_store(pw_next + j, m);
-    m = fr_sqr(m);
+    m = mulp(m, m);
   }
 }
 
