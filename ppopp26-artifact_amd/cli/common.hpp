@@ -78,6 +78,11 @@ inline void read_file(const char* filename, void* data, size_t bytes, uint64_t s
   file.seekg(0, std::ios::end);
   size_t size = file.tellg();
   file.seekg(0, std::ios::beg);
+  // the reference reads the whole file into the buffer whatever its size;
+  // here the read stops at the buffer and a short file is zero-padded
+  if (size != bytes)
+    std::cerr << "[input] " << filename << ": " << size << " bytes, buffer " << bytes
+              << (size > bytes ? " (extra bytes ignored)" : " (rest zero-filled)") << std::endl;
   file.read(reinterpret_cast<char*>(data), std::min(size, bytes));
   if (!file) std::cerr << "Error reading file: " << filename << std::endl;
   if (size < bytes) std::memset((char*)data + size, 0, bytes - size);
