@@ -71,15 +71,29 @@ int pl_ntt_batch(const void* d_in, void* d_out, int lg, uint32_t npoly, int inve
                  const pl_domain* d, void* d_work, pl_stream_t stream);
 size_t pl_ntt_work_bytes(int lg, uint32_t npoly);
 
-/* Pass-level access to the same transform (counterpart of the reference's
- * per-stage entry Ntt::stage_no_rotate, function.cu:312-314; the
- * intermediate layout is this library's own).  Running passes
+/* Pass-level access to the same transform (the intermediate layout is this
+ * library's own; the reference's per-chunk step with its own layout is
+ * pl_ntt_stage_no_rotate below).  Running passes
  * 0 .. pl_ntt_num_passes(lg)-1 in order, each reading the previous pass's
  * output, equals pl_ntt_batch.  Intermediate passes may run in place; the
  * last pass writes natural order and must not (d_in != d_out). */
 int pl_ntt_num_passes(int lg);
 int pl_ntt_pass(const void* d_in, void* d_out, int lg, uint32_t npoly, int pass, int inverse,
                 const pl_domain* d, pl_stream_t stream);
+
+/* The segmented NTT's per-chunk step with the reference's own semantics
+ * [Ntt::stage_no_rotate, PLONK/utils/function.cu:312-314;
+ *  _CT_NTT_without_rotate, zksnark_ntt/ntt_kernel/ct_mixed_radix_wide.cuh:610-751,
+ *  launched by CTkernel_no_rotate, ct_mixed_radix_wide.cuh:884-974].
+ * d_chunk holds chunk_elems (a power of two >= 2^iterations) elements
+ * gathered with the driver's ct_stage_indices (ntt.cu:168-178) as pairs
+ * (2t, 2t+1); the call runs DIT stages [stage, stage + iterations) of a
+ * forward NTT of size 2^lg_domain (lg_domain <= 28) in place, the result left
+ * in gathered order; twiddles use the global thread chunk_id*chunk_elems/2 + t.
+ * iterations in 1..8 or 10 (radix max(6, iterations) in {6,7,8,10}; 9 is
+ * rejected as by the reference).  d_params: a Params buffer (pl_params_fill). */
+int pl_ntt_stage_no_rotate(void* d_chunk, uint64_t chunk_elems, int lg_domain, int iterations, int stage,
+                           uint64_t chunk_id, const void* d_params, pl_stream_t stream);
 
 /* ---- coset low-degree extension ---------------------------------------
  * [our_lde, lde.cu:1267-1365 (lde_spread_powers_kernel lde.cu:623-648,

@@ -415,3 +415,152 @@ void or_perm_ratio(const uint64_t* a, const uint64_t* b, const uint64_t* c,
     or_mul(x, x, w);
   }
 }
+
+/* ---- segmented NTT step (stage_no_rotate) -------------------------------- */
+
+/* bit_rev(i, nbits) of kernels.cuh:11-16 (32-bit index_t) */
+static uint32_t rev32(uint32_t i, unsigned nbits) {
+  uint32_t r = 0;
+  for (int b = 0; b < 32; b++) r |= ((i >> b) & 1u) << (31 - b);
+  return nbits ? r >> (32 - nbits) : 0;
+}
+
+/* o = omega_{2^lg}^e (roots cached; square-and-multiply over e's bits) */
+static void root_pow(uint64_t* o, int lg, uint64_t e) {
+  static uint64_t cache[33][4];
+  static int have[33];
+  if (!have[lg]) {
+    or_root_of_unity(cache[lg], lg);
+    have[lg] = 1;
+  }
+  uint64_t acc[4], b[4];
+  or_one(acc);
+  memcpy(b, cache[lg], 32);
+  while (e) {
+    if (e & 1) or_mul(acc, acc, b);
+    e >>= 1;
+    if (e) or_mul(b, b, b);
+  }
+  memcpy(o, acc, 32);
+}
+
+/* ntt.cu:168-178 */
+void or_ct_stage_indices(uint64_t tid, int stage, int iterations, uint64_t* idx0, uint64_t* idx1) {
+  const uint64_t inp_ntt_size = 1ull << stage;
+  const uint64_t out_ntt_size = 1ull << (stage + iterations - 1);
+  const uint64_t thread_ntt_pos = (tid & (out_ntt_size - 1)) >> (iterations - 1);
+  uint64_t i0 = tid & ~(out_ntt_size - 1);
+  i0 += (tid << stage) & (out_ntt_size - 1);
+  i0 = i0 * 2 + thread_ntt_pos;
+  *idx0 = i0;
+  *idx1 = i0 + inp_ntt_size;
+}
+
+/* ntt.cu:190-195 */
+uint64_t or_rotate_stage_output_index(uint64_t global_idx, int stage, int iterations) {
+  const uint64_t mask = (1ull << (stage + iterations)) - (1ull << stage);
+  uint64_t rotated = global_idx & mask;
+  rotated = (rotated >> 1) | (rotated << (iterations - 1));
+  return (global_idx & ~mask) | (rotated & mask);
+}
+
+int or_stage_no_rotate(uint64_t* chunk, uint64_t chunk_elems, int lg_domain, int iterations, int stage,
+                       uint64_t chunk_id) {
+  const int it = iterations;
+  /* CTkernel_no_rotate: radix = max(6, it), cases 6, 7, 8, 10 */
+  const int radix = it < 6 ? 6 : it;
+  if (it < 1 || !(radix == 6 || radix == 7 || radix == 8 || radix == 10)) return -1;
+  if (lg_domain > 28 || stage < 0 || stage + it > lg_domain) return -1;
+  if (chunk_elems < 2 || (chunk_elems & (chunk_elems - 1)) || chunk_elems < (1ull << it)) return -1;
+  const uint64_t nthreads = chunk_elems / 2;
+  const uint64_t block = nthreads < (1ull << (radix - 1)) ? nthreads : (1ull << (radix - 1));
+  /* offset = chunk_id * blockDim * gridDim, index_t arithmetic */
+  const uint32_t offset = (uint32_t)(chunk_id * nthreads);
+  uint64_t* r0 = (uint64_t*)malloc(nthreads * 32);
+  uint64_t* r1 = (uint64_t*)malloc(nthreads * 32);
+  uint64_t* x = (uint64_t*)malloc(nthreads * 32);
+  /* intermediate_mul mode (ct_mixed_radix_wide.cuh:915-966): 0 at stage 0,
+   * 2 (table of w_{2^L}^(row col), rows of 2^radix) at (6,6) L=12, (6,12)
+   * L=18, (7,7) L=14, (8,8) L=16, else 1 (windowed w_{2^28} powers) */
+  int mode = stage == 0 ? 0 : 1, L = 0;
+  if (stage != 0) {
+    if (radix == 6 && stage == 6) { mode = 2; L = 12; }
+    if (radix == 6 && stage == 12) { mode = 2; L = 18; }
+    if (radix == 7 && stage == 7) { mode = 2; L = 14; }
+    if (radix == 8 && stage == 8) { mode = 2; L = 16; }
+  }
+  for (uint64_t t = 0; t < nthreads; t++) {
+    const uint32_t gtid = (uint32_t)t + offset;
+    memcpy(r0 + 4 * t, chunk + 4 * (2 * t), 32);
+    memcpy(r1 + 4 * t, chunk + 4 * (2 * t + 1), 32);
+    if (mode != 0) {
+      const uint32_t diff_mask = (1u << (it - 1)) - 1;
+      const uint32_t j = (gtid & diff_mask) * 2;
+      const uint32_t gsize = 1u << (stage + it - 1);
+      const uint32_t p = (gtid & (gsize - 1)) >> (it - 1);
+      uint64_t f0[4], f1[4];
+      if (mode == 1) {
+        const unsigned nbits = 28 - stage;
+        const uint32_t e0 = rev32(j, nbits) * p;
+        const uint32_t e1 = p << (nbits - 1);
+        uint64_t a[4];
+        root_pow(f0, 28, e0);
+        root_pow(a, 28, e1);
+        or_mul(f1, a, f0); /* second_root *= first_root */
+      } else {
+        int sh = L - lg_domain;
+        if (sh < 0) sh = 0;
+        const unsigned nb = (unsigned)sh + (unsigned)it;
+        const uint64_t i0 = ((uint64_t)p << radix) + rev32(j, nb);
+        const uint64_t i1 = ((uint64_t)p << radix) + rev32(j + 1, nb);
+        /* table[row * 2^radix + col] = w_{2^L}^(row col) */
+        root_pow(f0, L, (i0 >> radix) * (i0 & ((1ull << radix) - 1)));
+        root_pow(f1, L, (i1 >> radix) * (i1 & ((1ull << radix) - 1)));
+      }
+      or_mul(r0 + 4 * t, r0 + 4 * t, f0);
+      or_mul(r1 + 4 * t, r1 + 4 * t, f1);
+    }
+    uint64_t tt[4];
+    memcpy(tt, r1 + 4 * t, 32);
+    or_sub(r1 + 4 * t, r0 + 4 * t, tt);
+    or_add(r0 + 4 * t, r0 + 4 * t, tt);
+  }
+  uint64_t* tw = (uint64_t*)malloc((1ull << it) * 32);
+  for (int s = 1; s < it; s++) {
+    const uint64_t laneMask = 1ull << (s - 1), thrdMask = (1ull << s) - 1;
+    /* d_radix6_twiddles[rank << (6 - (s+1))] (s < 6) or
+     * radixX[rank << (radix - (s+1))]: w_{2^(s+1)}^rank either way */
+    for (uint64_t k = 0; k <= thrdMask; k++) {
+      if (s < 6)
+        root_pow(tw + 4 * k, 6, k << (6 - (s + 1)));
+      else
+        root_pow(tw + 4 * k, radix, k << (radix - (s + 1)));
+    }
+    for (uint64_t t = 0; t < nthreads; t++) {
+      const uint64_t rank = (t % block) & thrdMask;
+      memcpy(x + 4 * t, rank < laneMask ? r1 + 4 * t : r0 + 4 * t, 32);
+    }
+    for (uint64_t t = 0; t < nthreads; t++) {
+      const uint64_t rank = (t % block) & thrdMask;
+      const int pos = rank < laneMask;
+      const uint64_t* xp = x + 4 * (t ^ laneMask);
+      if (pos)
+        memcpy(r1 + 4 * t, xp, 32);
+      else
+        memcpy(r0 + 4 * t, xp, 32);
+      uint64_t tt[4];
+      or_mul(tt, tw + 4 * rank, r1 + 4 * t);
+      or_sub(r1 + 4 * t, r0 + 4 * t, tt);
+      or_add(r0 + 4 * t, r0 + 4 * t, tt);
+    }
+  }
+  for (uint64_t t = 0; t < nthreads; t++) {
+    memcpy(chunk + 4 * (2 * t), r0 + 4 * t, 32);
+    memcpy(chunk + 4 * (2 * t + 1), r1 + 4 * t, 32);
+  }
+  free(tw);
+  free(r0);
+  free(r1);
+  free(x);
+  return 0;
+}

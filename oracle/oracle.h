@@ -22,6 +22,10 @@
  *   eval            big-int Horner.
  *   div             big-int synthetic division + q(X)(X-z) + a(z) = a(X).
  *   perm ratio      big-int restatement of the product formula.
+ *   stage_no_rotate the reference driver's composition (bit-reverse, then per
+ *                   step: ct_stage_indices gather, one call per chunk,
+ *                   rotate_stage_output_index scatter) equals the NTT, for
+ *                   schedules covering every factor rule (tests/test_segmented.py).
  */
 #ifndef PL_ORACLE_H
 #define PL_ORACLE_H
@@ -100,6 +104,19 @@ void or_perm_ratio(const uint64_t* a, const uint64_t* b, const uint64_t* c,
                    const uint64_t* beta, const uint64_t* gamma,
                    const uint64_t* k1, const uint64_t* k2, const uint64_t* k3,
                    int lg_n, uint64_t* ratio);
+
+/* The segmented NTT's per-chunk step, forward direction: one call of
+ * Ntt::stage_no_rotate(chunk, lg_domain, iterations, stage, chunk_id)
+ * (function.cu:312-314 -> zksnark_ntt.cu:194-232, 440-454 -> ntt.cuh:642-665
+ * -> CTkernel_no_rotate, ct_mixed_radix_wide.cuh:884-974 ->
+ * _CT_NTT_without_rotate, ct_mixed_radix_wide.cuh:610-751), restated as its
+ * thread program over the chunk's chunk_elems / 2 threads.  Returns 0, or -1
+ * for a configuration the reference rejects (radix 9, sizes). */
+int or_stage_no_rotate(uint64_t* chunk, uint64_t chunk_elems, int lg_domain, int iterations, int stage,
+                       uint64_t chunk_id);
+/* the driver's index algebra (ntt.cu:168-195) */
+void or_ct_stage_indices(uint64_t tid, int stage, int iterations, uint64_t* idx0, uint64_t* idx1);
+uint64_t or_rotate_stage_output_index(uint64_t global_idx, int stage, int iterations);
 
 #ifdef __cplusplus
 }

@@ -59,6 +59,12 @@ def lib():
         L.or_div.argtypes = [_u64p, ctypes.c_uint64, _u64p, _u64p]
         L.or_naive_div.argtypes = [_u64p, ctypes.c_uint64, _u64p, _u64p, _u64p]
         L.or_perm_ratio.argtypes = [_u64p] * 11 + [ctypes.c_int, _u64p]
+        L.or_stage_no_rotate.argtypes = [_u64p, ctypes.c_uint64, ctypes.c_int, ctypes.c_int, ctypes.c_int,
+                                         ctypes.c_uint64]
+        L.or_stage_no_rotate.restype = ctypes.c_int
+        L.or_ct_stage_indices.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int, _u64p, _u64p]
+        L.or_rotate_stage_output_index.argtypes = [ctypes.c_uint64, ctypes.c_int, ctypes.c_int]
+        L.or_rotate_stage_output_index.restype = ctypes.c_uint64
     return _lib
 
 
@@ -227,3 +233,33 @@ def perm_ratio(cols, beta, gamma, ks, lg):
     args += [P(np.ascontiguousarray(k)) for k in ks]
     lib().or_perm_ratio(*args, lg, P(out))
     return out
+
+
+def stage_no_rotate(chunk, lg_domain, iterations, stage, chunk_id):
+    """One Ntt::stage_no_rotate call on a copy of `chunk` (n x 4 uint64)."""
+    out = np.ascontiguousarray(chunk, dtype=np.uint64).copy()
+    rc = lib().or_stage_no_rotate(P(out), out.shape[0], lg_domain, iterations, stage, chunk_id)
+    if rc != 0:
+        raise ValueError("configuration rejected")
+    return out
+
+
+def ct_stage_indices(tids, stage, iterations):
+    """ntt.cu:168-178, vectorised: global element pair of each gathered thread."""
+    tids = np.asarray(tids, dtype=np.uint64)
+    inp = np.uint64(1) << np.uint64(stage)
+    outs = np.uint64(1) << np.uint64(stage + iterations - 1)
+    pos = (tids & (outs - np.uint64(1))) >> np.uint64(iterations - 1)
+    i0 = tids & ~(outs - np.uint64(1))
+    i0 = i0 + ((tids << np.uint64(stage)) & (outs - np.uint64(1)))
+    i0 = i0 * np.uint64(2) + pos
+    return i0, i0 + inp
+
+
+def rotate_stage_output_index(g, stage, iterations):
+    """ntt.cu:190-195, vectorised."""
+    g = np.asarray(g, dtype=np.uint64)
+    mask = (np.uint64(1) << np.uint64(stage + iterations)) - (np.uint64(1) << np.uint64(stage))
+    rot = g & mask
+    rot = (rot >> np.uint64(1)) | (rot << np.uint64(iterations - 1))
+    return (g & ~mask) | (rot & mask)

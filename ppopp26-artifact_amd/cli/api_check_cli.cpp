@@ -3,7 +3,8 @@
 //   api_check <out_dir> <lg>
 // Files (raw 32-byte elements): x.bin y.bin (inputs), ntt.bin intt.bin,
 // gp.bin gp_incl.bin, eval.bin, div.bin, add.bin mul.bin div_ew.bin,
-// add_cpu.bin (CPU-dispatched add), lde.bin (our_lde semantics, lambda=2).
+// add_cpu.bin (CPU-dispatched add), lde.bin (our_lde semantics, lambda=2),
+// stage0.bin stage6.bin (Ntt::stage_no_rotate on seeded chunks).
 #include <string>
 
 #include "common.hpp"
@@ -68,6 +69,20 @@ int main(int argc, char** argv) {
     cli::check(pl_lde(x.mutable_gpu_data(), L.mutable_gpu_data(), ld, nullptr, nullptr), "pl_lde");
     pl_lde_domain_destroy(ld);
     dump(dir + "/lde.bin", L);
+    // Ntt::stage_no_rotate: the whole of x as one chunk at stage 0, and the
+    // first 1024 elements as chunk 1 of stages [6, 12)
+    SyncedMemory s0 = seeded(n, 1001), s1 = seeded(1024, 1001);
+    ntt.stage_no_rotate(s0, lg, 6, 0, 0);
+    ntt.stage_no_rotate(s1, lg, 6, 6, 1);
+    dump(dir + "/stage0.bin", s0);
+    dump(dir + "/stage6.bin", s1);
+    bool threw = false;
+    try {
+      intt.stage_no_rotate(s1, lg, 6, 6, 1);
+    } catch (const std::logic_error&) {
+      threw = true;
+    }
+    if (!threw) throw std::runtime_error("Intt::stage_no_rotate did not refuse");
     std::cout << "api_check ok" << std::endl;
     return 0;
   } catch (const std::exception& ex) {

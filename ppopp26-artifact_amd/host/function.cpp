@@ -4,6 +4,7 @@
 #include <hip/hip_runtime.h>
 
 #include <cstring>
+#include <stdexcept>
 
 #include "../../include/pl.h"
 #include "../csrc/fr_host.hpp"
@@ -201,6 +202,16 @@ void Ntt::forward_(SyncedMemory input, SyncedMemory output, hipStream_t s) {
   void* pi = input.mutable_gpu_data_async(s);
   void* po = output.mutable_gpu_data_async(s);
   check(pl_ntt(pi, po, lg, inverse_ ? 1 : 0, domain(lg), nullptr, (pl_stream_t)s), "pl_ntt");
+}
+
+void Ntt::stage_no_rotate(SyncedMemory input, int lg_domain_size, int iterations, int stage, int chunk_id,
+                          hipStream_t s) {
+  if (inverse_) throw std::logic_error("stage_no_rotate: forward transform only (Ntt::stage_no_rotate)");
+  void* pi = input.mutable_gpu_data_async(s);
+  const void* pp = Params.mutable_gpu_data_async(s);
+  check(pl_ntt_stage_no_rotate(pi, elems(input), lg_domain_size, iterations, stage, (uint64_t)chunk_id, pp,
+                               (pl_stream_t)s),
+        "pl_ntt_stage_no_rotate");
 }
 
 Intt::Intt(int domain_size, hipStream_t stream) : Ntt(domain_size, stream) { inverse_ = true; }

@@ -81,6 +81,13 @@ class Ntt {
   explicit Ntt(int domain_size, hipStream_t stream = (hipStream_t)0);
   SyncedMemory forward(SyncedMemory input, hipStream_t stream = (hipStream_t)0);
   void forward_(SyncedMemory input, SyncedMemory output, hipStream_t stream = (hipStream_t)0);
+  // [Ntt::stage_no_rotate, function.cu:312-314] forward DIT stages
+  // [stage, stage + iterations) of a size-2^lg_domain_size NTT on a chunk
+  // gathered with the driver's ct_stage_indices, in place, in gathered order
+  // (pl_ntt_stage_no_rotate).  Forward only, as in the reference; an Intt
+  // object throws.
+  void stage_no_rotate(SyncedMemory input, int lg_domain_size, int iterations, int stage, int chunk_id,
+                       hipStream_t stream = (hipStream_t)0);
   pl_domain* domain(int lg) { return cache_->get(lg); }
 
  protected:

@@ -46,6 +46,7 @@ SIGNATURES = {
     "pl_ntt_work_bytes": (ctypes.c_size_t, [_i, _u32]),
     "pl_ntt_num_passes": (_i, [_i]),
     "pl_ntt_pass": (_i, [_vp, _vp, _i, _u32, _i, _i, _vp, _vp]),
+    "pl_ntt_stage_no_rotate": (_i, [_vp, _u64, _i, _i, _i, _u64, _vp, _vp]),
     "pl_lde_domain_create": (_i, [_i, _i, _vp, ctypes.POINTER(_vp)]),
     "pl_lde_domain_destroy": (_i, [_vp]),
     "pl_lde_domain_bytes": (ctypes.c_size_t, [_vp]),
@@ -467,6 +468,22 @@ def ntt_pass(x, out, dom, p, npoly=1, inverse=False, stream=None):
     check("pl_ntt_pass", lib().pl_ntt_pass(_ptr(x), _ptr(out), dom.lg, npoly, p, 1 if inverse else 0, dom.h,
                                            _stream(stream)))
     return out
+
+
+def params(stream=None):
+    """The reference-layout Params buffer (4 x 16384 elements) on the device."""
+    import torch
+    out = torch.empty((4 * 16384, 4), dtype=torch.int64, device="cuda")
+    check("pl_params_fill", lib().pl_params_fill(_ptr(out), _stream(stream)))
+    return out
+
+
+def stage_no_rotate(chunk, lg_domain, iterations, stage, chunk_id, params_buf, stream=None):
+    """One Ntt::stage_no_rotate call on a device chunk (n x 4 int64 tensor), in place."""
+    check("pl_ntt_stage_no_rotate",
+          lib().pl_ntt_stage_no_rotate(_ptr(chunk), chunk.shape[0], lg_domain, iterations, stage, chunk_id,
+                                       _ptr(params_buf), _stream(stream)))
+    return chunk
 
 
 def ntt_num_passes(lg):

@@ -45,6 +45,8 @@ def test_cpp_api(tmp_path):
     np.testing.assert_array_equal(f("add_cpu.bin"), po.add(x, y))
     np.testing.assert_array_equal(f("mul_cpu.bin"), po.mul(x, y))
     np.testing.assert_array_equal(f("lde.bin"), po.lde(x, lg, 2, po.ref_lde_shift(2)))
+    np.testing.assert_array_equal(f("stage0.bin"), po.stage_no_rotate(x, lg, 6, 0, 0))
+    np.testing.assert_array_equal(f("stage6.bin"), po.stage_no_rotate(x[:1024], lg, 6, 6, 1))
 
 
 @pytest.mark.parametrize("mode", ["--check-split", "--check-baseline"])
