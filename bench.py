@@ -10,6 +10,7 @@ input already in HBM): K back-to-back NTTs bracketed by HIP events on the
 launch stream.  Beside it, in the same line:
 
   lde      coset LDE 2^24 -> 2^26 (lambda = 2, shift 7^4), device-resident
+  lde_l3   coset LDE 2^24 -> 2^27 (lambda = 3, shift 7^8)
   batched  the ragged proof batch (64 polynomials, lg 12..20: 40 NTT, 16 LDE
            lambda = 2, 8 evaluation + division), one hipGraph launch versus one
            launch sequence per polynomial; per-run latency p50 / p99
@@ -461,6 +462,9 @@ def main():
     if not args.no_lde:
         result["lde"] = bench_lde(pl, synth, lg, 2, args.steps, args.warmup, stream, peak)
         progress("lde: %.3f ms" % result["lde"]["ms"])
+        torch.cuda.empty_cache()
+        result["lde_l3"] = bench_lde(pl, synth, lg, 3, max(1, args.steps // 2), args.warmup, stream, peak)
+        progress("lde lambda 3: %.3f ms" % result["lde_l3"]["ms"])
         torch.cuda.empty_cache()
     if not args.no_miss:
         result["miss_ops"] = bench_miss_ops(pl, synth, args.steps, stream, peak)
