@@ -164,27 +164,25 @@ __global__ void __launch_bounds__(kOpThreads) inv_down_kernel(const fr* in, cons
     }
     e[k] = v;
   });
-  // thread product, then block exclusive prefix / suffix of thread products
-  fr tp = e[0];
-  static_for<1, kOpPer>([&](auto k_) { tp = fr_mul(tp, e[decltype(k_)::value]); });
-  fr suf = block_scan_prod(tp, l, true);  // product of threads > t
-  // inverse of (prefix through this thread) = binv * suffix(> t)
-  fr inv_upto = fr_mul(fr_load(binv + blockIdx.x), suf);
-  fr pre = block_scan_prod(tp, l, false);  // product of threads < t
-  // local prefix Q_k = pre * e_0 ... e_k, kept in e[] (Q_-1 = pre)
-  e[0] = fr_mul(pre, e[0]);
+  // local prefix L_k = e_0 ... e_k in e[]; the thread product is L_15
   static_for<1, kOpPer>([&](auto k_) {
     constexpr int k = decltype(k_)::value;
     e[k] = fr_mul(e[k - 1], e[k]);
   });
-  // walk back: inv(x_k) = inv(Q_k) * Q_{k-1}; inv(Q_{k-1}) = inv(Q_k) * x_k
-  fr cur = inv_upto;  // inv(Q_15)
+  const fr tp = e[kOpPer - 1];
+  fr suf = block_scan_prod(tp, l, true);   // product of threads > t
+  fr pre = block_scan_prod(tp, l, false);  // product of threads < t
+  // inv(L_15) = binv * suffix(> t) * prefix(< t): the block product over
+  // everything outside this thread's run, times the block's inverse
+  fr cur = fr_mul(fr_mul(fr_load(binv + blockIdx.x), suf), pre);
+  // walk back: inv(x_k) = inv(L_k) L_{k-1} (inv(x_0) = inv(L_0));
+  // inv(L_{k-1}) = inv(L_k) x_k
   static_for<0, kOpPer>([&](auto k_) {
     constexpr int k = kOpPer - 1 - decltype(k_)::value;
-    const fr qprev = k > 0 ? e[k - 1] : pre;
-    fr inv_k = fr_mul(cur, qprev);
+    fr inv_k = cur;
     if (k > 0) {
-      // x_k = Q_k / Q_{k-1}: recover from the original input (one again for zeros)
+      inv_k = fr_mul(cur, e[k > 0 ? k - 1 : 0]);
+      // x_k from the original input (one again for zeros)
       fr xk = base + k < n ? fr_load(in + base + k) : fr_one();
       if ((zmask >> k) & 1) xk = fr_one();
       cur = fr_mul(cur, xk);
@@ -249,12 +247,14 @@ __global__ void __launch_bounds__(256) perm_terms_kernel(RatioArgs a) {
   const fr beta = fr_load(a.beta), gamma = fr_load(a.gamma);
   const fr wi = fr_mul(fr_load(a.w0 + (i & 16383)), fr_load(a.w1 + (i >> 14)));
   const fr bx = fr_mul(beta, wi);  // beta * w^i
-  fr num = fr_one(), den = fr_one();
+  fr num, den;
 #pragma unroll
   for (int j = 0; j < 3; j++) {
     const fr wj = fr_add(fr_load(a.w[j] + i), gamma);
-    num = fr_mul(num, fr_add(wj, fr_mul_u(bx, a.k[j])));
-    den = fr_mul(den, fr_add(wj, fr_mul(beta, fr_load(a.s[j] + i))));
+    const fr nj = fr_add(wj, fr_mul_u(bx, a.k[j]));
+    const fr dj = fr_add(wj, fr_mul(beta, fr_load(a.s[j] + i)));
+    num = j == 0 ? nj : fr_mul(num, nj);  // the first factor needs no product
+    den = j == 0 ? dj : fr_mul(den, dj);
   }
   fr_store(a.num + i, num);
   fr_store(a.den + i, den);

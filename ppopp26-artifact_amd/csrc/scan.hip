@@ -126,7 +126,7 @@ __global__ void __launch_bounds__(kScanThreads) pfx_down_kernel(const fr* in, fr
   static_for<0, kPer>([&](auto k_) {
     constexpr int k = decltype(k_)::value;
     if (base + k < n) {
-      fr ex = k == 0 ? pre : fr_mul(pre, e[k - 1]);
+      fr ex = k == 0 ? pre : fr_mul(pre, e[k > 0 ? k - 1 : 0]);
       if (out_incl) fr_store(out_incl + base + k, fr_mul(pre, e[k]));
       fr_store(out_excl + base + k, ex);
     }
