@@ -5,22 +5,24 @@
 // both operands are split into nine 29-bit limbs: every partial product is
 // below 2^58 and a column holds at most 17 of them plus a carry below 2^35, so
 // a column sums in one 64-bit accumulator with no carry handling: a chain of
-// v_mad_u64_u32 whose carry-out masks are never read (153 multiply-adds and
-// about 80 other instructions per product, against 120 + 120 folds + about
+// v_mad_i64_i32 whose carry-out masks are never read (153 multiply-adds and
+// about 70 other instructions per product, against 120 + 120 folds + about
 // 50 for the saturated form).  `mont29` is the plain-C statement (host tests);
 // the device runs the same columns as generated asm blocks (`mont29_dev`).
 //
-//   R' = 2^261 (nine digits), r = 1 mod 2^29 so the digit is m = -acc mod 2^29.
-//   mont29(a, w) = a * w / R' mod r, output in [0, 1.03 r) for a < 2r, w < r.
+//   R' = 2^261 (nine digits), r = 1 mod 2^29: with the signed digit
+//   m = acc mod 2^29, subtracting m r clears the low 29 bits exactly.
+//   a * w / R' mod r, output in (0, 1.031 r) for a < 2^256, w < r.
 //
 // Data stays in the stored form (x R mod r, R = 2^256, eight 32-bit limbs,
 // lazily in [0, 2r)); a factor w enters as w R' mod r = (w R) 2^5 mod r in
 // nine 29-bit limbs (`fr9`), so mont29(x R, w R') = x w R.  Tables and
 // constants for the passes are kept in that form.
 //
-// Column bound: a_i, w_j, m_i, r_j < 2^29 (a < 2^256 gives a_8 < 2^24):
-// 17 * 2^58 + 2^35 < 2^63.  Output bound: (a w + m r) / R' < (a r + R' r) / R'
-// = r (1 + a / R'), below 1.03 r for a < 2r and below 1.031 r for any a < 2^256.
+// Column bound: a_i, w_j, m_i, r_j < 2^29 (a < 2^256 gives a_8 < 2^24): the
+// signed column sum stays within 9 * 2^58 + 2^35 < 2^62 in magnitude.  Value:
+// T = (a w - m r) / R' lies in (-r, a r / R') = (-r, 0.031 r), and the
+// product returns T + r.
 #pragma once
 #include <stdint.h>
 #if defined(__HIPCC__)
@@ -47,6 +49,8 @@ PL_HD29 constexpr uint32_t r29(int i) {
        : i == 4 ? 0x1d80553bu : i == 5 ? 0x0c0404d0u : i == 6 ? 0x1520cce7u : i == 7 ? 0x0a6533afu
        : 0x0073eda7u;
 }
+// -r_i as a 32-bit two's complement (the signed multiply-add's operand)
+PL_HD29 constexpr uint32_t nr29(int i) { return 0u - r29(i); }
 
 // bits [s, s + 32) of the 64-bit value hi:lo
 PL_HD29 uint32_t funnel_r(uint32_t hi, uint32_t lo, int s) {
@@ -82,29 +86,51 @@ PL_HD29 void join29(const uint32_t* t, uint32_t* x) {
   x[7] = (t[7] >> 21) | (t[8] << 8);
 }
 
-// t = a w / 2^261 mod r (a, w in nine 29-bit limbs); t normalized, t < 1.03 r
-// when a < 2r and w < r.  Product scanning: column k adds a_i w_{k-i} and
-// m_i r_{k-i} (k - i >= 1; r_0 = 1 is the digit step itself).
+// t = a w / 2^261 - r (a, w in nine 29-bit limbs; a w / 2^261 taken mod r as
+// below): product scanning with a signed digit.  Column k adds a_i w_{k-i}
+// and subtracts m_i r_{k-i} (k - i >= 1) in a signed 64-bit accumulator; the
+// digit m_k = acc mod 2^29 makes acc - m_k r_0 = acc - m_k divisible by 2^29,
+// so the column closes with an arithmetic shift.  The value
+// T = (a w - m r) / 2^261 lies in (-r, 0.031 r) for a < 2^256, w < r; t_0..t_7
+// are its normalized limbs and t_8 its signed top (two's complement in 32
+// bits).  add_r256 then gives T + r in (0, 1.031 r).
 PL_HD29 void mont29(const uint32_t* a, const uint32_t* w, uint32_t* t) {
   uint32_t m[9];
-  uint64_t acc = 0;
+  int64_t acc = 0;
 #pragma unroll
   for (int k = 0; k < 17; k++) {
     const int lo = k > 8 ? k - 8 : 0, hi = k < 8 ? k : 8;
 #pragma unroll
-    for (int i = lo; i <= hi; i++) acc += (uint64_t)a[i] * w[k - i];
+    for (int i = lo; i <= hi; i++) acc += (int64_t)a[i] * (int64_t)w[k - i];
 #pragma unroll
     for (int i = lo; i <= hi; i++)
-      if (i < k) acc += (uint64_t)m[i] * r29(k - i);
-    if (k <= 8) {
-      m[k] = (0u - (uint32_t)acc) & kMask29;
-      acc = (acc + m[k]) >> 29;
-    } else {
+      if (i < k) acc -= (int64_t)m[i] * (int64_t)r29(k - i);
+    if (k <= 8)
+      m[k] = (uint32_t)acc & kMask29;
+    else
       t[k - 9] = (uint32_t)acc & kMask29;
-      acc >>= 29;
-    }
+    acc >>= 29;  // arithmetic: floor division
   }
   t[8] = (uint32_t)acc;
+}
+
+// x (eight 32-bit limbs, a value mod 2^256) += r
+PL_HD29 void add_r256(uint32_t* x) {
+  constexpr uint32_t kR[8] = {0x00000001u, 0xffffffffu, 0xfffe5bfeu, 0x53bda402u,
+                              0x09a1d805u, 0x3339d808u, 0x299d7d48u, 0x73eda753u};
+#if defined(__HIP_DEVICE_COMPILE__)
+  uint32_t c = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) x[i] = __builtin_addc(x[i], kR[i], c, &c);
+#else
+  uint64_t c = 0;
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const uint64_t s = (uint64_t)x[i] + kR[i] + c;
+    x[i] = (uint32_t)s;
+    c = s >> 32;
+  }
+#endif
 }
 
 #undef PL_HD29
@@ -165,7 +191,7 @@ __device__ __forceinline__ void mont29_dev(const uint32_t* x, const uint32_t* w,
     constexpr int k = decltype(k_)::value;
     Mont29Col<k, UB>::run(acc[k], k ? acc[k ? k - 1 : 0] : 0, x, w, m);
     if constexpr (k <= 8)
-      m[k] = (0u - (uint32_t)acc[k]) & kMask29;
+      m[k] = (uint32_t)acc[k] & kMask29;
     else
       t[k - 9] = (uint32_t)acc[k] & kMask29;
   });
@@ -182,6 +208,7 @@ __device__ __forceinline__ fr fr_mul29(const fr& a, const uint32_t (&w)[9]) {
   mont29_dev<UB>(x, w, t);
   fr o;
   join29(t, o.l);
+  add_r256(o.l);
   return o;
 }
 

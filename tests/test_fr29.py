@@ -1,9 +1,9 @@
 """The radix-2^29 Montgomery product of the NTT passes (csrc/fr29.hpp) and
 its constants, on the host.
 
-* fr29.hpp's plain-C product, built with g++, against Python integers:
-  mont29(x, w) = x w 2^-261 mod r and below 1.031 r for any 256-bit x and
-  w < r (the bound the lazy [0, 2r) butterflies rely on);
+* fr29.hpp's plain-C product (signed digit, then + r), built with g++,
+  against Python integers: x w 2^-261 mod r, and in (0, 1.031 r) for any
+  256-bit x and w < r (the bound the lazy [0, 2r) butterflies rely on);
 * the DFT-16 constants in ntt_pass.hpp against the reference's own
   16th roots of unity (tests/golden/ref_constants.json), times 2^261;
 * r and 2r as written in fr29.hpp;
@@ -42,6 +42,7 @@ def mul29(tmp_path_factory):
         "  pl::split29(x, a);\n"
         "  pl::mont29(a, w9, t);\n"
         "  pl::join29(t, y);\n"
+        "  pl::add_r256(y);\n"
         "}\n" % os.path.abspath(os.path.join(CSRC, "fr29.hpp"))
     )
     lib = d / "libfr29.so"
@@ -119,7 +120,7 @@ def test_generated_columns_current_and_complete():
         ops = ["acc", "d"] + [o.strip() for o in re.findall(r'"[vs]"\(([^)]*\)?)\)', ins)]
         pairs = []
         for t in text.split("\\n\\t"):
-            m = re.match(r"v_mad_u64_u32 %0, %1, %(\d+), %(\d+), (%0|0)", t)
+            m = re.match(r"v_mad_i64_i32 %0, %1, %(\d+), %(\d+), (%0|0)", t)
             if m:
                 pairs.append((ops[int(m.group(1))], ops[int(m.group(2))]))
         want = set()
@@ -127,5 +128,5 @@ def test_generated_columns_current_and_complete():
             for i in range(max(0, k - 8), min(k, 8) + 1):
                 want.add(("x[%d]" % i, "w[%d]" % (k - i)))
                 if i < k:
-                    want.add(("m[%d]" % i, "r29(%d)" % (k - i)))
+                    want.add(("m[%d]" % i, "nr29(%d)" % (k - i)))
         assert sorted(pairs) == sorted(want), k
