@@ -138,9 +138,10 @@ def bench_ntt(pl, synth, lg, steps, warmup, stream, peak):
         for _ in range(2):
             pl.ntt_pass(src, dst, dom, p)
         pass_ms.append(time_events(lambda: pl.ntt_pass(src, dst, dom, p), steps, stream))
-    # the library's plan: ceil(lg/8) passes whose sizes differ by <= 1, larger first
-    base, extra = lg // npass, lg % npass
-    m_pass = [base + (1 if p < extra else 0) for p in range(npass)]
+    # the pass split in use (tuned or default); pass kernels hold 2^3 elements
+    # per lane, so a 2^m-point column runs ntt_pass_kernel<m - 3, ...>
+    m_pass = pl.plan_get(lg)
+    assert len(m_pass) == npass
     dom_ms = pass_ms[0]
     mad = (n // 2) * m_pass[0] * MAD_PER_MULMOD
     achieved = mad / (dom_ms * 1e-3)
@@ -154,7 +155,7 @@ def bench_ntt(pl, synth, lg, steps, warmup, stream, peak):
         "traffic": TRAFFIC_PASS0_2P24 if lg == 24 else None,
         "peak_source": "measured v_mad_u64_u32 rate (pl_probe_mad_rate); spec full rate %.2f" % (
             PEAK_MAD_SPEC / 1e12),
-        "kernel": "ntt_pass_kernel<%d,false,false> (pass 0 of %d, 2^%d-point columns)" % (m_pass[0] - 4, npass,
+        "kernel": "ntt_pass_kernel<%d,false,false> (pass 0 of %d, 2^%d-point columns)" % (m_pass[0] - 3, npass,
                                                                                           m_pass[0]),
         "alg_per_launch": "%d mulmod = (N/2)*%d, x128 u32 multiply-adds; %d bytes = 64 N" % (
             (n // 2) * m_pass[0], m_pass[0], byts),

@@ -99,7 +99,7 @@ bool plan_valid(int lg, const int* m, int P) {
   if (lg < 4 || lg > 28 || P < 1 || P > PL_MAX_PASSES) return false;
   int sum = 0;
   for (int i = 0; i < P; i++) {
-    if (m[i] < 4 || m[i] > 8) return false;
+    if (m[i] < kLgPerT || m[i] > kLgPerT + (kLgPerT < 4 ? kLgPerT : 4)) return false;
     sum += m[i];
   }
   return sum == lg;
@@ -119,7 +119,8 @@ NttPlan default_plan(int lg) {
   NttPlan p;
   p.lg = lg;
   if (lg < 4) return p;
-  const int P = (lg + 7) / 8;
+  const int MX = kLgPerT + (kLgPerT < 4 ? kLgPerT : 4);
+  const int P = (lg + MX - 1) / MX;
   int m[PL_MAX_PASSES];
   const int base = lg / P, rem = lg % P;
   for (int i = 0; i < P; i++) m[i] = base + (i < rem ? 1 : 0);
@@ -336,7 +337,7 @@ hipError_t ntt_launch_one(const Domain& d, const NttCall& c, int p, const fr* in
     hipLaunchKernelGGL(ntt_tiny_kernel, dim3(grid_for(total, 256)), dim3(256), 0, s, a, c.npoly);
     return hipGetLastError();
   }
-  launch_pass_la(P.m[p] - 4, c.inverse, p == P.P - 1, a, s);
+  launch_pass_la(P.m[p] - kLgPerT, c.inverse, p == P.P - 1, a, s);
   return hipGetLastError();
 }
 

@@ -3,7 +3,7 @@
 //
 // Algorithm: four-step (Bailey) decimation in frequency, applied recursively.
 // A size-N transform, N = 2^lg, is split into passes of M' = 2^m points
-// (4 <= m <= 8).  With N = N1*N2, j = j1 + N1*j2, k = k2 + N2*k1:
+// (3 <= m <= 6).  With N = N1*N2, j = j1 + N1*j2, k = k2 + N2*k1:
 //
 //   X[k2 + N2 k1] = sum_j1 w_N1^(j1 k1) * ( w_N^(j1 k2) * sum_j2 x[j1 + N1 j2] w_N2^(j2 k2) )
 //
@@ -16,24 +16,16 @@
 // (ntt.cu:979-1037; ntt_kernel/algorithm.cuh:8-59; ct_mixed_radix_wide.cuh:47-189)
 // with the same mathematical result.
 //
-// Workgroup: 128 threads own a tile of 2048 elements (16 per thread, all in
-// VGPRs).  A column of M' = 16*A points is computed as DFT_16 over rho2 (one
-// per thread, in registers), an inner twiddle w_M'^(rho1 k2), one exchange
-// through LDS, then DFT_A over rho1 (16/A per thread, in registers).  The
-// tile holds C = 2048/M' adjacent columns, so every global access moves
-// C*32 >= 256 contiguous bytes.  The in-register DFTs are radix-2 DIF
-// networks whose twiddles (powers of w_16) are compile-time constants read
-// from SGPRs.
-//
-// Arithmetic: every product in a pass multiplies a data value by a constant
-// or table factor, so it uses the radix-2^29 product of fr29.hpp with the
-// factor held as w R' mod r (R' = 2^261): the DFT constants below, and the
-// domain's inner and pass twiddle tables, which are built in that form.  The
-// data stays in the stored form x R mod r, kept lazily in [0, 2r) between
-// operations and reduced once before each store, so a pass's output is
-// canonical and bit-identical to any exact implementation.  The optional
-// per-element factor table of the first pass (coset powers) is in the stored
-// form and uses the saturated product of fr.hpp.
+// Workgroup: 128 lanes own a tile of 1024 elements (8 per lane, all in
+// VGPRs, about 110-128 VGPRs per lane, so four waves share each SIMD: one
+// wave alone cannot issue multiply-adds back to back, and four reach ~15 %
+// more than two).  A column of M' = 8*A points is computed as DFT_8 over
+// rho2 (one per lane, in registers), an inner twiddle w_M'^(rho1 k2), one
+// exchange through LDS (16 KiB per workgroup in two rounds), then DFT_A over
+// rho1 (8/A per lane, in registers).  The tile holds C = 1024/M' adjacent
+// columns, so every global access moves C*32 >= 512 contiguous bytes.  The
+// in-register DFTs are radix-2 DIF networks whose twiddles (powers of w_16)
+// are compile-time constants read from SGPRs.
 #pragma once
 #include "fr29.hpp"
 #include "pl_internal.hpp"
@@ -93,7 +85,7 @@ __device__ __forceinline__ void dft_dif(fr (&x)[NX]) {
       static_for<0, h>([&](auto i_) {
         constexpr int i = decltype(i_)::value;
         // one butterfly at a time: the empty asm statements order the
-        // butterflies, so the live set stays at the 16 elements plus one
+        // butterflies, so the live set stays at the lane's elements plus one
         // butterfly's temporaries
         fr a = x[OFF + s + i];
         fr b = x[OFF + s + i + h];
@@ -137,21 +129,29 @@ struct PassArgs {
 };
 
 // Tile geometry (compile-time; see the file header):
-//   PL_NTT_LGTILE  log2 elements per workgroup tile (16 per thread)
+//   PL_NTT_LGPER   log2 elements per lane (3: 8 elements, DFT_8 in registers)
+//   PL_NTT_LGTILE  log2 elements per workgroup tile (128 lanes x 8)
 //   PL_NTT_PHASES  LDS exchange rounds (8/PHASES limbs each)
 //   PL_NTT_WAVES   waves per SIMD the register allocation must allow
+// Passes have 2^m-point columns, m = PL_NTT_LGPER + LA, LA <= min(4, LGPER).
 #ifndef PL_NTT_LGTILE
-#define PL_NTT_LGTILE 11
+#define PL_NTT_LGTILE 10
 #endif
+#ifndef PL_NTT_LGPER
+#define PL_NTT_LGPER 3
+#endif
+constexpr int kLgPerT = PL_NTT_LGPER;
+constexpr int kPerT = 1 << kLgPerT;
+constexpr int kPmG = kPerT / 4;  // factor-table loads in flight per group (first pass of an LDE)
 #ifndef PL_NTT_PHASES
 #define PL_NTT_PHASES 2
 #endif
 #ifndef PL_NTT_WAVES
-#define PL_NTT_WAVES 2
+#define PL_NTT_WAVES 4
 #endif
 constexpr int kLgTile = PL_NTT_LGTILE;
 constexpr int kTile = 1 << kLgTile;
-constexpr int kThreads = kTile / 16;
+constexpr int kThreads = kTile / kPerT;
 constexpr int kPhases = PL_NTT_PHASES;
 constexpr int kW = 8 / kPhases;  // limbs per exchange round
 
@@ -177,10 +177,10 @@ __device__ __forceinline__ void unpack_limbs(uint32_t* v, const typename lds_vec
 template <int LA, bool INV, bool LAST>
 __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, const uint64_t blk) {
   constexpr int A = 1 << LA;
-  constexpr int m = LA + 4;
+  constexpr int m = LA + kLgPerT;
   constexpr int LGC = kLgTile - m;  // log2 lines per tile
   constexpr int C = 1 << LGC;
-  constexpr int F2 = 16 / A;   // DFT_A groups per thread in sub-step 2
+  constexpr int F2 = kPerT / A;   // DFT_A groups per thread in sub-step 2
   typedef typename lds_vec<kW>::type vec;
   __shared__ vec lds[kTile];  // kW limbs of every tile element
 
@@ -188,7 +188,7 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, const uint64_t 
   const uint64_t line0 = blk << LGC;
   const uint32_t lgL = a.lgN - m;  // log2 lines per polynomial = log2(N / M')
 
-  // ---- sub-step 1: 16 points of one (line, rho1), DFT_16 over rho2 --------
+  // ---- sub-step 1: kPerT points of one (line, rho1), DFT over rho2 ----------
   // rho = rho1 + A*rho2.  Intermediate passes map lanes to adjacent columns
   // (coalesced strided rows); the last pass reads contiguous sub-rows, so
   // lanes take adjacent rho1.
@@ -201,7 +201,7 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, const uint64_t 
     rho1 = t >> LGC;
   }
   const uint64_t line = line0 + lc;
-  fr x[16];
+  fr x[kPerT];
   if (line < a.nlines) {
     uint64_t poly, pos0, estride;
     if (LAST) {
@@ -232,46 +232,46 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, const uint64_t 
       estride = 1ull << a.lgS;
     }
     const fr* src = a.in + poly * a.in_pstride;
-    static_for<0, 16>([&](auto r2_) {
+    static_for<0, kPerT>([&](auto r2_) {
       constexpr int r2 = decltype(r2_)::value;
       x[r2] = fr_load(src + pos0 + (uint64_t)(rho1 + A * r2) * estride);
     });
     if (!LAST && a.premul) {
       const fr* pm = a.premul + ((uint64_t)(poly & a.pm_mask) << a.lgN) + pos0;
-      // four factors at a time: each group's loads start after the previous
-      // group's products (the asm statements order them), so at most four
-      // factors are held in registers beside the 16 elements
-      static_for<0, 4>([&](auto g_) {
+      // kPmG factors at a time: each group's loads start after the previous
+      // group's products (the asm statements order them), so at most kPmG
+      // factors are held in registers beside the lane's elements
+      static_for<0, kPerT / kPmG>([&](auto g_) {
         constexpr int g = decltype(g_)::value;
         const fr* q = pm;
         asm volatile("" : "+v"(q));
-        fr f[4];
-        static_for<0, 4>([&](auto r_) {
+        fr f[kPmG];
+        static_for<0, kPmG>([&](auto r_) {
           constexpr int r = decltype(r_)::value;
-          f[r] = fr_load(q + (uint64_t)(rho1 + A * (4 * g + r)) * estride);
+          f[r] = fr_load(q + (uint64_t)(rho1 + A * (kPmG * g + r)) * estride);
         });
-        static_for<0, 4>([&](auto r_) {
+        static_for<0, kPmG>([&](auto r_) {
           constexpr int r = decltype(r_)::value;
-          x[4 * g + r] = fr_mul(x[4 * g + r], f[r]);
-          pin(x[4 * g + r]);
+          x[kPmG * g + r] = fr_mul(x[kPmG * g + r], f[r]);
+          pin(x[kPmG * g + r]);
         });
       });
     }
   } else {
-    static_for<0, 16>([&](auto r2_) { x[decltype(r2_)::value] = fr_zero(); });
+    static_for<0, kPerT>([&](auto r2_) { x[decltype(r2_)::value] = fr_zero(); });
   }
-  dft_dif<4, INV, 0>(x);  // x[brev4(k2)] = sum_rho2 z[rho1 + A rho2] w16^(rho2 k2)
+  dft_dif<kLgPerT, INV, 0>(x);  // x[brev4(k2)] = sum_rho2 z[rho1 + A rho2] w16^(rho2 k2)
   if constexpr (LA > 0) {
-    static_for<1, 16>([&](auto k2_) {
+    static_for<1, kPerT>([&](auto k2_) {
       constexpr int k2 = decltype(k2_)::value;
       const int e = rho1 * k2;  // < M'
-      pin(x[brev(k2, 4)]);
-      x[brev(k2, 4)] = fr_mul29(x[brev(k2, 4)], fr_load(a.inner29 + (e << (8 - m))));
+      pin(x[brev(k2, kLgPerT)]);
+      x[brev(k2, kLgPerT)] = fr_mul29(x[brev(k2, kLgPerT)], fr_load(a.inner29 + (e << (8 - m))));
     });
   }
   // Exchange through LDS in kPhases rounds of kW limbs (2 rounds of 16 bytes
-  // = 32 KiB for a 2048-element tile, so four workgroups = two waves per SIMD
-  // fit a CU).  LDS slot of W[line][rho1][k2]; consecutive lanes hit
+  // = 16 KiB for a 1024-element tile, so eight workgroups = four waves per
+  // SIMD fit a CU).  LDS slot of W[line][rho1][k2]; consecutive lanes hit
   // consecutive slots.  Each round's gathered limbs go back into the
   // registers that round vacated: y lives in x (x[f*A + r1] afterwards).
   // The line index is XOR-swizzled with rho1 so that both the line-major
@@ -283,17 +283,17 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, const uint64_t 
   asm volatile("" : "+v"(t2));
   const int lc2 = t2 & (C - 1);
   const int g = t2 >> LGC;
-  fr (&y)[16] = x;
+  fr (&y)[kPerT] = x;
   static_for<0, kPhases>([&](auto h_) {
     constexpr int h = decltype(h_)::value;
-    uint32_t hv[16][kW];
-    static_for<0, 16>([&](auto k2_) {
+    uint32_t hv[kPerT][kW];
+    static_for<0, kPerT>([&](auto k2_) {
       constexpr int k2 = decltype(k2_)::value;
 #pragma unroll
-      for (int w = 0; w < kW; w++) hv[k2][w] = x[brev(k2, 4)].l[kW * h + w];
+      for (int w = 0; w < kW; w++) hv[k2][w] = x[brev(k2, kLgPerT)].l[kW * h + w];
     });
     if constexpr (h > 0) __syncthreads();  // previous round's reads done before overwrite
-    static_for<0, 16>([&](auto k2_) {
+    static_for<0, kPerT>([&](auto k2_) {
       constexpr int k2 = decltype(k2_)::value;
       const int slot = ((k2 * A + rho1) << LGC) + (lc ^ (rho1 & (C - 1)));
       lds[slot] = pack_limbs<kW>(hv[k2]);
@@ -313,7 +313,7 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, const uint64_t 
   if constexpr (LA > 0) {
     static_for<0, F2>([&](auto f_) { dft_dif<LA, INV, decltype(f_)::value * A>(y); });
   }
-  // y[f*A + brev_LA(k1)] = Z[k2 + 16*k1], k2 = g*F2 + f
+  // y[f*A + brev_LA(k1)] = Z[k2 + kPerT*k1], k2 = g*F2 + f
 
   const uint64_t line2 = line0 + lc2;
   if (line2 >= a.nlines) return;
@@ -333,7 +333,7 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, const uint64_t 
       const int k2 = g * F2 + f;
       static_for<0, A>([&](auto k1_) {
         constexpr int k1 = decltype(k1_)::value;
-        const uint64_t k = (uint64_t)(k2 + 16 * k1);
+        const uint64_t k = (uint64_t)(k2 + kPerT * k1);
         fr v = y[f * A + brev(k1, LA)];
         if (a.do_scale) v = fr_mul29(v, a.scale29);
         fr_store(dst + ((lp + (k << lgL)) << lgI), fr_reduce_once(v));
@@ -351,7 +351,7 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, const uint64_t 
       const int k2 = g * F2 + f;
       static_for<0, A>([&](auto k1_) {
         constexpr int k1 = decltype(k1_)::value;
-        const uint64_t k = (uint64_t)(k2 + 16 * k1);
+        const uint64_t k = (uint64_t)(k2 + kPerT * k1);
         fr v = y[f * A + brev(k1, LA)];
         pin(v);
         v = fr_mul29(v, fr_load(tw + (k << a.lgS)));
@@ -392,7 +392,7 @@ __global__ void __launch_bounds__(kThreads, PL_NTT_WAVES)
 // Host launcher for one radix: LA = log2(column size) - 4.
 template <int LA, bool INV, bool LAST>
 static void launch_pass(const PassArgs& a, hipStream_t s) {
-  constexpr int m = LA + 4;
+  constexpr int m = LA + kLgPerT;
   uint64_t tiles = (a.nlines + (kTile >> m) - 1) >> (kLgTile - m);
   hipLaunchKernelGGL((ntt_pass_kernel<LA, INV, LAST>), dim3((unsigned)tiles), dim3(kThreads), 0, s, a);
 }

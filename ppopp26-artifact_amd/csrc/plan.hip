@@ -127,7 +127,7 @@ hipError_t phase_plan(PhasePlan& ph, const std::vector<Item>& items, Bump& bump,
     }
     for (int p = 0; p < d->plan.P; p++) {
       pass_args_regular(*d, c, p, a);
-      const int la = d->plan.m[p] - 4;
+      const int la = d->plan.m[p] - kLgPerT;
       const bool last = p == d->plan.P - 1;
       const auto key = std::make_tuple(p, la, c.inverse ? 1 : 0, last ? 1 : 0);
       auto f = gidx.find(key);

@@ -15,7 +15,7 @@
 #include <functional>
 #include <vector>
 
-#include "pl_internal.hpp"
+#include "ntt_pass.hpp"
 
 namespace pl {
 
@@ -28,9 +28,10 @@ static void compositions(int lg, int P, bool nonincreasing, std::vector<std::vec
       if (left == 0) out.push_back(cur);
       return;
     }
-    const int hi = (nonincreasing && !cur.empty()) ? std::min(8, cur.back()) : 8;
-    for (int m = 4; m <= hi; m++) {
-      if (left - m < 4 * (P - k - 1) || left - m > 8 * (P - k - 1)) continue;
+    const int MX = kLgPerT + (kLgPerT < 4 ? kLgPerT : 4), MN = kLgPerT;
+    const int hi = (nonincreasing && !cur.empty()) ? std::min(MX, cur.back()) : MX;
+    for (int m = MN; m <= hi; m++) {
+      if (left - m < MN * (P - k - 1) || left - m > MX * (P - k - 1)) continue;
       cur.push_back(m);
       rec(left - m, k + 1);
       cur.pop_back();
@@ -42,7 +43,8 @@ static void compositions(int lg, int P, bool nonincreasing, std::vector<std::vec
 std::vector<std::vector<int>> plan_candidates(int lg) {
   std::vector<std::vector<int>> c;
   if (lg < 4 || lg > 28) return c;
-  const int P0 = (lg + 7) / 8;
+  const int MX = kLgPerT + (kLgPerT < 4 ? kLgPerT : 4);
+  const int P0 = (lg + MX - 1) / MX;
   compositions(lg, P0, false, c);
   if (P0 + 1 <= PL_MAX_PASSES) compositions(lg, P0 + 1, true, c);
   return c;

@@ -44,10 +44,11 @@ def test_library_loads_and_host_functions():
     assert L.pl_version() >= 100
     assert L.pl_status_string(0) == b"ok"
     assert L.pl_status_string(-3) == b"unsupported size"
-    # plan: ceil(lg/8) passes of 4..8 points per column; tiny sizes one pass
-    assert [L.pl_ntt_num_passes(lg) for lg in (1, 3, 4, 8, 9, 16, 17, 24, 25, 28)] == [1, 1, 1, 1, 2, 2, 3, 3, 4, 4]
+    # plan: ceil(lg/6) passes of 3..6 points per column; tiny sizes one pass
+    assert [L.pl_ntt_num_passes(lg) for lg in (1, 3, 4, 6, 7, 12, 13, 18, 19, 24, 25, 28)] == \
+        [1, 1, 1, 1, 2, 2, 3, 3, 4, 4, 5, 5]
     assert L.pl_ntt_num_passes(29) == -3 and L.pl_ntt_num_passes(0) == -3
-    assert L.pl_ntt_work_bytes(8, 1) == 0          # single pass runs in place
+    assert L.pl_ntt_work_bytes(6, 1) == 0          # single pass runs in place
     assert L.pl_ntt_work_bytes(24, 1) == 32 << 24  # one intermediate buffer
     assert L.pl_ntt_work_bytes(12, 3) == 3 * (32 << 12)
     assert L.pl_lde_work_bytes(24, 2) == 32 << 26

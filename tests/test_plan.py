@@ -18,11 +18,13 @@ def pl():
 
 
 def test_default_plans(pl):
-    assert pl.plan_get(24) == [8, 8, 8]
-    assert pl.plan_get(20) == [7, 7, 6]
+    # ceil(lg/6) passes of 2^3..2^6-point columns, sizes within one, larger first
+    assert pl.plan_get(24) == [6, 6, 6, 6]
+    assert pl.plan_get(20) == [5, 5, 5, 5]
+    assert pl.plan_get(13) == [5, 4, 4]
     assert pl.plan_get(12) == [6, 6]
     assert pl.plan_get(9) == [5, 4]
-    assert pl.plan_get(8) == [8]
+    assert pl.plan_get(6) == [6]
     assert pl.plan_get(3) == []  # tiny kernel
 
 
@@ -32,33 +34,33 @@ def test_candidates_are_valid_splits(pl):
         assert cands, lg
         assert pl.plan_get(lg) in cands  # the default is among them
         for m in cands:
-            assert sum(m) == lg and all(4 <= x <= 8 for x in m), (lg, m)
+            assert sum(m) == lg and all(3 <= x <= 6 for x in m), (lg, m)
         assert len(set(map(tuple, cands))) == len(cands)
 
 
 def test_plan_set_and_errors(pl):
-    pl.plan_set(24, [6, 6, 6, 6])
-    assert pl.plan_get(24) == [6, 6, 6, 6]
-    assert pl.lib().pl_ntt_num_passes(24) == 4
+    pl.plan_set(24, [6, 6, 4, 4, 4])
+    assert pl.plan_get(24) == [6, 6, 4, 4, 4]
+    assert pl.lib().pl_ntt_num_passes(24) == 5
     pl.plan_set(24, [])
-    assert pl.plan_get(24) == [8, 8, 8]
+    assert pl.plan_get(24) == [6, 6, 6, 6]
     import ctypes
-    bad = (ctypes.c_int * 8)(8, 8, 7)
-    assert pl.lib().pl_plan_set(24, bad, 3) == -1          # wrong sum
-    bad = (ctypes.c_int * 8)(9, 8, 7)
-    assert pl.lib().pl_plan_set(24, bad, 3) == -1          # m > 8
-    bad = (ctypes.c_int * 8)(3, 3)
-    assert pl.lib().pl_plan_set(6, bad, 2) == -1           # m < 4
+    bad = (ctypes.c_int * 8)(6, 6, 6, 5)
+    assert pl.lib().pl_plan_set(24, bad, 4) == -1          # wrong sum
+    bad = (ctypes.c_int * 8)(8, 8, 8)
+    assert pl.lib().pl_plan_set(24, bad, 3) == -1          # m > 6
+    bad = (ctypes.c_int * 8)(2, 4)
+    assert pl.lib().pl_plan_set(6, bad, 2) == -1           # m < 3
 
 
 def test_plan_cache_file(pl, tmp_path):
     p = tmp_path / "plans.txt"
-    p.write_text("# comment\n20 3 8 6 6 0.1234\n13 2 7 6 0.01\n12 3 4 4 4 0.5\n14 2 9 5 1.0\nbad line\n")
-    assert pl.lib().pl_plan_load(str(p).encode()) == 3     # the 9-point split is rejected
-    assert pl.plan_get(20) == [8, 6, 6]
-    assert pl.plan_get(13) == [7, 6]
+    p.write_text("# comment\n20 4 6 6 4 4 0.1234\n13 3 5 5 3 0.01\n12 3 4 4 4 0.5\n14 2 8 6 1.0\nbad line\n")
+    assert pl.lib().pl_plan_load(str(p).encode()) == 3     # the 8-point split is rejected
+    assert pl.plan_get(20) == [6, 6, 4, 4]
+    assert pl.plan_get(13) == [5, 5, 3]
     assert pl.plan_get(12) == [4, 4, 4]
-    assert pl.plan_get(14) == [7, 7]
+    assert pl.plan_get(14) == [5, 5, 4]
     assert pl.lib().pl_plan_load(str(tmp_path / "missing.txt").encode()) == -1
 
 
