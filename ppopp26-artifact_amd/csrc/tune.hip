@@ -1,8 +1,9 @@
 // Pass-split autotuner for the NTT (C-ABI pl_plan_*, pl_autotune).
 //
-// A size-2^lg transform runs as P passes of 2^m[p]-point columns, 4 <= m <= 8,
-// sum m = lg.  Every split computes the same exact transform; they differ in
-// twiddle-table traffic, tile shape per pass and launch count.  The tuner
+// A size-2^lg transform runs as P passes of 2^m[p]-point columns, sum m = lg,
+// kLgPerT <= m <= kLgPerT + min(kLgPerT, 4) (3..6 with 8 elements per lane).
+// Every split computes the same exact transform; they differ in twiddle-table
+// traffic, tile shape per pass and launch count.  The tuner
 // times the candidate splits of each size on the device (a batch of
 // max(1, 2^22 / N) polynomials per call, so small sizes are measured at the
 // throughput the batched form sees rather than at launch latency), keeps
@@ -19,7 +20,7 @@
 
 namespace pl {
 
-// compositions of lg into P parts in [4, 8]; non-increasing only when
+// compositions of lg into P parts in [MN, MX]; non-increasing only when
 // nonincreasing is set
 static void compositions(int lg, int P, bool nonincreasing, std::vector<std::vector<int>>& out) {
   std::vector<int> cur;
