@@ -68,6 +68,16 @@ __global__ void pow_window_kernel(fr* out, uint64_t n, const fr* w0, const fr* w
   fr_store(out + i, v);
 }
 
+// pass-kernel tables: w R' mod r -> pbd words (balanced digits), in place
+__global__ void pbd_kernel(fr* t, uint64_t n) {
+  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  const fr v = fr_load(t + i);
+  fr o;
+  to_pbd(v.l, o.l);
+  fr_store(t + i, o);
+}
+
 static inline fr to_dev(const host::fr64& h) {
   fr f;
   memcpy(f.l, h.l, 32);
@@ -207,6 +217,12 @@ hipError_t domain_build(Domain& d, int lg, hipStream_t s) {
     // the inverse transform's 2^-lg rides on the first pass's twiddles
     if ((e = fill_pow(d.tw_inv[p], n, P.lgS[p], wi, p == 0 ? &n_inv32 : &c32, win, s)) != hipSuccess) goto done;
   }
+  {
+    // the pass kernels' tables (inner29_*, tw_*) are contiguous to the end
+    const uint64_t n29 = (uint64_t)(cur - d.inner29_fwd);
+    hipLaunchKernelGGL(pbd_kernel, dim3(grid_for(n29, 256)), dim3(256), 0, s, d.inner29_fwd, n29);
+    if ((e = hipGetLastError()) != hipSuccess) goto done;
+  }
   e = hipStreamSynchronize(s);
 done:
   (void)hipFree(win);
@@ -291,7 +307,11 @@ static void fill_pass_args(const Domain& d, const NttCall& c, int p, const fr* i
   a.lgI = c.lgI;
   a.pm_mask = c.pm_mask;
   a.scale = to_dev(d.n_inv);
-  a.scale29 = to_dev(host::mul(d.n_inv, host::from_u64(32)));
+  {
+    const fr sc = to_dev(host::mul(d.n_inv, host::from_u64(32)));
+    to_pbd(sc.l, a.scale29.l);
+  }
+  a.in_ext = (P.P == 0 || p == 0) ? 1 : 0;
   a.in = in;
   a.out = out;
   a.in_pstride = in_pstride;

@@ -25,37 +25,40 @@
 // rho1 (8/A per lane, in registers).  The tile holds C = 1024/M' adjacent
 // columns, so every global access moves C*32 >= 512 contiguous bytes.  The
 // in-register DFTs are radix-2 DIF networks whose twiddles (powers of w_16)
-// are compile-time constants read from SGPRs.
+// are compile-time constants read from SGPRs.  Between passes the data is
+// held as pbd words (fr29.hpp), not reduced; the last pass writes canonical
+// values in [0, r).
 #pragma once
 #include "fr29.hpp"
 #include "pl_internal.hpp"
 
 namespace pl {
 
-// w_16^e and w_16^-e, e = 0..7, as w R' mod r (R' = 2^261) in nine 29-bit
-// limbs: the constant operand of the radix-2^29 product (fr29.hpp)
-static __constant__ const uint32_t kW16r[2][8][9] = {
-    {{0x1fffffbau, 0x0000022fu, 0x1cb61180u, 0x0a4e5c00u, 0x0ee8b1a2u, 0x16e6aedfu, 0x1907f8bbu, 0x0853ddf7u, 0x004d043fu},
-     {0x04056c96u, 0x0f64e4d8u, 0x1c1c7b56u, 0x1cbda467u, 0x14e761dau, 0x128ae056u, 0x1d2f3144u, 0x0f463b6du, 0x006dc1feu},
-     {0x085dfee9u, 0x0f4995e4u, 0x1941c136u, 0x0b284957u, 0x1c313523u, 0x1f28f6a5u, 0x07ad921du, 0x0d72d344u, 0x002ca644u},
-     {0x181392abu, 0x140c77edu, 0x0175e03du, 0x09f5a4e3u, 0x0f090202u, 0x0c6252cbu, 0x1c504ed8u, 0x038e921bu, 0x00408014u},
-     {0x1139f60bu, 0x10567552u, 0x1e6cfcddu, 0x03679a01u, 0x08c51075u, 0x1f9df144u, 0x1eb67e0bu, 0x1a6828a4u, 0x00231dc6u},
-     {0x0142510bu, 0x02a002b3u, 0x09a9466eu, 0x1f776142u, 0x002a304fu, 0x10591c89u, 0x04f2cdf3u, 0x0d6e4f9eu, 0x006990bbu},
-     {0x16551b88u, 0x1f9842feu, 0x0f9a0688u, 0x1b208bceu, 0x1e704151u, 0x09d87033u, 0x1d460dfbu, 0x063990abu, 0x0050578fu},
-     {0x17ddaa4du, 0x1d2e4774u, 0x06b5ad2eu, 0x19075af7u, 0x06ef6bbdu, 0x125fb8efu, 0x07ff9cadu, 0x15571281u, 0x00687c13u}},
-    {{0x1fffffbau, 0x0000022fu, 0x1cb61180u, 0x0a4e5c00u, 0x0ee8b1a2u, 0x16e6aedfu, 0x1907f8bbu, 0x0853ddf7u, 0x004d043fu},
-     {0x082255b4u, 0x02d1b883u, 0x18e15291u, 0x0240ab08u, 0x1690e97eu, 0x19a44be1u, 0x0d213039u, 0x150e212eu, 0x000b7193u},
-     {0x09aae479u, 0x0067bcf9u, 0x0ffcf937u, 0x00277a31u, 0x1f1013eau, 0x022b949cu, 0x17dabeecu, 0x042ba303u, 0x00239618u},
-     {0x1ebdaef6u, 0x1d5ffd44u, 0x15edb951u, 0x1bd0a4bdu, 0x1d5624ebu, 0x1baae847u, 0x102dfef3u, 0x1cf6e411u, 0x000a5cebu},
-     {0x0ec609f6u, 0x0fa98aa5u, 0x012a02e2u, 0x17e06bfeu, 0x14bb44c6u, 0x0c66138cu, 0x166a4edbu, 0x0ffd0b0au, 0x0050cfe0u},
-     {0x07ec6d56u, 0x0bf3880au, 0x1e211f82u, 0x1152611cu, 0x0e775339u, 0x1fa1b205u, 0x18d07e0eu, 0x06d6a193u, 0x00336d93u},
-     {0x17a20118u, 0x10b66a13u, 0x06553e89u, 0x101fbca8u, 0x014f2018u, 0x0cdb0e2bu, 0x0d733ac9u, 0x1cf2606bu, 0x00474762u},
-     {0x1bfa936bu, 0x109b1b1fu, 0x037a8469u, 0x1e8a6198u, 0x0898f360u, 0x1979247au, 0x17f19ba2u, 0x1b1ef841u, 0x00062ba8u}}};
+// w_16^e and w_16^-e, e = 0..7, as w R' mod r (R' = 2^261) in balanced
+// digits (b_i in [-2^28, 2^28), sum b_i 2^(29 i)): the wave-uniform factor of
+// the radix-2^29 product (fr29.hpp)
+static __constant__ const uint32_t kW16b[2][8][9] = {
+    {{0xffffffbau, 0x00000230u, 0xfcb61180u, 0x0a4e5c01u, 0x0ee8b1a2u, 0xf6e6aedfu, 0xf907f8bcu, 0x0853ddf8u, 0x004d043fu},
+     {0x04056c96u, 0x0f64e4d8u, 0xfc1c7b56u, 0xfcbda468u, 0xf4e761dbu, 0xf28ae057u, 0xfd2f3145u, 0x0f463b6eu, 0x006dc1feu},
+     {0x085dfee9u, 0x0f4995e4u, 0xf941c136u, 0x0b284958u, 0xfc313523u, 0xff28f6a6u, 0x07ad921eu, 0x0d72d344u, 0x002ca644u},
+     {0xf81392abu, 0xf40c77eeu, 0x0175e03eu, 0x09f5a4e3u, 0x0f090202u, 0x0c6252cbu, 0xfc504ed8u, 0x038e921cu, 0x00408014u},
+     {0xf139f60bu, 0xf0567553u, 0xfe6cfcdeu, 0x03679a02u, 0x08c51075u, 0xff9df144u, 0xfeb67e0cu, 0xfa6828a5u, 0x00231dc7u},
+     {0x0142510bu, 0x02a002b3u, 0x09a9466eu, 0xff776142u, 0x002a3050u, 0xf0591c89u, 0x04f2cdf4u, 0x0d6e4f9eu, 0x006990bbu},
+     {0xf6551b88u, 0xff9842ffu, 0x0f9a0689u, 0xfb208bceu, 0xfe704152u, 0x09d87034u, 0xfd460dfbu, 0x063990acu, 0x0050578fu},
+     {0xf7ddaa4du, 0xfd2e4775u, 0x06b5ad2fu, 0xf9075af7u, 0x06ef6bbeu, 0xf25fb8efu, 0x07ff9caeu, 0xf5571281u, 0x00687c14u}},
+    {{0xffffffbau, 0x00000230u, 0xfcb61180u, 0x0a4e5c01u, 0x0ee8b1a2u, 0xf6e6aedfu, 0xf907f8bcu, 0x0853ddf8u, 0x004d043fu},
+     {0x082255b4u, 0x02d1b883u, 0xf8e15291u, 0x0240ab09u, 0xf690e97eu, 0xf9a44be2u, 0x0d21303au, 0xf50e212eu, 0x000b7194u},
+     {0x09aae479u, 0x0067bcf9u, 0x0ffcf937u, 0x00277a31u, 0xff1013eau, 0x022b949du, 0xf7dabeecu, 0x042ba304u, 0x00239618u},
+     {0xfebdaef6u, 0xfd5ffd45u, 0xf5edb952u, 0xfbd0a4beu, 0xfd5624ecu, 0xfbaae848u, 0xf02dfef4u, 0xfcf6e412u, 0x000a5cecu},
+     {0x0ec609f6u, 0x0fa98aa5u, 0x012a02e2u, 0xf7e06bfeu, 0xf4bb44c7u, 0x0c66138du, 0xf66a4edbu, 0x0ffd0b0bu, 0x0050cfe0u},
+     {0x07ec6d56u, 0x0bf3880au, 0xfe211f82u, 0xf152611du, 0x0e77533au, 0xffa1b205u, 0xf8d07e0fu, 0x06d6a194u, 0x00336d93u},
+     {0xf7a20118u, 0xf0b66a14u, 0x06553e8au, 0xf01fbca8u, 0x014f2019u, 0x0cdb0e2bu, 0x0d733ac9u, 0xfcf2606bu, 0x00474763u},
+     {0xfbfa936bu, 0xf09b1b20u, 0x037a846au, 0xfe8a6198u, 0x0898f361u, 0xf979247au, 0xf7f19ba3u, 0xfb1ef842u, 0x00062ba9u}}};
 
 template <bool INV>
-__device__ __forceinline__ void w16r(int e, uint32_t (&w)[9]) {
+__device__ __forceinline__ void w16b(int e, uint32_t (&w)[9]) {
 #pragma unroll
-  for (int i = 0; i < 9; i++) w[i] = kW16r[INV ? 1 : 0][e][i];
+  for (int i = 0; i < 9; i++) w[i] = kW16b[INV ? 1 : 0][e][i];
 }
 
 // an ordering point for the eight limbs of v (no instruction)
@@ -71,11 +74,15 @@ __host__ __device__ constexpr int brev(int v, int bits) {
 }
 
 // Radix-2 DIF DFT of size 2^LOGB on x[OFF + i], natural order in,
-// bit-reversed order out: afterwards x[OFF + brev(k)] = X[k].  Values are kept
-// lazily in [0, 2r) (fr_add_lazy / fr_sub_lazy; the radix-2^29 product returns
-// below 1.03 r); the pass reduces them once before storing.
+// bit-reversed order out: afterwards x[OFF + brev(k)] = X[k].  Elements are
+// signed-limb values (v9): sums and differences are limb-wise, and with every
+// input limb a balanced digit (|l| <= 2^28) each output limb is a signed sum
+// of at most 2^LOGB balanced digits or product digits (below 2^29), which
+// stays within 32 bits for LOGB <= 3 (the first term of every such sum enters
+// with a plus sign, so the extremes are 2^31 - 1 and -2^31).
 template <int LOGB, bool INV, int OFF, int NX>
-__device__ __forceinline__ void dft_dif(fr (&x)[NX]) {
+__device__ __forceinline__ void dft_dif(v9 (&x)[NX]) {
+  static_assert(LOGB <= 3, "limb growth: at most three butterfly levels");
   constexpr int B = 1 << LOGB;
   static_for<0, LOGB>([&](auto st_) {
     constexpr int st = decltype(st_)::value;
@@ -87,19 +94,19 @@ __device__ __forceinline__ void dft_dif(fr (&x)[NX]) {
         // one butterfly at a time: the empty asm statements order the
         // butterflies, so the live set stays at the lane's elements plus one
         // butterfly's temporaries
-        fr a = x[OFF + s + i];
-        fr b = x[OFF + s + i + h];
-        pin(a);
-        pin(b);
-        fr sum = fr_add_lazy(a, b);
-        fr d = fr_sub_lazy(a, b);
+        v9 a = x[OFF + s + i];
+        v9 b = x[OFF + s + i + h];
+        pin9(a);
+        pin9(b);
+        v9 sum = v9_add(a, b);
+        v9 d = v9_sub(a, b);
         if constexpr (i != 0) {
           uint32_t w[9];
-          w16r<INV>((i * 8) / h, w);
-          d = fr_mul29<true>(d, w);
+          w16b<INV>((i * 8) / h, w);
+          d = v9_mul<true, false, false>(d, w);
         }
-        pin(sum);
-        pin(d);
+        pin9(sum);
+        pin9(d);
         x[OFF + s + i] = sum;
         x[OFF + s + i + h] = d;
       });
@@ -111,9 +118,9 @@ struct PassArgs {
   const fr* in;
   fr* out;
   const fr* premul;     // !LAST: optional factor per input element [(poly & pm_mask) << lgN | pos]
-  const fr* tw;         // !LAST: pass twiddles (w R' mod r), index j + S*k within the sub-problem
+  const fr* tw;         // !LAST: pass twiddles (pbd words of w R' mod r), index j + S*k within the sub-problem
   const fr* inner;      // w_256^e (or w_256^-e), e < 256, stored form (small-size kernel)
-  const fr* inner29;    // the same as w R' mod r (pass kernels)
+  const fr* inner29;    // the same as pbd words of w R' mod r (pass kernels)
   uint64_t in_pstride;  // element stride between input polys (0: shared input)
   uint64_t nlines;      // lines (columns) in this launch
   uint32_t pm_mask;
@@ -122,16 +129,17 @@ struct PassArgs {
   uint32_t lgS;         // log2 column stride (0 in the last pass)
   uint32_t lgI;         // LAST: output interleave
   uint32_t ndig;        // LAST: number of earlier passes
-  uint32_t do_scale;    // multiply every output by `scale`
+  uint32_t do_scale;    // multiply every output by `scale` (single-pass transforms)
+  uint32_t in_ext;      // the input is caller data (any 256-bit values), not a previous pass's pbd words
   uint32_t lgrad[PL_MAX_PASSES];  // LAST: log2 radices of the earlier passes, pass 1 first
   fr scale;             // stored form (small-size kernel)
-  fr scale29;           // scale R' mod r (pass kernels)
+  fr scale29;           // pbd word of scale R' mod r (pass kernels)
 };
 
 // Tile geometry (compile-time; see the file header):
 //   PL_NTT_LGPER   log2 elements per lane (3: 8 elements, DFT_8 in registers)
 //   PL_NTT_LGTILE  log2 elements per workgroup tile (128 lanes x 8)
-//   PL_NTT_PHASES  LDS exchange rounds (8/PHASES limbs each)
+//   PL_NTT_PHASES  LDS exchange rounds (9/PHASES limbs each)
 //   PL_NTT_WAVES   waves per SIMD the register allocation must allow
 // Passes have 2^m-point columns, m = PL_NTT_LGPER + LA, LA <= min(4, LGPER).
 #ifndef PL_NTT_LGTILE
@@ -144,7 +152,7 @@ constexpr int kLgPerT = PL_NTT_LGPER;
 constexpr int kPerT = 1 << kLgPerT;
 constexpr int kPmG = kPerT / 4;  // factor-table loads in flight per group (first pass of an LDE)
 #ifndef PL_NTT_PHASES
-#define PL_NTT_PHASES 2
+#define PL_NTT_PHASES 3
 #endif
 #ifndef PL_NTT_WAVES
 #define PL_NTT_WAVES 4
@@ -153,27 +161,43 @@ constexpr int kLgTile = PL_NTT_LGTILE;
 constexpr int kTile = 1 << kLgTile;
 constexpr int kThreads = kTile / kPerT;
 constexpr int kPhases = PL_NTT_PHASES;
-constexpr int kW = 8 / kPhases;  // limbs per exchange round
+constexpr int kW = 9 / kPhases;  // limbs per exchange round
+static_assert(kW * kPhases == 9, "the exchange rounds cover the nine limbs");
 
 template <int W> struct lds_vec;
 template <> struct lds_vec<4> { typedef uint4 type; };
+template <> struct lds_vec<3> { typedef uint3 type; };
 template <> struct lds_vec<2> { typedef uint2 type; };
 template <> struct lds_vec<1> { typedef uint32_t type; };
 
 template <int W>
 __device__ __forceinline__ typename lds_vec<W>::type pack_limbs(const uint32_t* v) {
   if constexpr (W == 4) return make_uint4(v[0], v[1], v[2], v[3]);
+  else if constexpr (W == 3) return make_uint3(v[0], v[1], v[2]);
   else if constexpr (W == 2) return make_uint2(v[0], v[1]);
   else return v[0];
 }
 template <int W>
 __device__ __forceinline__ void unpack_limbs(uint32_t* v, const typename lds_vec<W>::type& q) {
   if constexpr (W == 4) { v[0] = q.x; v[1] = q.y; v[2] = q.z; v[3] = q.w; }
+  else if constexpr (W == 3) { v[0] = q.x; v[1] = q.y; v[2] = q.z; }
   else if constexpr (W == 2) { v[0] = q.x; v[1] = q.y; }
   else v[0] = q;
 }
 
-// One pass over columns of M' = 16*A points, A = 2^LA; blk = tile index.
+// One pass over columns of M' = 8*A points, A = 2^LA; blk = tile index.
+//
+// Arithmetic: the lane's elements are signed-limb values (v9).  The input is
+// unpacked to balanced digits (caller data: any 256-bit value; later passes:
+// the previous pass's pbd words), so each in-register DFT starts from
+// balanced digits and its limbs stay within 32 bits; every product takes
+// such limbs.  Value bounds (V = input magnitude, at most 2^256 < 2.21 r): a
+// DFT_8 output is below 8 V = 17.7 r; an inner-twiddle product of it lies in
+// (-1.25 r, 0.25 r) (x w / 2^261 < 0.0142 x for w < r); element k2 = 0, which
+// has no inner twiddle, is reduced into (-0.51 r, 0.51 r); so a DFT_A output
+// is below 10 r, and the pass twiddle product with centred digits
+// (mont29 BM) lies in (-0.65 r, 0.65 r): its balanced digits fit a pbd word
+// (top digit below 2^23).  The last pass reduces its outputs to [0, r).
 template <int LA, bool INV, bool LAST>
 __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, const uint64_t blk) {
   constexpr int A = 1 << LA;
@@ -201,7 +225,7 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, const uint64_t 
     rho1 = t >> LGC;
   }
   const uint64_t line = line0 + lc;
-  fr x[kPerT];
+  v9 x[kPerT];
   if (line < a.nlines) {
     uint64_t poly, pos0, estride;
     if (LAST) {
@@ -232,45 +256,74 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, const uint64_t 
       estride = 1ull << a.lgS;
     }
     const fr* src = a.in + poly * a.in_pstride;
-    static_for<0, kPerT>([&](auto r2_) {
-      constexpr int r2 = decltype(r2_)::value;
-      x[r2] = fr_load(src + pos0 + (uint64_t)(rho1 + A * r2) * estride);
-    });
-    if (!LAST && a.premul) {
-      const fr* pm = a.premul + ((uint64_t)(poly & a.pm_mask) << a.lgN) + pos0;
-      // kPmG factors at a time: each group's loads start after the previous
-      // group's products (the asm statements order them), so at most kPmG
-      // factors are held in registers beside the lane's elements
-      static_for<0, kPerT / kPmG>([&](auto g_) {
-        constexpr int g = decltype(g_)::value;
-        const fr* q = pm;
-        asm volatile("" : "+v"(q));
-        fr f[kPmG];
-        static_for<0, kPmG>([&](auto r_) {
-          constexpr int r = decltype(r_)::value;
-          f[r] = fr_load(q + (uint64_t)(rho1 + A * (kPmG * g + r)) * estride);
+    // balanced digits: caller data is any 256-bit value (optionally times a
+    // premultiplication factor), a previous pass's output is a pbd word.  Each
+    // branch loads and unpacks on its own, so no loaded word lives across it.
+    if (a.in_ext) {
+      fr raw[kPerT];
+      static_for<0, kPerT>([&](auto r2_) {
+        constexpr int r2 = decltype(r2_)::value;
+        raw[r2] = fr_load(src + pos0 + (uint64_t)(rho1 + A * r2) * estride);
+      });
+      if (!LAST && a.premul) {
+        const fr* pm = a.premul + ((uint64_t)(poly & a.pm_mask) << a.lgN) + pos0;
+        // kPmG factors at a time: each group's loads start after the previous
+        // group's products (the asm statements order them), so at most kPmG
+        // factors are held in registers beside the lane's elements
+        static_for<0, kPerT / kPmG>([&](auto g_) {
+          constexpr int g = decltype(g_)::value;
+          const fr* q = pm;
+          asm volatile("" : "+v"(q));
+          fr f[kPmG];
+          static_for<0, kPmG>([&](auto r_) {
+            constexpr int r = decltype(r_)::value;
+            f[r] = fr_load(q + (uint64_t)(rho1 + A * (kPmG * g + r)) * estride);
+          });
+          static_for<0, kPmG>([&](auto r_) {
+            constexpr int r = decltype(r_)::value;
+            raw[kPmG * g + r] = fr_mul(raw[kPmG * g + r], f[r]);
+            pin(raw[kPmG * g + r]);
+          });
         });
-        static_for<0, kPmG>([&](auto r_) {
-          constexpr int r = decltype(r_)::value;
-          x[kPmG * g + r] = fr_mul(x[kPmG * g + r], f[r]);
-          pin(x[kPmG * g + r]);
-        });
+      }
+      static_for<0, kPerT>([&](auto r2_) {
+        constexpr int r2 = decltype(r2_)::value;
+        unpack_ext(raw[r2].l, x[r2].l);
+      });
+    } else {
+      static_for<0, kPerT>([&](auto r2_) {
+        constexpr int r2 = decltype(r2_)::value;
+        const fr raw = fr_load(src + pos0 + (uint64_t)(rho1 + A * r2) * estride);
+        pbd_unpack(raw.l, x[r2].l);
       });
     }
   } else {
-    static_for<0, kPerT>([&](auto r2_) { x[decltype(r2_)::value] = fr_zero(); });
+    static_for<0, kPerT>([&](auto r2_) {
+#pragma unroll
+      for (int i = 0; i < 9; i++) x[decltype(r2_)::value].l[i] = 0;
+    });
   }
-  dft_dif<kLgPerT, INV, 0>(x);  // x[brev4(k2)] = sum_rho2 z[rho1 + A rho2] w16^(rho2 k2)
+  dft_dif<kLgPerT, INV, 0>(x);  // x[brev3(k2)] = sum_rho2 z[rho1 + A rho2] w8^(rho2 k2)
   if constexpr (LA > 0) {
+    // k2 = 0 takes no twiddle: back to balanced digits of a small value, so
+    // that DFT_A again starts from balanced digits (the products' outputs
+    // below are balanced, mont29 BT)
+    {
+      v9 t0;
+      reduce_centered(x[0].l, t0.l);
+      x[0] = t0;
+    }
     static_for<1, kPerT>([&](auto k2_) {
       constexpr int k2 = decltype(k2_)::value;
       const int e = rho1 * k2;  // < M'
-      pin(x[brev(k2, kLgPerT)]);
-      x[brev(k2, kLgPerT)] = fr_mul29(x[brev(k2, kLgPerT)], fr_load(a.inner29 + (e << (8 - m))));
+      pin9(x[brev(k2, kLgPerT)]);
+      uint32_t w[9];
+      pbd_unpack(fr_load(a.inner29 + (e << (8 - m))).l, w);
+      x[brev(k2, kLgPerT)] = v9_mul<false, false, true>(x[brev(k2, kLgPerT)], w);
     });
   }
-  // Exchange through LDS in kPhases rounds of kW limbs (2 rounds of 16 bytes
-  // = 16 KiB for a 1024-element tile, so eight workgroups = four waves per
+  // Exchange through LDS in kPhases rounds of kW limbs (3 rounds of 12 bytes
+  // = 12 KiB for a 1024-element tile, so eight workgroups = four waves per
   // SIMD fit a CU).  LDS slot of W[line][rho1][k2]; consecutive lanes hit
   // consecutive slots.  Each round's gathered limbs go back into the
   // registers that round vacated: y lives in x (x[f*A + r1] afterwards).
@@ -283,7 +336,7 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, const uint64_t 
   asm volatile("" : "+v"(t2));
   const int lc2 = t2 & (C - 1);
   const int g = t2 >> LGC;
-  fr (&y)[kPerT] = x;
+  v9 (&y)[kPerT] = x;
   static_for<0, kPhases>([&](auto h_) {
     constexpr int h = decltype(h_)::value;
     uint32_t hv[kPerT][kW];
@@ -328,15 +381,29 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, const uint64_t 
     }
     const uint32_t lgI = a.lgI;
     fr* dst = a.out + ((poly >> lgI) << (a.lgN + lgI)) + (poly & ((1ull << lgI) - 1));
+    fr* const base = dst + (lp << lgI);
     static_for<0, F2>([&](auto f_) {
       constexpr int f = decltype(f_)::value;
       const int k2 = g * F2 + f;
       static_for<0, A>([&](auto k1_) {
         constexpr int k1 = decltype(k1_)::value;
         const uint64_t k = (uint64_t)(k2 + kPerT * k1);
-        fr v = y[f * A + brev(k1, LA)];
-        if (a.do_scale) v = fr_mul29(v, a.scale29);
-        fr_store(dst + ((lp + (k << lgL)) << lgI), fr_reduce_once(v));
+        v9 v = y[f * A + brev(k1, LA)];
+        pin9(v);
+        if (a.do_scale) {
+          uint32_t w[9];
+          pbd_unpack(a.scale29.l, w);
+          v = v9_mul<false, false, false>(v, w);
+        }
+        uint32_t tt[9];
+        reduce_below_2r(v.l, tt);
+        fr o;
+        join29(tt, o.l);
+        // the address from an opaque copy of the lane's base, so that the
+        // eight addresses are not all computed ahead and held
+        fr* d = base;
+        asm volatile("" : "+v"(d));
+        fr_store(d + (k << (lgL + lgI)), fr_reduce_once(o));
       });
     });
   } else {
@@ -352,11 +419,14 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, const uint64_t 
       static_for<0, A>([&](auto k1_) {
         constexpr int k1 = decltype(k1_)::value;
         const uint64_t k = (uint64_t)(k2 + kPerT * k1);
-        fr v = y[f * A + brev(k1, LA)];
-        pin(v);
-        v = fr_mul29(v, fr_load(tw + (k << a.lgS)));
-        if (a.do_scale) v = fr_mul29(v, a.scale29);
-        fr_store(dst + (k << a.lgS), fr_reduce_once(v));
+        v9 v = y[f * A + brev(k1, LA)];
+        pin9(v);
+        uint32_t w[9];
+        pbd_unpack(fr_load(tw + (k << a.lgS)).l, w);
+        v = v9_mul<false, true, true>(v, w);
+        fr o;
+        pbd_pack(v.l, o.l);
+        fr_store(dst + (k << a.lgS), o);
       });
     });
   }
@@ -389,7 +459,7 @@ __global__ void __launch_bounds__(kThreads, PL_NTT_WAVES)
   ntt_pass_body<LA, INV, LAST>(a, b - seg_block0[lo]);
 }
 
-// Host launcher for one radix: LA = log2(column size) - 4.
+// Host launcher for one radix: LA = log2(column size) - kLgPerT.
 template <int LA, bool INV, bool LAST>
 static void launch_pass(const PassArgs& a, hipStream_t s) {
   constexpr int m = LA + kLgPerT;

@@ -4,11 +4,13 @@ its constants, on the host.
 * fr29.hpp's plain-C product (signed digit, then + r), built with g++,
   against Python integers: x w 2^-261 mod r, and in (0, 1.031 r) for any
   256-bit x and w < r (the bound the lazy [0, 2r) butterflies rely on);
-* the DFT-16 constants in ntt_pass.hpp against the reference's own
-  16th roots of unity (tests/golden/ref_constants.json), times 2^261;
+* the DFT constants in ntt_pass.hpp (balanced digits) against the
+  reference's own 16th roots of unity (tests/golden/ref_constants.json),
+  times 2^261;
 * r and 2r as written in fr29.hpp;
-* mont29_columns.inc is what its generator writes, and every column block
-  adds each product term exactly once.
+* mont29_columns.inc is what its generator writes, every column block adds
+  each product term exactly once, and the balanced-digit blocks close the
+  previous column as (acc + 2^28) >> 29.
 
 The device asm blocks are exercised bit-for-bit by every GPU NTT/LDE test.
 """
@@ -82,17 +84,22 @@ def _table(text, name, rows, cols):
     return [[nums[(d * rows + e) * cols:(d * rows + e + 1) * cols] for e in range(rows)] for d in range(2)]
 
 
+def _s32(v):
+    return v - (1 << 32) if v >> 31 else v
+
+
 def test_dft16_constants_are_reference_roots_times_2_261():
     ref = json.load(open(os.path.join(HERE, "golden", "ref_constants.json")))
     rinv = pow(1 << 256, -1, R)
     w = int(ref["forward_roots_of_unity"][4], 16) * rinv % R
     wi = int(ref["inverse_roots_of_unity"][4], 16) * rinv % R
     assert pow(w, 16, R) == 1 and pow(w, 8, R) == R - 1 and w * wi % R == 1
-    tab = _table(open(os.path.join(CSRC, "ntt_pass.hpp")).read(), "kW16r[2][8][9]", 8, 9)
+    tab = _table(open(os.path.join(CSRC, "ntt_pass.hpp")).read(), "kW16b[2][8][9]", 8, 9)
     for d, base in ((0, w), (1, wi)):
         for e in range(8):
-            assert all(v <= M29 for v in tab[d][e])
-            v = sum(l << (29 * i) for i, l in enumerate(tab[d][e]))
+            digits = [_s32(v) for v in tab[d][e]]
+            assert all(-(1 << 28) <= v <= (1 << 28) for v in digits)
+            v = sum(l << (29 * i) for i, l in enumerate(digits))
             assert v == pow(base, e, R) * (1 << 261) % R, (d, e)
 
 
@@ -111,13 +118,20 @@ def test_generated_columns_current_and_complete():
     inc = os.path.join(CSRC, "mont29_columns.inc")
     out = subprocess.check_output(["python3", gen]).decode()
     assert out == open(inc).read()
-    # in each of the 36 blocks every product term of its column appears once
-    spec = re.findall(r"struct Mont29Col<(\d+), (true|false)> \{.*?asm\(\"([^\"]*)\"\s*: ([^\n]*)\s*: ([^;]*)\);",
-                      out, re.S)
-    assert len(spec) == 36
-    for k, ub, text, _outs, ins in spec:
+    # in each of the 70 blocks every product term of its column appears once
+    spec = re.findall(r"struct Mont29Col<(\d+), (true|false), (true|false)> \{.*?asm\(\"([^\"]*)\"\s*: ([^\n]*)"
+                      r"\s*: ([^;]*)\);", out, re.S)
+    assert len(spec) == 70
+    for k, ub, rnd, text, _outs, ins in spec:
         k = int(k)
         ops = ["acc", "d"] + [o.strip() for o in re.findall(r'"[vs]"\(([^)]*\)?)\)', ins)]
+        first = text.split("\\n\\t")[:2]
+        if rnd == "true":
+            # (prev + 2^28) >> 29: the carry of a balanced digit
+            assert first[0] == "v_mad_i64_i32 %0, %1, 1, %3, %2" and ops[3] == "1u << 28", k
+            assert first[1] == "v_ashrrev_i64 %0, 29, %0", k
+        elif k >= 1:
+            assert first[0] == "v_ashrrev_i64 %0, 29, %2", k
         pairs = []
         for t in text.split("\\n\\t"):
             m = re.match(r"v_mad_i64_i32 %0, %1, %(\d+), %(\d+), (%0|0)", t)
