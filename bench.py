@@ -21,6 +21,8 @@ launch stream.  Beside it, in the same line:
            three rates; outputs compared by digest between the two forms
   roofline the dominant kernel (NTT pass 0), per-launch HIP events on its
            stream, against the measured v_mad_u64_u32 peak
+Each timed section first runs its own step, untimed, for at least 300 ms of
+device time, so that the clocks have settled under that load.
   cpu_baseline  the oracle's radix-2 NTT 2^24 on one host core
 """
 import argparse
@@ -81,6 +83,15 @@ def time_events(fn, reps, stream):
     return e0.elapsed_time(e1) / reps  # ms
 
 
+def settle(fn, stream, min_ms=300.0, chunk=4):
+    """Untimed: run fn until at least min_ms of device time has passed.  The
+    clocks drop for some tens of milliseconds when a heavy VALU load starts
+    and then recover; timing starts once they have settled under this load."""
+    spent = 0.0
+    while spent < min_ms:
+        spent += time_events(fn, chunk, stream) * chunk
+
+
 def time_each(fn, reps, stream):
     """per-run milliseconds (event pair around each run)"""
     import torch
@@ -128,6 +139,7 @@ def bench_ntt(pl, synth, lg, steps, warmup, stream, peak):
     step = lambda: pl.ntt(x, dom, out=y, work=work)  # noqa: E731
     for _ in range(warmup):
         step()
+    settle(step, stream)
     torch.cuda.synchronize()
     ms = time_events(step, steps, stream)
     npass = pl.ntt_num_passes(lg)
@@ -179,6 +191,7 @@ def bench_lde(pl, synth, lg, lb, steps, warmup, stream, peak):
     step = lambda: pl.lde(c, dom, out=out, work=work)  # noqa: E731
     for _ in range(warmup):
         step()
+    settle(step, stream)
     torch.cuda.synchronize()
     ms = time_events(step, steps, stream)
     mulmod = (m // 2) * lg + m
@@ -221,8 +234,11 @@ def bench_batch(pl, synth, steps, stream):
     for _ in range(3):
         b.run()
         b.run_plain()
+    settle(b.run, stream, chunk=2)
     torch.cuda.synchronize()
     t_b = time_each(b.run, steps, stream)
+    settle(b.run_plain, stream, chunk=2)
+    torch.cuda.synchronize()
     t_p = time_each(b.run_plain, steps, stream)
     mb, mp = pct(t_b, 50), pct(t_p, 50)
     b.close()
