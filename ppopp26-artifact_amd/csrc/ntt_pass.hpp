@@ -198,8 +198,11 @@ __device__ __forceinline__ void unpack_limbs(uint32_t* v, const typename lds_vec
 // is below 10 r, and the pass twiddle product with centred digits
 // (mont29 BM) lies in (-0.65 r, 0.65 r): its balanced digits fit a pbd word
 // (top digit below 2^23).  The last pass reduces its outputs to [0, r).
-template <int LA, bool INV, bool LAST>
+// SCALE: multiply every output by a.scale29 (single-pass inverse transforms;
+// a separate instance, so the other last-pass kernels keep no registers for it)
+template <int LA, bool INV, bool LAST, bool SCALE>
 __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, const uint64_t blk) {
+  static_assert(!SCALE || LAST, "the scale factor is applied by the last pass");
   constexpr int A = 1 << LA;
   constexpr int m = LA + kLgPerT;
   constexpr int LGC = kLgTile - m;  // log2 lines per tile
@@ -266,10 +269,12 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, const uint64_t 
         raw[r2] = fr_load(src + pos0 + (uint64_t)(rho1 + A * r2) * estride);
       });
       if (!LAST && a.premul) {
-        const fr* pm = a.premul + ((uint64_t)(poly & a.pm_mask) << a.lgN) + pos0;
+        const fr* pm = a.premul + ((uint64_t)(poly & a.pm_mask) << a.lgN) + pos0 + (uint64_t)rho1 * estride;
         // kPmG factors at a time: each group's loads start after the previous
         // group's products (the asm statements order them), so at most kPmG
-        // factors are held in registers beside the lane's elements
+        // factors are held in registers beside the lane's elements; the
+        // group's addresses are the opaque base plus wave-uniform offsets, so
+        // no per-lane address is computed ahead
         static_for<0, kPerT / kPmG>([&](auto g_) {
           constexpr int g = decltype(g_)::value;
           const fr* q = pm;
@@ -277,7 +282,7 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, const uint64_t 
           fr f[kPmG];
           static_for<0, kPmG>([&](auto r_) {
             constexpr int r = decltype(r_)::value;
-            f[r] = fr_load(q + (uint64_t)(rho1 + A * (kPmG * g + r)) * estride);
+            f[r] = fr_load(q + (uint64_t)(A * (kPmG * g + r)) * estride);
           });
           static_for<0, kPmG>([&](auto r_) {
             constexpr int r = decltype(r_)::value;
@@ -363,25 +368,42 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, const uint64_t 
       });
     });
   });
+  // no barrier follows: lanes of lines beyond the launch leave here
+  const uint64_t line2 = line0 + lc2;
+  if (line2 >= a.nlines) return;
+  // last pass: the lane's output base, computed before DFT_A so that its
+  // 64-bit temporaries are not live beside the eight results
+  fr* base = nullptr;
+  const uint32_t lgI = a.lgI;
+  if constexpr (LAST) {
+    uint64_t poly, lp;
+    if (lgI) {
+      poly = line2 & ((1ull << lgI) - 1);
+      lp = line2 >> lgI;
+    } else {
+      poly = line2 >> lgL;
+      lp = line2 & ((1ull << lgL) - 1);
+    }
+    fr* dst = a.out + ((poly >> lgI) << (a.lgN + lgI)) + (poly & ((1ull << lgI) - 1));
+    base = dst + (lp << lgI);
+    asm volatile("" : "+v"(base));
+  }
   if constexpr (LA > 0) {
     static_for<0, F2>([&](auto f_) { dft_dif<LA, INV, decltype(f_)::value * A>(y); });
   }
   // y[f*A + brev_LA(k1)] = Z[k2 + kPerT*k1], k2 = g*F2 + f
 
-  const uint64_t line2 = line0 + lc2;
-  if (line2 >= a.nlines) return;
   if (LAST) {
-    uint64_t poly, lp;
-    if (a.lgI) {
-      poly = line2 & ((1ull << a.lgI) - 1);
-      lp = line2 >> a.lgI;
-    } else {
-      poly = line2 >> lgL;
-      lp = line2 & ((1ull << lgL) - 1);
+    if constexpr (SCALE) {
+      // single-pass inverse transforms: the factor 2^-lg, once per element
+      uint32_t w[9];
+      pbd_unpack(a.scale29.l, w);
+      static_for<0, kPerT>([&](auto i_) {
+        constexpr int i = decltype(i_)::value;
+        pin9(y[i]);
+        y[i] = v9_mul<false, false, false>(y[i], w);
+      });
     }
-    const uint32_t lgI = a.lgI;
-    fr* dst = a.out + ((poly >> lgI) << (a.lgN + lgI)) + (poly & ((1ull << lgI) - 1));
-    fr* const base = dst + (lp << lgI);
     static_for<0, F2>([&](auto f_) {
       constexpr int f = decltype(f_)::value;
       const int k2 = g * F2 + f;
@@ -390,11 +412,6 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, const uint64_t 
         const uint64_t k = (uint64_t)(k2 + kPerT * k1);
         v9 v = y[f * A + brev(k1, LA)];
         pin9(v);
-        if (a.do_scale) {
-          uint32_t w[9];
-          pbd_unpack(a.scale29.l, w);
-          v = v9_mul<false, false, false>(v, w);
-        }
         uint32_t tt[9];
         reduce_below_2r(v.l, tt);
         fr o;
@@ -403,7 +420,9 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, const uint64_t 
         // eight addresses are not all computed ahead and held
         fr* d = base;
         asm volatile("" : "+v"(d));
-        fr_store(d + (k << (lgL + lgI)), fr_reduce_once(o));
+        fr c = fr_reduce_once(o);
+        pin(c);  // one element's reduction at a time
+        fr_store(d + (k << (lgL + lgI)), c);
       });
     });
   } else {
@@ -433,16 +452,16 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, const uint64_t 
 }
 
 
-template <int LA, bool INV, bool LAST>
+template <int LA, bool INV, bool LAST, bool SCALE>
 __global__ void __launch_bounds__(kThreads, PL_NTT_WAVES) ntt_pass_kernel(PassArgs a) {
-  ntt_pass_body<LA, INV, LAST>(a, blockIdx.x);
+  ntt_pass_body<LA, INV, LAST, SCALE>(a, blockIdx.x);
 }
 
 // Ragged form: one launch runs the tiles of many polynomials of different
 // sizes (one PassArgs per segment).  seg_block0[s] is segment s's first
 // block, ascending; a block finds its segment by binary search (uniform
 // scalar loads, a few steps for the tens of segments of a batch).
-template <int LA, bool INV, bool LAST>
+template <int LA, bool INV, bool LAST, bool SCALE>
 __global__ void __launch_bounds__(kThreads, PL_NTT_WAVES)
     ntt_pass_ragged_kernel(const PassArgs* __restrict__ segs, const uint32_t* __restrict__ seg_block0,
                            uint32_t nsegs) {
@@ -456,47 +475,56 @@ __global__ void __launch_bounds__(kThreads, PL_NTT_WAVES)
   // read the arguments in place; a private copy would put the dynamically
   // indexed lgrad[] in scratch
   const PassArgs& a = segs[lo];
-  ntt_pass_body<LA, INV, LAST>(a, b - seg_block0[lo]);
+  ntt_pass_body<LA, INV, LAST, SCALE>(a, b - seg_block0[lo]);
 }
 
 // Host launcher for one radix: LA = log2(column size) - kLgPerT.
-template <int LA, bool INV, bool LAST>
+template <int LA, bool INV, bool LAST, bool SCALE = false>
 static void launch_pass(const PassArgs& a, hipStream_t s) {
   constexpr int m = LA + kLgPerT;
   uint64_t tiles = (a.nlines + (kTile >> m) - 1) >> (kLgTile - m);
-  hipLaunchKernelGGL((ntt_pass_kernel<LA, INV, LAST>), dim3((unsigned)tiles), dim3(kThreads), 0, s, a);
+  hipLaunchKernelGGL((ntt_pass_kernel<LA, INV, LAST, SCALE>), dim3((unsigned)tiles), dim3(kThreads), 0, s, a);
 }
 
-template <int LA, bool INV, bool LAST>
+template <int LA, bool INV, bool LAST, bool SCALE = false>
 static void launch_pass_ragged(const PassArgs* segs, const uint32_t* seg_block0, uint32_t nsegs, uint32_t nblocks,
                                hipStream_t s) {
-  hipLaunchKernelGGL((ntt_pass_ragged_kernel<LA, INV, LAST>), dim3(nblocks), dim3(kThreads), 0, s, segs, seg_block0,
-                     nsegs);
+  hipLaunchKernelGGL((ntt_pass_ragged_kernel<LA, INV, LAST, SCALE>), dim3(nblocks), dim3(kThreads), 0, s, segs,
+                     seg_block0, nsegs);
 }
 
-#define PL_DEFINE_PASS_LAUNCHER(LA)                                        \
-  void launch_pass_la##LA(bool inv, bool last, const PassArgs& a, hipStream_t s) { \
-    if (inv) {                                                             \
-      if (last) launch_pass<LA, true, true>(a, s);                         \
-      else launch_pass<LA, true, false>(a, s);                             \
-    } else {                                                               \
-      if (last) launch_pass<LA, false, true>(a, s);                        \
-      else launch_pass<LA, false, false>(a, s);                            \
-    }                                                                      \
-  }                                                                        \
-  void launch_pass_ragged_la##LA(bool inv, bool last, const PassArgs* segs, const uint32_t* sb, \
-                                 uint32_t ns, uint32_t nb, hipStream_t s) {                   \
-    if (inv) {                                                             \
-      if (last) launch_pass_ragged<LA, true, true>(segs, sb, ns, nb, s);   \
-      else launch_pass_ragged<LA, true, false>(segs, sb, ns, nb, s);       \
-    } else {                                                               \
-      if (last) launch_pass_ragged<LA, false, true>(segs, sb, ns, nb, s);  \
-      else launch_pass_ragged<LA, false, false>(segs, sb, ns, nb, s);      \
-    }                                                                      \
+// scale (a.do_scale) only with inv && last
+#define PL_DEFINE_PASS_LAUNCHER(LA)                                                            \
+  void launch_pass_la##LA(bool inv, bool last, const PassArgs& a, hipStream_t s) {             \
+    if (inv) {                                                                                 \
+      if (last) {                                                                              \
+        if (a.do_scale) launch_pass<LA, true, true, true>(a, s);                               \
+        else launch_pass<LA, true, true>(a, s);                                                \
+      } else {                                                                                 \
+        launch_pass<LA, true, false>(a, s);                                                    \
+      }                                                                                        \
+    } else {                                                                                   \
+      if (last) launch_pass<LA, false, true>(a, s);                                            \
+      else launch_pass<LA, false, false>(a, s);                                                \
+    }                                                                                          \
+  }                                                                                            \
+  void launch_pass_ragged_la##LA(bool inv, bool last, bool scale, const PassArgs* segs, const uint32_t* sb, \
+                                 uint32_t ns, uint32_t nb, hipStream_t s) {                    \
+    if (inv) {                                                                                 \
+      if (last) {                                                                              \
+        if (scale) launch_pass_ragged<LA, true, true, true>(segs, sb, ns, nb, s);              \
+        else launch_pass_ragged<LA, true, true>(segs, sb, ns, nb, s);                          \
+      } else {                                                                                 \
+        launch_pass_ragged<LA, true, false>(segs, sb, ns, nb, s);                              \
+      }                                                                                        \
+    } else {                                                                                   \
+      if (last) launch_pass_ragged<LA, false, true>(segs, sb, ns, nb, s);                      \
+      else launch_pass_ragged<LA, false, false>(segs, sb, ns, nb, s);                          \
+    }                                                                                          \
   }
 
 #define PL_DECLARE_RAGGED(LA)                                                                  \
-  void launch_pass_ragged_la##LA(bool inv, bool last, const PassArgs* segs, const uint32_t* sb, \
+  void launch_pass_ragged_la##LA(bool inv, bool last, bool scale, const PassArgs* segs, const uint32_t* sb, \
                                  uint32_t ns, uint32_t nb, hipStream_t s);
 // Kernel arguments of pass p of the call c when every pass is a regular
 // tile pass (false for the tiny and single-pass premultiplied forms, which

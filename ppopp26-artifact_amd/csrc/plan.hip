@@ -77,7 +77,7 @@ hipError_t phase_plan(PhasePlan& ph, const std::vector<Item>& items, Bump& bump,
   hipError_t e = hipSuccess;
   ph = PhasePlan();
   ph.items = items;
-  std::map<std::tuple<int, int, int, int>, size_t> gidx;
+  std::map<std::tuple<int, int, int, int, int>, size_t> gidx;
   int n_eval = 0;
   for (size_t i = 0; i < ph.items.size(); i++) {
     const Item& it = ph.items[i];
@@ -129,7 +129,7 @@ hipError_t phase_plan(PhasePlan& ph, const std::vector<Item>& items, Bump& bump,
       pass_args_regular(*d, c, p, a);
       const int la = d->plan.m[p] - kLgPerT;
       const bool last = p == d->plan.P - 1;
-      const auto key = std::make_tuple(p, la, c.inverse ? 1 : 0, last ? 1 : 0);
+      const auto key = std::make_tuple(p, la, c.inverse ? 1 : 0, last ? 1 : 0, a.do_scale ? 1 : 0);
       auto f = gidx.find(key);
       if (f == gidx.end()) {
         RaggedGroup g;
@@ -137,6 +137,7 @@ hipError_t phase_plan(PhasePlan& ph, const std::vector<Item>& items, Bump& bump,
         g.la = la;
         g.inv = c.inverse;
         g.last = last;
+        g.scale = a.do_scale != 0;
         f = gidx.emplace(key, ph.groups.size()).first;
         ph.groups.push_back(g);
       }
@@ -185,11 +186,11 @@ hipError_t phase_prepare(const PhasePlan& ph, hipStream_t s) {
 static void launch_ragged(const RaggedGroup& g, hipStream_t s) {
   const uint32_t ns = (uint32_t)g.segs.size();
   switch (g.la) {
-    case 0: launch_pass_ragged_la0(g.inv, g.last, g.d_segs, g.d_seg_block0, ns, g.nblocks, s); break;
-    case 1: launch_pass_ragged_la1(g.inv, g.last, g.d_segs, g.d_seg_block0, ns, g.nblocks, s); break;
-    case 2: launch_pass_ragged_la2(g.inv, g.last, g.d_segs, g.d_seg_block0, ns, g.nblocks, s); break;
-    case 3: launch_pass_ragged_la3(g.inv, g.last, g.d_segs, g.d_seg_block0, ns, g.nblocks, s); break;
-    default: launch_pass_ragged_la4(g.inv, g.last, g.d_segs, g.d_seg_block0, ns, g.nblocks, s); break;
+    case 0: launch_pass_ragged_la0(g.inv, g.last, g.scale, g.d_segs, g.d_seg_block0, ns, g.nblocks, s); break;
+    case 1: launch_pass_ragged_la1(g.inv, g.last, g.scale, g.d_segs, g.d_seg_block0, ns, g.nblocks, s); break;
+    case 2: launch_pass_ragged_la2(g.inv, g.last, g.scale, g.d_segs, g.d_seg_block0, ns, g.nblocks, s); break;
+    case 3: launch_pass_ragged_la3(g.inv, g.last, g.scale, g.d_segs, g.d_seg_block0, ns, g.nblocks, s); break;
+    default: launch_pass_ragged_la4(g.inv, g.last, g.scale, g.d_segs, g.d_seg_block0, ns, g.nblocks, s); break;
   }
 }
 

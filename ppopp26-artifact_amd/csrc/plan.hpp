@@ -57,7 +57,7 @@ struct Bump {
 
 struct RaggedGroup {
   int level = 0, la = 0;
-  bool inv = false, last = false;
+  bool inv = false, last = false, scale = false;
   std::vector<PassArgs> segs;
   std::vector<uint32_t> seg_block0;
   PassArgs* d_segs = nullptr;
