@@ -66,6 +66,8 @@ def parse():
     ap.add_argument("--plans", default=os.path.join(ROOT, "ppopp26-artifact_amd", "tune", "mi355x_plans.txt"),
                     help="autotuned pass-split cache (measured and written when absent)")
     ap.add_argument("--no-tune", action="store_true", help="default pass splits")
+    ap.add_argument("--settle-ms", type=float, default=300.0,
+                    help="untimed device time each section runs before timing (0: none, e.g. under counters)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain device copy of 2^lg * 32 bytes (HBM counter calibration)")
     return ap.parse_args()
@@ -83,10 +85,14 @@ def time_events(fn, reps, stream):
     return e0.elapsed_time(e1) / reps  # ms
 
 
-def settle(fn, stream, min_ms=300.0, chunk=4):
+SETTLE_MS = 300.0
+
+
+def settle(fn, stream, min_ms=None, chunk=4):
     """Untimed: run fn until at least min_ms of device time has passed.  The
     clocks drop for some tens of milliseconds when a heavy VALU load starts
     and then recover; timing starts once they have settled under this load."""
+    min_ms = SETTLE_MS if min_ms is None else min_ms
     spent = 0.0
     while spent < min_ms:
         spent += time_events(fn, chunk, stream) * chunk
@@ -424,6 +430,8 @@ def cpu_baseline(lg):
 
 def main():
     args = parse()
+    global SETTLE_MS
+    SETTLE_MS = args.settle_ms
     if args.gpus != 1:
         print("bench.py: this build measures one GPU only (no multi-GPU path); run with --gpus 1",
               file=sys.stderr)
