@@ -46,7 +46,7 @@ MAD_PER_MULMOD = 128  # 8-limb Montgomery product: 2 * 8^2 limb multiply-adds
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs scaled by a plain copy of
 # known size (profiles/ntt_pass_2p24_hbm_traffic.txt): the data (2^24 x 32 B
 # read and written) plus the pass's 2^24-entry twiddle table read once
-TRAFFIC_PASS0_2P24 = 1622295808
+TRAFFIC_PASS0_2P24 = 1611690880
 
 
 def parse():
@@ -173,7 +173,7 @@ def bench_ntt(pl, synth, lg, steps, warmup, stream, peak):
         "traffic": TRAFFIC_PASS0_2P24 if lg == 24 else None,
         "peak_source": "measured v_mad_u64_u32 rate (pl_probe_mad_rate); spec full rate %.2f" % (
             PEAK_MAD_SPEC / 1e12),
-        "kernel": "ntt_pass_kernel<%d,false,false> (pass 0 of %d, 2^%d-point columns)" % (m_pass[0] - 3, npass,
+        "kernel": "ntt_pass_kernel<%d,false,false,false> (pass 0 of %d, 2^%d-point columns)" % (m_pass[0] - 3, npass,
                                                                                           m_pass[0]),
         "alg_per_launch": "%d mulmod = (N/2)*%d, x128 u32 multiply-adds; %d bytes = 64 N" % (
             (n // 2) * m_pass[0], m_pass[0], byts),
