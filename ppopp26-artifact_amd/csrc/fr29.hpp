@@ -1,28 +1,31 @@
-// Montgomery product in radix 2^29 for the NTT passes.
+// Montgomery product in radix 2^29 for the NTT passes and the scans.
 //
 // The saturated 8 x 32-bit product (fr.hpp) spends one carry fold per
 // multiply-add because a column of 32x32-bit products overflows 64 bits.  Here
-// both operands are split into nine 29-bit limbs: every partial product is
-// below 2^58 and a column holds at most 17 of them plus a carry below 2^35, so
-// a column sums in one 64-bit accumulator with no carry handling: a chain of
-// v_mad_i64_i32 whose carry-out masks are never read (153 multiply-adds and
-// about 70 other instructions per product, against 120 + 120 folds + about
-// 50 for the saturated form).  `mont29` is the plain-C statement (host tests);
-// the device runs the same columns as generated asm blocks (`mont29_dev`).
+// both operands are nine limbs of radix 2^29: every partial product is below
+// 2^59 in magnitude and a column holds at most 17 of them plus a carry, so a
+// column sums in one signed 64-bit accumulator with no carry handling: a chain
+// of v_mad_i64_i32 whose carry-out masks are never read (153 multiply-adds and
+// 17 column shifts per product, against 120 + 120 folds + about 50 for the
+// saturated form).  `mont29` is the plain-C statement (host tests); the device
+// runs the same columns as generated asm blocks (`mont29_dev`).
 //
-//   R' = 2^261 (nine digits), r = 1 mod 2^29: with the signed digit
-//   m = acc mod 2^29, subtracting m r clears the low 29 bits exactly.
-//   a * w / R' mod r, output in (0, 1.031 r) for a < 2^256, w < r.
+//   R' = 2^261 (nine digits), r = 1 mod 2^29: with the digit m = acc mod 2^29
+//   (or its balanced form), subtracting m r clears the low 29 bits exactly.
 //
-// Data stays in the stored form (x R mod r, R = 2^256, eight 32-bit limbs,
-// lazily in [0, 2r)); a factor w enters as w R' mod r = (w R) 2^5 mod r in
-// nine 29-bit limbs (`fr9`), so mont29(x R, w R') = x w R.  Tables and
-// constants for the passes are kept in that form.
+// Two ways of holding the data:
+//  * the scans keep elements in the stored form (x R mod r, R = 2^256, eight
+//    32-bit limbs) and `fr_mul29` splits, multiplies, joins and adds r: a w / R'
+//    mod r in (0, 1.031 r) for a < 2^256, w < r;
+//  * the NTT passes keep elements as nine signed 32-bit limbs (`v9`) from load
+//    to store: sums and differences are limb-wise, and the product takes any
+//    signed limbs when the factor has balanced digits (|w_j| <= 2^28).  Factors
+//    reach the passes as packed balanced digits ("pbd" words, below).
+// A factor w enters as w R' mod r = (w R) 2^5 mod r, so mont29(x R, w R') =
+// x w R: the data never changes form.
 //
-// Column bound: a_i, w_j, m_i, r_j < 2^29 (a < 2^256 gives a_8 < 2^24): the
-// signed column sum stays within 9 * 2^58 + 2^35 < 2^62 in magnitude.  Value:
-// T = (a w - m r) / R' lies in (-r, a r / R') = (-r, 0.031 r), and the
-// product returns T + r.
+// Column bound: |a_i| < 2^31 with |w_j| <= 2^28, or a_i, w_j < 2^29: the
+// signed column sum stays below 9 * 2^59 + 8 * 2^58 + 2^35 < 2^62.8.
 #pragma once
 #include <math.h>
 #include <stdint.h>
