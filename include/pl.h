@@ -72,7 +72,9 @@ int pl_ntt_batch(const void* d_in, void* d_out, int lg, uint32_t npoly, int inve
 size_t pl_ntt_work_bytes(int lg, uint32_t npoly);
 
 /* Pass-level access to the same transform (the intermediate layout is this
- * library's own; the reference's per-chunk step with its own layout is
+ * library's own, and intermediate values are 32-byte packed balanced-digit
+ * words congruent to the field elements, not canonical Montgomery values;
+ * the reference's per-chunk step with its own layout is
  * pl_ntt_stage_no_rotate below).  Running passes
  * 0 .. pl_ntt_num_passes(lg)-1 in order, each reading the previous pass's
  * output, equals pl_ntt_batch.  Intermediate passes may run in place; the
