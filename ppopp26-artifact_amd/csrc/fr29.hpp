@@ -382,6 +382,15 @@ __device__ __forceinline__ v9 v9_mul(const v9& x, const uint32_t (&w)[9]) {
   return t;
 }
 
+// the canonical value in [0, r) of v (|v_i| < 2^31)
+__device__ __forceinline__ fr v9_canonical(const v9& v) {
+  uint32_t t[9];
+  reduce_below_2r(v.l, t);
+  fr o;
+  join29(t, o.l);
+  return fr_reduce_once(o);
+}
+
 // an ordering point for the nine limbs of v (no instruction)
 __device__ __forceinline__ void pin9(v9& v) {
   asm volatile("" : "+v"(v.l[0]), "+v"(v.l[1]), "+v"(v.l[2]), "+v"(v.l[3]), "+v"(v.l[4]), "+v"(v.l[5]),

@@ -195,6 +195,54 @@ int sim_column(int la, int last, int ext, const uint32_t* in, const uint32_t* w1
   return g_bad;
 }
 
+// The permutation-ratio terms of one element (ops.hip perm_terms_kernel) in
+// the same arithmetic; returns the number of violated bounds.  b32: 32 S(beta)
+// mod r; k29: three pbd words; the other inputs are stored-form words.
+int sim_perm_terms(const uint32_t* w0, const uint32_t* w1, const uint32_t* b32, const uint32_t* gamma,
+                   const uint32_t* k29, const uint32_t* w, const uint32_t* s, uint32_t* num_out,
+                   uint32_t* den_out) {
+  g_bad = 0;
+  uint32_t B[9], b[9];
+  unpack_ext(b32, B);
+  unpack_ext(gamma, b);
+  const V g = from_u(b);
+  unpack_ext(w0, b);
+  const V x0 = from_u(b);
+  unpack_ext(w1, b);
+  const V wi = mul(x0, b, false, false);
+  const V bx = mul(wi, B, false, false);
+  V num, den;
+  for (int j = 0; j < 3; j++) {
+    unpack_ext(w + 8 * j, b);
+    const V wg = add(from_u(b), g);
+    uint32_t K[9];
+    pbd_unpack(k29 + 8 * j, K);
+    const V nj = add(wg, mul(bx, K, false, true));
+    unpack_ext(s + 8 * j, b);
+    const V dj = add(wg, mul(from_u(b), B, false, true));
+    uint32_t u[9];
+    if (j == 0) {
+      num = nj;
+      den = dj;
+    } else {
+      to_u(nj, u);
+      num = mul(num, u, false, false);
+      to_u(dj, u);
+      den = mul(den, u, false, false);
+    }
+  }
+  uint32_t u[9], t[9];
+  to_u(num, u);
+  reduce_below_2r(u, t);
+  join29(t, num_out);
+  reduce_once(num_out);
+  to_u(den, u);
+  reduce_below_2r(u, t);
+  join29(t, den_out);
+  reduce_once(den_out);
+  return g_bad;
+}
+
 // fr29.hpp's host functions, one call each
 void t_mont29(const uint32_t* a, const uint32_t* w, uint32_t* t, int bm, int bt) {
   if (bm && bt) mont29<true, true>(a, w, t);

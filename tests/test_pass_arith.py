@@ -16,7 +16,10 @@ csrc/ntt_pass.hpp) on the host, against Python integers.
   output in range, and the outputs equal the column DFT computed with Python
   integers (times the pass twiddles, or canonical in the last pass).  Inputs
   include the extremes: all 2^256 - 1 (the largest DC sums), all digits at
-  -2^28 or 2^28 - 1, and chains of two passes.
+  -2^28 or 2^28 - 1, and chains of two passes;
+* the permutation-ratio terms (ops.hip perm_terms_kernel) of one element in
+  the same arithmetic: every bound holds and num / den carry exactly the
+  common factor 2^-10 (random and all-(2^256 - 1) inputs).
 """
 import ctypes
 import json
@@ -275,3 +278,39 @@ def test_two_passes_chained(lib):
         x = [in_value(mid, j, False) % R for j in range(Mp)]
         X = column_dft(x, la, False)
         assert [out_value(out, k, True) for k in range(Mp)] == X
+
+
+def test_perm_terms_model(lib):
+    rinv = pow(1 << 256, -1, R)
+    rng = random.Random(21)
+    for trial in range(300):
+        big = trial < 20
+        vals = [((1 << 256) - 1) if big else rng.randrange(R) for _ in range(10)]
+        w0, w1, beta, gamma = vals[:4]
+        ws, ss = vals[4:7], vals[7:10]
+        k = [1, 7, 13] if trial % 2 else [rng.randrange(R) for _ in range(3)]
+        b32 = 32 * beta % R
+        k29 = []
+        for kj in k:
+            b = balanced(1024 * kj * (1 << 256) % R)  # pbd word of (32 k) R' = 1024 k R
+            word = sum(((x & M29) if i < 8 else (x & 0xFFFFFF)) << (29 * i) for i, x in enumerate(b))
+            k29 += limbs(word, 8, 32)
+        out_n = (U32 * 8)()
+        out_d = (U32 * 8)()
+        bad = lib.sim_perm_terms(arr(limbs(w0, 8, 32), 8), arr(limbs(w1, 8, 32), 8), arr(limbs(b32, 8, 32), 8),
+                                 arr(limbs(gamma, 8, 32), 8), arr(k29, 24),
+                                 arr(sum((limbs(x, 8, 32) for x in ws), []), 24),
+                                 arr(sum((limbs(x, 8, 32) for x in ss), []), 24), out_n, out_d)
+        assert bad == 0, trial
+        # field values of the stored-form inputs (x R): a = x R^-1
+        f = lambda x: x * rinv % R  # noqa: E731
+        wi = f(w0) * f(w1) % R
+        num = den = 1
+        for j in range(3):
+            num = num * (f(ws[j]) + f(gamma) + f(beta) * k[j] * wi) % R
+            den = den * (f(ws[j]) + f(gamma) + f(beta) * f(ss[j])) % R
+        inv1024 = pow(1024, -1, R)
+        got_n = sum(out_n[i] << (32 * i) for i in range(8))
+        got_d = sum(out_d[i] << (32 * i) for i in range(8))
+        assert got_n == num * inv1024 % R * (1 << 256) % R
+        assert got_d == den * inv1024 % R * (1 << 256) % R
