@@ -96,6 +96,25 @@ def test_perm_ratio_golden(pl, golden):
         np.testing.assert_array_equal(pl.to_host(out), k["perm_ratio_%d" % lg])
 
 
+def test_perm_ratio_extreme_columns(pl):
+    """columns, beta and gamma at r - 1 (and zeros mixed in), against the oracle"""
+    lg = 12
+    n = 1 << lg
+    rm1 = [0xFFFFFFFF00000000, 0x53BDA402FFFE5BFE, 0x3339D80809A1D805, 0x73EDA753299D7D48]
+    cols = []
+    for j in range(6):
+        c = np.zeros((n, 4), dtype=np.uint64)
+        c[j % 2::2] = rm1
+        cols.append(c)
+    beta = np.array([rm1], dtype=np.uint64)
+    gamma = np.array([rm1], dtype=np.uint64)
+    ks = [po.from_u64(1), po.from_u64(7), po.from_u64(13)]
+    dom = pl.Domain(lg)
+    dev = [pl.to_dev(c) for c in cols]
+    ratio = pl.perm_ratio(dev[:3], dev[3:], pl.to_dev(beta), pl.to_dev(gamma), np.concatenate(ks), dom)
+    np.testing.assert_array_equal(pl.to_host(ratio), po.perm_ratio(cols, beta, gamma, ks, lg))
+
+
 @pytest.mark.parametrize("lg", [12, 20])
 def test_perm_grand_product_vs_oracle(pl, lg):
     n = 1 << lg

@@ -39,6 +39,28 @@ def test_ntt_vs_oracle(pl, lg):
     np.testing.assert_array_equal(z, po.ntt(x, lg, inverse=True))
 
 
+R_MINUS_1 = [0xFFFFFFFF00000000, 0x53BDA402FFFE5BFE, 0x3339D80809A1D805, 0x73EDA753299D7D48]
+
+
+@pytest.mark.parametrize("lg", [6, 9, 12, 16, 20])
+@pytest.mark.parametrize("pattern", ["r-1", "alternating", "one-hot"])
+def test_ntt_extreme_inputs(pl, lg, pattern):
+    """Inputs at the top of the canonical range drive the largest sums through
+    the passes' signed limbs (every DC sum of r - 1), against the oracle."""
+    n = 1 << lg
+    x = np.zeros((n, 4), dtype=np.uint64)
+    if pattern == "r-1":
+        x[:] = R_MINUS_1
+    elif pattern == "alternating":
+        x[0::2] = R_MINUS_1
+    else:
+        x[n // 3] = R_MINUS_1
+    dom = pl.Domain(lg)
+    print("ntt extreme lg=%d %s" % (lg, pattern), flush=True)
+    np.testing.assert_array_equal(pl.to_host(pl.ntt(pl.to_dev(x), dom)), po.ntt(x, lg))
+    np.testing.assert_array_equal(pl.to_host(pl.ntt(pl.to_dev(x), dom, inverse=True)), po.ntt(x, lg, inverse=True))
+
+
 @pytest.mark.parametrize("lg", [22, 24])
 def test_ntt_roundtrip_large(pl, lg):
     # size-independent properties at the benchmark sizes:

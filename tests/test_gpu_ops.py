@@ -38,6 +38,17 @@ def test_lde_vs_oracle(pl, lg_n, lb):
     np.testing.assert_array_equal(out, po.lde(c, lg_n, lb, po.ref_lde_shift(lb)))
 
 
+@pytest.mark.parametrize("lg_n,lb", [(12, 2), (16, 3)])
+def test_lde_extreme_inputs(pl, lg_n, lb):
+    """all coefficients r - 1: the largest sums through the coset passes"""
+    c = np.zeros((1 << lg_n, 4), dtype=np.uint64)
+    c[:] = [0xFFFFFFFF00000000, 0x53BDA402FFFE5BFE, 0x3339D80809A1D805, 0x73EDA753299D7D48]
+    print("lde extreme", lg_n, lb, flush=True)
+    dom = pl.LdeDomain(lg_n, lb)
+    out = pl.to_host(pl.lde(pl.to_dev(c), dom))
+    np.testing.assert_array_equal(out, po.lde(c, lg_n, lb, po.ref_lde_shift(lb)))
+
+
 def test_lde_in_place(pl):
     import torch
     lg_n, lb = 10, 2
