@@ -6,6 +6,15 @@
 
 namespace pl {
 
+// x times a premultiplication factor (pbd word of c R'), canonical: x c
+__device__ __forceinline__ fr premul_one(const fr& x, const fr& f) {
+  v9 a;
+  unpack_ext(x.l, a.l);
+  uint32_t w[9];
+  pbd_unpack(f.l, w);
+  return v9_canonical(v9_mul<false, false, false>(a, w));
+}
+
 // Sizes 2^1 .. 2^3: one thread per output, direct sum.  Blocks hold whole
 // polynomials (blockDim is a multiple of 8), all reads precede the barrier,
 // so in-place calls are safe.
@@ -22,7 +31,7 @@ __global__ void __launch_bounds__(256) ntt_tiny_kernel(PassArgs a, uint32_t npol
     const fr* pm = a.premul ? a.premul + ((uint64_t)(poly & a.pm_mask) << lg) : nullptr;
     for (uint32_t j = 0; j < n; j++) {
       fr v = fr_load(src + j);
-      if (pm) v = fr_mul(v, fr_load(pm + j));
+      if (pm) v = premul_one(v, fr_load(pm + j));
       const fr w = fr_load(a.inner + (((j * k) & (n - 1)) << (8 - lg)));
       acc = fr_add(acc, fr_mul(v, w));
     }
@@ -43,7 +52,7 @@ __global__ void premul_kernel(const fr* in, fr* out, uint64_t in_pstride, const 
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= total) return;
   uint64_t poly = i >> lg, pos = i & ((1ull << lg) - 1);
-  fr v = fr_mul(fr_load(in + poly * in_pstride + pos), fr_load(premul + ((uint64_t)(poly & pm_mask) << lg) + pos));
+  fr v = premul_one(fr_load(in + poly * in_pstride + pos), fr_load(premul + ((uint64_t)(poly & pm_mask) << lg) + pos));
   fr_store(out + i, v);
 }
 
@@ -229,7 +238,8 @@ done:
   return e;
 }
 
-// cp[t << lg_n | i] = (shift * w_M^t)^i, M = 2^(lg_n + lg_blowup)
+// cp[t << lg_n | i] = (shift * w_M^t)^i, M = 2^(lg_n + lg_blowup), as pbd words
+// of the value times R' (the factor form of the pass kernels' product)
 hipError_t coset_table_build(fr** cp, size_t* bytes, int lg_n, int lg_blowup, const host::fr64& shift,
                              hipStream_t s) {
   const uint64_t n = 1ull << lg_n;
@@ -240,10 +250,15 @@ hipError_t coset_table_build(fr** cp, size_t* bytes, int lg_n, int lg_blowup, co
   fr* win = nullptr;
   if ((e = hipMalloc((void**)&win, 2 * 16384 * sizeof(fr))) != hipSuccess) return e;
   const host::fr64 wM = host::root_of_unity(lg_n + lg_blowup);
+  const host::fr64 c32 = host::from_u64(32);  // stored form -> c R' mod r
   host::fr64 st = shift;
   for (uint64_t t = 0; t < B; t++) {
-    if ((e = fill_pow(*cp + t * n, n, 0, st, nullptr, win, s)) != hipSuccess) break;
+    if ((e = fill_pow(*cp + t * n, n, 0, st, &c32, win, s)) != hipSuccess) break;
     st = host::mul(st, wM);
+  }
+  if (e == hipSuccess) {
+    hipLaunchKernelGGL(pbd_kernel, dim3(grid_for(n * B, 256)), dim3(256), 0, s, *cp, n * B);
+    e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(s);
   (void)hipFree(win);

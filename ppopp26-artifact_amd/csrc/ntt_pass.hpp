@@ -117,7 +117,7 @@ __device__ __forceinline__ void dft_dif(v9 (&x)[NX]) {
 struct PassArgs {
   const fr* in;
   fr* out;
-  const fr* premul;     // !LAST: optional factor per input element [(poly & pm_mask) << lgN | pos]
+  const fr* premul;     // !LAST: optional factor per input element [(poly & pm_mask) << lgN | pos], pbd words of c R''
   const fr* tw;         // !LAST: pass twiddles (pbd words of w R' mod r), index j + S*k within the sub-problem
   const fr* inner;      // w_256^e (or w_256^-e), e < 256, stored form (small-size kernel)
   const fr* inner29;    // the same as pbd words of w R' mod r (pass kernels)
@@ -263,38 +263,48 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, const uint64_t 
     // premultiplication factor), a previous pass's output is a pbd word.  Each
     // branch loads and unpacks on its own, so no loaded word lives across it.
     if (a.in_ext) {
-      fr raw[kPerT];
-      static_for<0, kPerT>([&](auto r2_) {
-        constexpr int r2 = decltype(r2_)::value;
-        raw[r2] = fr_load(src + pos0 + (uint64_t)(rho1 + A * r2) * estride);
-      });
       if (!LAST && a.premul) {
         const fr* pm = a.premul + ((uint64_t)(poly & a.pm_mask) << a.lgN) + pos0 + (uint64_t)rho1 * estride;
-        // kPmG factors at a time: each group's loads start after the previous
-        // group's products (the asm statements order them), so at most kPmG
-        // factors are held in registers beside the lane's elements; the
-        // group's addresses are the opaque base plus wave-uniform offsets, so
-        // no per-lane address is computed ahead
+        const fr* sp = src + pos0 + (uint64_t)rho1 * estride;
+        // factors are pbd words of c R' (the coset powers); the product with
+        // balanced output digits lies in (-r, 0.03 r) and leaves balanced
+        // digits, as the DFT wants.  kPmG elements and factors at a time: each
+        // group's loads start after the previous group's products (the asm
+        // statements order them), so few loaded words are held beside the
+        // lane's results; addresses are opaque bases plus wave-uniform offsets
         static_for<0, kPerT / kPmG>([&](auto g_) {
           constexpr int g = decltype(g_)::value;
           const fr* q = pm;
-          asm volatile("" : "+v"(q));
-          fr f[kPmG];
+          const fr* d = sp;
+          asm volatile("" : "+v"(q), "+v"(d));
+          fr f[kPmG], v[kPmG];
           static_for<0, kPmG>([&](auto r_) {
             constexpr int r = decltype(r_)::value;
+            v[r] = fr_load(d + (uint64_t)(A * (kPmG * g + r)) * estride);
             f[r] = fr_load(q + (uint64_t)(A * (kPmG * g + r)) * estride);
           });
           static_for<0, kPmG>([&](auto r_) {
             constexpr int r = decltype(r_)::value;
-            raw[kPmG * g + r] = fr_mul(raw[kPmG * g + r], f[r]);
-            pin(raw[kPmG * g + r]);
+            constexpr int k = kPmG * g + r;
+            v9 xx;
+            unpack_ext(v[r].l, xx.l);
+            uint32_t w[9];
+            pbd_unpack(f[r].l, w);
+            x[k] = v9_mul<false, false, true>(xx, w);
+            pin9(x[k]);
           });
         });
+      } else {
+        fr raw[kPerT];
+        static_for<0, kPerT>([&](auto r2_) {
+          constexpr int r2 = decltype(r2_)::value;
+          raw[r2] = fr_load(src + pos0 + (uint64_t)(rho1 + A * r2) * estride);
+        });
+        static_for<0, kPerT>([&](auto r2_) {
+          constexpr int r2 = decltype(r2_)::value;
+          unpack_ext(raw[r2].l, x[r2].l);
+        });
       }
-      static_for<0, kPerT>([&](auto r2_) {
-        constexpr int r2 = decltype(r2_)::value;
-        unpack_ext(raw[r2].l, x[r2].l);
-      });
     } else {
       static_for<0, kPerT>([&](auto r2_) {
         constexpr int r2 = decltype(r2_)::value;
