@@ -46,7 +46,7 @@ MAD_PER_MULMOD = 128  # 8-limb Montgomery product: 2 * 8^2 limb multiply-adds
 # rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE runs scaled by a plain copy of
 # known size (profiles/ntt_pass_2p24_hbm_traffic.txt): the data (2^24 x 32 B
 # read and written) plus the pass's 2^24-entry twiddle table read once
-TRAFFIC_PASS0_2P24 = 1611690880
+TRAFFIC_PASS0_2P24 = 1611643008
 
 
 def parse():
