@@ -66,9 +66,20 @@ __device__ fr block_excl_prod(fr v, uint4 (*l)[kScanThreads], fr* block_total) {
   return excl;
 }
 
+// Block geometry of a scan kernel: 256 threads x 2^LG elements.  LG = kLgPer
+// for the bulk; LG = 0 for a top level of at most 256 values, where the
+// latency of the per-thread chains (2^LG dependent products) dominates.
+template <int LG>
+struct ScanGeom {
+  static constexpr int kPer = 1 << LG;
+  static constexpr int kLgSeg = 8 + LG;
+};
+
 // partial[b] = product of block b's elements.  The product is order-free,
-// so thread t takes elements b*4096 + 256 k + t: every load is coalesced.
+// so thread t takes elements b*kSeg + 256 k + t: every load is coalesced.
+template <int LG>
 __global__ void __launch_bounds__(kScanThreads) pfx_reduce_kernel(const fr* in, uint64_t n, fr* partial) {
+  constexpr int kPer = ScanGeom<LG>::kPer, kLgSeg = ScanGeom<LG>::kLgSeg, kLgPer = LG;
   __shared__ uint4 l[2][kScanThreads];
   const uint64_t b0 = (uint64_t)blockIdx.x << kLgSeg;
   fr v[kPer];
@@ -101,8 +112,10 @@ __global__ void __launch_bounds__(kScanThreads) pfx_reduce_kernel(const fr* in, 
 
 // out_excl[i] = carry_b * prod_{block start <= j < i} in[j]; optional inclusive
 // output and block total (single-block use).  carry == nullptr means one.
+template <int LG>
 __global__ void __launch_bounds__(kScanThreads) pfx_down_kernel(const fr* in, fr* out_excl, fr* out_incl, uint64_t n,
                                                                  const fr* carry, fr* total) {
+  constexpr int kPer = ScanGeom<LG>::kPer, kLgSeg = ScanGeom<LG>::kLgSeg;
   __shared__ uint4 l[2][kScanThreads];
   const uint64_t base = ((uint64_t)blockIdx.x << kLgSeg) + (uint64_t)threadIdx.x * kPer;
   fr e[kPer];
@@ -145,18 +158,23 @@ static uint64_t pfx_scratch_elems(uint64_t n) {
 
 static hipError_t prefix_product(const fr* in, fr* out_excl, fr* out_incl, uint64_t n, fr* total, fr* scratch,
                                  hipStream_t s) {
+  if (n <= (uint64_t)kScanThreads) {
+    hipLaunchKernelGGL(pfx_down_kernel<0>, dim3(1), dim3(kScanThreads), 0, s, in, out_excl, out_incl, n,
+                       (const fr*)nullptr, total);
+    return hipGetLastError();
+  }
   if (n <= (uint64_t)kSeg) {
-    hipLaunchKernelGGL(pfx_down_kernel, dim3(1), dim3(kScanThreads), 0, s, in, out_excl, out_incl, n,
+    hipLaunchKernelGGL(pfx_down_kernel<kLgPer>, dim3(1), dim3(kScanThreads), 0, s, in, out_excl, out_incl, n,
                        (const fr*)nullptr, total);
     return hipGetLastError();
   }
   const uint64_t nb = (n + kSeg - 1) / kSeg;
   fr* part = scratch;
   fr* part_excl = scratch + nb;
-  hipLaunchKernelGGL(pfx_reduce_kernel, dim3(sgrid(n)), dim3(kScanThreads), 0, s, in, n, part);
+  hipLaunchKernelGGL(pfx_reduce_kernel<kLgPer>, dim3(sgrid(n)), dim3(kScanThreads), 0, s, in, n, part);
   hipError_t e = prefix_product(part, part_excl, nullptr, nb, total, scratch + 2 * nb, s);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(pfx_down_kernel, dim3(sgrid(n)), dim3(kScanThreads), 0, s, in, out_excl, out_incl, n,
+  hipLaunchKernelGGL(pfx_down_kernel<kLgPer>, dim3(sgrid(n)), dim3(kScanThreads), 0, s, in, out_excl, out_incl, n,
                      (const fr*)part_excl, (fr*)nullptr);
   return hipGetLastError();
 }
@@ -179,9 +197,11 @@ __host__ __device__ constexpr uint32_t rp_one_limb(int i) {
        : i == 4 ? 0x5dbeee8bu : i == 5 ? 0xfe2eedcdu : i == 6 ? 0x9eefbe41u : 0x4d043f42u;
 }
 
-// block suffix combine: V_t <- V_t + m^(16*2^s) V_{t+2^s}; returns the suffix
-// value starting at thread t (the block's carry folded in by the caller)
+// block suffix combine: V_t <- V_t + m^(2^LG * 2^s) V_{t+2^s}; returns the
+// suffix value starting at thread t (the block's carry folded in by the caller)
+template <int LG>
 __device__ fr block_suffix(fr v, uint4 (*l)[kScanThreads], const fr* pw) {
+  constexpr int kLgPer = LG;
   const int t = threadIdx.x;
   lds_put(l, t, v);
   __syncthreads();
@@ -200,7 +220,9 @@ __device__ fr block_suffix(fr v, uint4 (*l)[kScanThreads], const fr* pw) {
 // order-free: thread t takes elements start + 256 j + t (coalesced loads),
 // H_t = sum_j a m^(256 j) by Horner in m^256, then m^t H_t; a plain sum over
 // the block finishes it.
+template <int LG>
 __global__ void __launch_bounds__(kScanThreads) sfx_reduce_kernel(const fr* a, uint64_t n, const fr* pw, fr* partial) {
+  constexpr int kPer = ScanGeom<LG>::kPer, kLgSeg = ScanGeom<LG>::kLgSeg;
   __shared__ uint4 l[2][kScanThreads];
   const int t = threadIdx.x;
   const uint64_t b0 = (uint64_t)blockIdx.x << kLgSeg;
@@ -240,8 +262,10 @@ __global__ void __launch_bounds__(kScanThreads) sfx_reduce_kernel(const fr* a, u
 
 // out[i] = s_{i+1} within each block given carry[b] = s at the block's end
 // (nullptr: zero); total (block 0) = s_0.  out may equal a.
+template <int LG>
 __global__ void __launch_bounds__(kScanThreads) sfx_down_kernel(const fr* a, fr* out, uint64_t n, const fr* pw,
                                                                  const fr* carry, fr* total) {
+  constexpr int kPer = ScanGeom<LG>::kPer, kLgSeg = ScanGeom<LG>::kLgSeg, kLgPer = LG;
   __shared__ uint4 l[2][kScanThreads];
   const int t = threadIdx.x;
   const uint64_t base = ((uint64_t)blockIdx.x << kLgSeg) + (uint64_t)t * kPer;
@@ -261,7 +285,7 @@ __global__ void __launch_bounds__(kScanThreads) sfx_down_kernel(const fr* a, fr*
   // fold the block carry into the last thread: s(start of thread 255) = L + m^kPer K
   fr K = carry ? fr_load(carry + blockIdx.x) : fr_zero();
   if (t == kScanThreads - 1) h = fr_add(h, mulp(K, fr_load(pw + kLgPer)));
-  fr v = block_suffix(h, l, pw);  // s at the start of this thread's run
+  fr v = block_suffix<LG>(h, l, pw);  // s at the start of this thread's run
   // carry-in at this thread's end = s at the start of thread t+1
   fr cin = t + 1 < kScanThreads ? lds_get(l, t + 1) : K;
   if (total && blockIdx.x == 0 && t == 0) fr_store(total, v);
@@ -284,63 +308,74 @@ hipError_t fill_zero(fr* p, uint64_t n, hipStream_t s) {
   return hipGetLastError();
 }
 
-// pw[j] = m^(2^j) R' mod r, j = 0..12; m = neg ? -(*src) : *src.  32 m (five
-// doublings) is m R' in the stored form; squares stay in that form.
-__global__ void sfx_powers_kernel(const fr* src, int neg, fr* pw) {
+// pw[j] = m^(2^j) R' mod r, j < count; m = neg ? -(*src) : *src.  32 m (five
+// doublings) is m R' in the stored form; squares stay in that form.  Level L
+// of a scan (values combined 4096^L at a time) uses pw + 12 L: its multiplier
+// is m^(4096^L), whose powers of two continue the same squaring chain.
+__global__ void sfx_powers_kernel(const fr* src, int neg, fr* pw, int count) {
   if (threadIdx.x != 0) return;
   fr m = fr_load(src);
   if (neg) m = fr_neg(m);
   for (int d = 0; d < 5; d++) m = fr_add(m, m);
-  for (int j = 0; j <= kLgSeg; j++) {
+  for (int j = 0; j < count; j++) {
     fr_store(pw + j, m);
-    m = mulp(m, m);
-  }
-}
-
-// pw_next[j] = pw[kLgSeg]^(2^j): powers of m^kSeg for the next level (R' form)
-__global__ void sfx_next_powers_kernel(const fr* pw, fr* pw_next) {
-  if (threadIdx.x != 0) return;
-  fr m = fr_load(pw + kLgSeg);
-  for (int j = 0; j <= kLgSeg; j++) {
-    fr_store(pw_next + j, m);
     m = mulp(m, m);
   }
 }
 
 constexpr int kPw = kLgSeg + 1;
 
-static uint64_t sfx_scratch_elems(uint64_t n) {
-  uint64_t tot = kPw;
+// levels of a suffix scan over n values, and the power-table entries they use
+static int sfx_levels(uint64_t n) {
+  int L = 1;
   while (n > (uint64_t)kSeg) {
     n = (n + kSeg - 1) / kSeg;
-    tot += 2 * n + kPw;
+    L++;
+  }
+  return L;
+}
+static int sfx_pw_count(uint64_t n) { return kPw + kLgSeg * (sfx_levels(n) - 1); }
+
+static uint64_t sfx_scratch_elems(uint64_t n) {
+  uint64_t tot = (uint64_t)sfx_pw_count(n);
+  while (n > (uint64_t)kSeg) {
+    n = (n + kSeg - 1) / kSeg;
+    tot += 2 * n;
   }
   return tot;
 }
 
-// suffix scan with powers table pw (level's m): out[i] = s_{i+1} (if out),
-// *total = s_0 (if total).  out == nullptr: reduction only.
+// suffix scan with powers table pw (this level's m): out[i] = s_{i+1} (if out),
+// *total = s_0 (if total).  out == nullptr: reduction only.  A top level of at
+// most 256 values runs one element per thread.
 static hipError_t suffix_scan(const fr* a, fr* out, uint64_t n, const fr* pw, fr* total, fr* scratch, hipStream_t s) {
   if (n <= (uint64_t)kSeg) {
+    const bool tiny = n <= (uint64_t)kScanThreads;
     if (out) {
-      hipLaunchKernelGGL(sfx_down_kernel, dim3(1), dim3(kScanThreads), 0, s, a, out, n, pw, (const fr*)nullptr, total);
+      if (tiny)
+        hipLaunchKernelGGL(sfx_down_kernel<0>, dim3(1), dim3(kScanThreads), 0, s, a, out, n, pw, (const fr*)nullptr,
+                           total);
+      else
+        hipLaunchKernelGGL(sfx_down_kernel<kLgPer>, dim3(1), dim3(kScanThreads), 0, s, a, out, n, pw,
+                           (const fr*)nullptr, total);
     } else {
       // one block: the reduce kernel's tree value is s_0
-      hipLaunchKernelGGL(sfx_reduce_kernel, dim3(1), dim3(kScanThreads), 0, s, a, n, pw, total);
+      if (tiny)
+        hipLaunchKernelGGL(sfx_reduce_kernel<0>, dim3(1), dim3(kScanThreads), 0, s, a, n, pw, total);
+      else
+        hipLaunchKernelGGL(sfx_reduce_kernel<kLgPer>, dim3(1), dim3(kScanThreads), 0, s, a, n, pw, total);
     }
     return hipGetLastError();
   }
   const uint64_t nb = (n + kSeg - 1) / kSeg;
   fr* part = scratch;
   fr* carry = scratch + nb;
-  fr* pw_next = scratch + 2 * nb;
-  hipLaunchKernelGGL(sfx_reduce_kernel, dim3(sgrid(n)), dim3(kScanThreads), 0, s, a, n, pw, part);
-  hipLaunchKernelGGL(sfx_next_powers_kernel, dim3(1), dim3(64), 0, s, pw, pw_next);
+  hipLaunchKernelGGL(sfx_reduce_kernel<kLgPer>, dim3(sgrid(n)), dim3(kScanThreads), 0, s, a, n, pw, part);
   // carry[b] = s at the end of block b = sum_{b' > b} part[b'] (m^4096)^(b'-b-1)
-  hipError_t e = suffix_scan(part, out ? carry : nullptr, nb, pw_next, total, scratch + 2 * nb + kPw, s);
+  hipError_t e = suffix_scan(part, out ? carry : nullptr, nb, pw + kLgSeg, total, scratch + 2 * nb, s);
   if (e != hipSuccess || !out) return e;
-  hipLaunchKernelGGL(sfx_down_kernel, dim3(sgrid(n)), dim3(kScanThreads), 0, s, a, out, n, pw, (const fr*)carry,
-                     (fr*)nullptr);
+  hipLaunchKernelGGL(sfx_down_kernel<kLgPer>, dim3(sgrid(n)), dim3(kScanThreads), 0, s, a, out, n, pw,
+                     (const fr*)carry, (fr*)nullptr);
   return hipGetLastError();
 }
 
@@ -359,22 +394,25 @@ hipError_t grand_product(fr* inout, fr* incl_out, uint64_t n, fr* scratch, hipSt
 
 hipError_t div_linear(fr* inout, uint64_t n, const fr* c, fr* rem, fr* scratch, hipStream_t s) {
   fr* pw = scratch;
-  hipLaunchKernelGGL(sfx_powers_kernel, dim3(1), dim3(64), 0, s, c, 1, pw);
-  return suffix_scan(inout, inout, n, pw, rem, scratch + kPw, s);
+  const int np = sfx_pw_count(n);
+  hipLaunchKernelGGL(sfx_powers_kernel, dim3(1), dim3(64), 0, s, c, 1, pw, np);
+  return suffix_scan(inout, inout, n, pw, rem, scratch + np, s);
 }
 
 hipError_t evaluate(const fr* coeffs, uint64_t n, const fr* x, fr* y, fr* scratch, hipStream_t s) {
   fr* pw = scratch;
-  hipLaunchKernelGGL(sfx_powers_kernel, dim3(1), dim3(64), 0, s, x, 0, pw);
-  return suffix_scan(coeffs, nullptr, n, pw, y, scratch + kPw, s);
+  const int np = sfx_pw_count(n);
+  hipLaunchKernelGGL(sfx_powers_kernel, dim3(1), dim3(64), 0, s, x, 0, pw, np);
+  return suffix_scan(coeffs, nullptr, n, pw, y, scratch + np, s);
 }
 
 // one scan for an evaluation + division pair: q(X) = (p(X) - p(z)) / (X - z)
 // (q[n-1] = 0) and y = p(z); the suffix s_i = a_i + z s_{i+1} gives both
 hipError_t eval_div(const fr* in, fr* q, uint64_t n, const fr* z, fr* y, fr* scratch, hipStream_t s) {
   fr* pw = scratch;
-  hipLaunchKernelGGL(sfx_powers_kernel, dim3(1), dim3(64), 0, s, z, 0, pw);
-  return suffix_scan(in, q, n, pw, y, scratch + kPw, s);
+  const int np = sfx_pw_count(n);
+  hipLaunchKernelGGL(sfx_powers_kernel, dim3(1), dim3(64), 0, s, z, 0, pw, np);
+  return suffix_scan(in, q, n, pw, y, scratch + np, s);
 }
 
 }  // namespace pl
