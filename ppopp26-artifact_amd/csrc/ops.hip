@@ -25,7 +25,6 @@ constexpr int kOpThreads = 256;
 constexpr int kOpLgPer = 3;
 constexpr int kOpPer = 1 << kOpLgPer;
 constexpr int kOpSeg = kOpThreads * kOpPer;
-constexpr int kOpLgSeg = 8 + kOpLgPer;
 
 // R^2 and R^3 mod r (R = 2^256)
 __host__ __device__ constexpr uint32_t rr_limb(int i) {
@@ -166,8 +165,34 @@ __device__ fr block_scan_excl(const fr& v, uint4 (*l)[kOpThreads], bool suffix, 
   return r;
 }
 
-// Phase 1: total[b] = product over block b of in[i] (zeros taken as one)
+// The same exclusive product by Hillis-Steele steps: 8 levels of products
+// instead of 16 (more products in all, a shorter chain), for a single-block
+// top level where the latency is the cost.
+__device__ fr block_scan_excl_short(fr v, uint4 (*l)[kOpThreads], bool suffix, const fr one) {
+  const int t = threadIdx.x;
+  lds_put2(l, t, v);
+  __syncthreads();
+  for (int s = 1; s < kOpThreads; s <<= 1) {
+    const int o = suffix ? t + s : t - s;
+    const bool has = suffix ? o < kOpThreads : o >= 0;
+    const fr other = has ? lds_get2(l, o) : one;
+    __syncthreads();
+    if (has) v = mul_pbd(v, other);
+    lds_put2(l, t, v);
+    __syncthreads();
+  }
+  const int e = suffix ? t + 1 : t - 1;
+  const fr ex = (suffix ? e < kOpThreads : e >= 0) ? lds_get2(l, e) : one;
+  __syncthreads();
+  return ex;
+}
+
+// Phase 1: total[b] = product over block b of in[i] (zeros taken as one).
+// Blocks of 256 x 2^LG elements: LG = kOpLgPer in bulk levels; a single-block
+// top level takes the smallest LG that covers it.
+template <int LG>
 __global__ void __launch_bounds__(kOpThreads) inv_reduce_kernel(const fr* in, uint64_t n, fr* total) {
+  constexpr int kOpPer = 1 << LG, kOpLgPer = LG, kOpLgSeg = 8 + LG;
   __shared__ uint4 l[2][kOpThreads];
   // order-free product: thread t takes elements b*kOpSeg + 256 k + t (coalesced)
   const uint64_t b0 = (uint64_t)blockIdx.x << kOpLgSeg;
@@ -181,22 +206,27 @@ __global__ void __launch_bounds__(kOpThreads) inv_reduce_kernel(const fr* in, ui
   // pairwise tree in registers: eight products of unpacked words, then the
   // products of products (all limbs normalized digits, so any pair is a valid
   // operand pair)
-  v9 p[kOpPer / 2];
-  static_for<0, kOpPer / 2>([&](auto k_) {
-    constexpr int k = decltype(k_)::value;
-    v9 x, y;
-    unpack_ext(v[k].l, x.l);
-    unpack_ext(v[k + kOpPer / 2].l, y.l);
-    p[k] = v9_mul<false, false, false>(x, y.l);
-  });
-  static_for<0, kOpLgPer - 1>([&](auto s_) {
-    constexpr int h = kOpPer / 2 >> (decltype(s_)::value + 1);
-    static_for<0, h>([&](auto k_) {
+  fr acc;
+  if constexpr (kOpPer == 1) {
+    acc = v[0];
+  } else {
+    v9 p[kOpPer / 2];
+    static_for<0, kOpPer / 2>([&](auto k_) {
       constexpr int k = decltype(k_)::value;
-      p[k] = v9_mul<false, false, false>(p[k], p[k + h].l);
+      v9 x, y;
+      unpack_ext(v[k].l, x.l);
+      unpack_ext(v[k + kOpPer / 2].l, y.l);
+      p[k] = v9_mul<false, false, false>(x, y.l);
     });
-  });
-  fr acc = v9_canonical(p[0]);
+    static_for<0, kOpLgPer - 1>([&](auto s_) {
+      constexpr int h = kOpPer / 2 >> (decltype(s_)::value + 1);
+      static_for<0, h>([&](auto k_) {
+        constexpr int k = decltype(k_)::value;
+        p[k] = v9_mul<false, false, false>(p[k], p[k + h].l);
+      });
+    });
+    acc = v9_canonical(p[0]);
+  }
   lds_put2(l, threadIdx.x, acc);
   __syncthreads();
   for (int s = kOpThreads / 2; s > 0; s >>= 1) {
@@ -213,8 +243,11 @@ __global__ void __launch_bounds__(kOpThreads) inv_reduce_kernel(const fr* in, ui
 // in[i] (num == nullptr: one); zero inputs give zero.  out may equal in or num.
 // Local prefixes L_k are kept as pbd words (products with balanced digits; a
 // product of two values below 1.3 r lies in (-r, 0.03 r), inside a pbd word).
+// SHORT: Hillis-Steele thread scans (single-block top level)
+template <int LG, bool SHORT>
 __global__ void __launch_bounds__(kOpThreads) inv_down_kernel(const fr* in, const fr* num, fr* out, uint64_t n,
                                                                const fr* binv) {
+  constexpr int kOpPer = 1 << LG, kOpLgSeg = 8 + LG;
   __shared__ uint4 l[2][kOpThreads];
   const int t = threadIdx.x;
   const uint64_t base = ((uint64_t)blockIdx.x << kOpLgSeg) + (uint64_t)t * kOpPer;
@@ -249,8 +282,8 @@ __global__ void __launch_bounds__(kOpThreads) inv_down_kernel(const fr* in, cons
     const fr o = rp_one();
     to_pbd(o.l, one.l);
   }
-  const fr suf = block_scan_excl(tp, l, true, one);   // product of threads > t
-  const fr pre = block_scan_excl(tp, l, false, one);  // product of threads < t
+  const fr suf = SHORT ? block_scan_excl_short(tp, l, true, one) : block_scan_excl(tp, l, true, one);
+  const fr pre = SHORT ? block_scan_excl_short(tp, l, false, one) : block_scan_excl(tp, l, false, one);
   // inv(L_last) = binv * suffix(> t) * prefix(< t): the block product over
   // everything outside this thread's run, times the block's inverse
   v9 cur;
@@ -325,16 +358,34 @@ static hipError_t batch_inverse_level(const fr* in, const fr* num, fr* out, uint
   const uint64_t nb = (n + kOpSeg - 1) / kOpSeg;
   fr* tot = scratch;
   fr* tinv = scratch + nb;
-  hipLaunchKernelGGL(inv_reduce_kernel, dim3((unsigned)nb), dim3(kOpThreads), 0, s, in, n, tot);
   if (nb == 1) {
-    hipLaunchKernelGGL(inv_single_kernel, dim3(1), dim3(64), 0, s, tot, with_num);
-    hipLaunchKernelGGL(inv_down_kernel, dim3((unsigned)nb), dim3(kOpThreads), 0, s, in, num, out, n, (const fr*)tot);
+    // one block: 256 x 2^LG with the smallest LG that covers n, short scans
+    const dim3 g(1), b(kOpThreads);
+    if (n <= 256) {
+      hipLaunchKernelGGL(inv_reduce_kernel<0>, g, b, 0, s, in, n, tot);
+      hipLaunchKernelGGL(inv_single_kernel, dim3(1), dim3(64), 0, s, tot, with_num);
+      hipLaunchKernelGGL((inv_down_kernel<0, true>), g, b, 0, s, in, num, out, n, (const fr*)tot);
+    } else if (n <= 512) {
+      hipLaunchKernelGGL(inv_reduce_kernel<1>, g, b, 0, s, in, n, tot);
+      hipLaunchKernelGGL(inv_single_kernel, dim3(1), dim3(64), 0, s, tot, with_num);
+      hipLaunchKernelGGL((inv_down_kernel<1, true>), g, b, 0, s, in, num, out, n, (const fr*)tot);
+    } else if (n <= 1024) {
+      hipLaunchKernelGGL(inv_reduce_kernel<2>, g, b, 0, s, in, n, tot);
+      hipLaunchKernelGGL(inv_single_kernel, dim3(1), dim3(64), 0, s, tot, with_num);
+      hipLaunchKernelGGL((inv_down_kernel<2, true>), g, b, 0, s, in, num, out, n, (const fr*)tot);
+    } else {
+      hipLaunchKernelGGL(inv_reduce_kernel<kOpLgPer>, g, b, 0, s, in, n, tot);
+      hipLaunchKernelGGL(inv_single_kernel, dim3(1), dim3(64), 0, s, tot, with_num);
+      hipLaunchKernelGGL((inv_down_kernel<kOpLgPer, true>), g, b, 0, s, in, num, out, n, (const fr*)tot);
+    }
     return hipGetLastError();
   }
+  hipLaunchKernelGGL(inv_reduce_kernel<kOpLgPer>, dim3((unsigned)nb), dim3(kOpThreads), 0, s, in, n, tot);
   // block totals are never zero (zeros were skipped): invert them as a batch
   hipError_t e = batch_inverse_level(tot, nullptr, tinv, nb, scratch + 2 * nb, with_num, s);
   if (e != hipSuccess) return e;
-  hipLaunchKernelGGL(inv_down_kernel, dim3((unsigned)nb), dim3(kOpThreads), 0, s, in, num, out, n, (const fr*)tinv);
+  hipLaunchKernelGGL((inv_down_kernel<kOpLgPer, false>), dim3((unsigned)nb), dim3(kOpThreads), 0, s, in, num, out, n,
+                     (const fr*)tinv);
   return hipGetLastError();
 }
 
