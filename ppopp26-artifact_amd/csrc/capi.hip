@@ -290,17 +290,25 @@ int pl_vec_op(int op, const void* d_a, const void* d_b, void* d_out, uint64_t n,
   return e == hipSuccess ? PL_OK : hip_fail(e);
 }
 
-int pl_perm_ratio(const void* const* d_w, const void* const* d_s, const void* d_beta, const void* d_gamma,
-                  const void* h_k, int lg, const pl_domain* d, void* d_ratio, void* d_work, pl_stream_t stream) {
-  if (!d_w || !d_s || !d_beta || !d_gamma || !h_k || !d || !d_ratio || d->d.lg != lg) return PL_ERR_ARG;
+// shared argument checks of the permutation entry points
+static int perm_args(const void* const* d_w, const void* const* d_s, const void* d_beta, const void* d_gamma,
+                     const void* h_k, int lg, const pl_domain* d, void* d_out, void* d_work, pl::host::fr64 k[3]) {
+  if (!d_w || !d_s || !d_beta || !d_gamma || !h_k || !d || !d_out || d->d.lg != lg) return PL_ERR_ARG;
   for (int j = 0; j < 3; j++)
     if (!d_w[j] || !d_s[j] || !aligned16(d_w[j]) || !aligned16(d_s[j])) return PL_ERR_ARG;
-  if (!aligned16(d_beta) || !aligned16(d_gamma) || !aligned16(d_ratio) || !aligned16(d_work)) return PL_ERR_ARG;
-  const uint64_t n = 1ull << lg;
-  pl::host::fr64 k[3];
-  memcpy(k, h_k, sizeof(k));
+  if (!aligned16(d_beta) || !aligned16(d_gamma) || !aligned16(d_out) || !aligned16(d_work)) return PL_ERR_ARG;
+  memcpy(k, h_k, 3 * sizeof(pl::host::fr64));
   for (int j = 0; j < 3; j++)
     if (pl::host::geq_p(k[j].l)) return PL_ERR_ARG;
+  return PL_OK;
+}
+
+int pl_perm_ratio(const void* const* d_w, const void* const* d_s, const void* d_beta, const void* d_gamma,
+                  const void* h_k, int lg, const pl_domain* d, void* d_ratio, void* d_work, pl_stream_t stream) {
+  pl::host::fr64 k[3];
+  const int st = perm_args(d_w, d_s, d_beta, d_gamma, h_k, lg, d, d_ratio, d_work, k);
+  if (st != PL_OK) return st;
+  const uint64_t n = 1ull << lg;
   Scratch sc;
   hipError_t e = sc.get(d_work, pl_vec_work_bytes(n), (hipStream_t)stream);
   if (e != hipSuccess) return hip_fail(e);
@@ -314,9 +322,18 @@ int pl_perm_ratio(const void* const* d_w, const void* const* d_s, const void* d_
 int pl_perm_grand_product(const void* const* d_w, const void* const* d_s, const void* d_beta,
                           const void* d_gamma, const void* h_k, int lg, const pl_domain* d, void* d_z,
                           void* d_work, pl_stream_t stream) {
-  int st = pl_perm_ratio(d_w, d_s, d_beta, d_gamma, h_k, lg, d, d_z, d_work, stream);
+  pl::host::fr64 k[3];
+  const int st = perm_args(d_w, d_s, d_beta, d_gamma, h_k, lg, d, d_z, d_work, k);
   if (st != PL_OK) return st;
-  return pl_grand_product(d_z, 1ull << lg, nullptr, d_work, stream);
+  const uint64_t n = 1ull << lg;
+  Scratch sc;
+  hipError_t e = sc.get(d_work, pl_vec_work_bytes(n), (hipStream_t)stream);
+  if (e != hipSuccess) return hip_fail(e);
+  const pl::fr* w[3] = {(const pl::fr*)d_w[0], (const pl::fr*)d_w[1], (const pl::fr*)d_w[2]};
+  const pl::fr* sg[3] = {(const pl::fr*)d_s[0], (const pl::fr*)d_s[1], (const pl::fr*)d_s[2]};
+  e = pl::perm_grand_product(w, sg, (const pl::fr*)d_beta, (const pl::fr*)d_gamma, d->d.pw0, d->d.pw1, k,
+                             (pl::fr*)d_z, n, sc.p, (hipStream_t)stream);
+  return e == hipSuccess ? PL_OK : hip_fail(e);
 }
 
 size_t pl_uniform_work_bytes(uint64_t n) { return (size_t)pl::uniform_work_bytes(n); }

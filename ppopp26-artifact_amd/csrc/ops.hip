@@ -187,6 +187,32 @@ __device__ fr block_scan_excl_short(fr v, uint4 (*l)[kOpThreads], bool suffix, c
   return ex;
 }
 
+// The exclusive prefix product of a and the exclusive suffix product of b
+// over the 256 thread values (pbd words), by Hillis-Steele steps with the two
+// scans interleaved: 8 levels of two independent products, every thread busy
+// (a missing partner multiplies by one).
+__device__ void block_scan2_short(fr& a, fr& b, uint4 (*la)[kOpThreads], uint4 (*lb)[kOpThreads], const fr one) {
+  const int t = threadIdx.x;
+  lds_put2(la, t, a);
+  lds_put2(lb, t, b);
+  __syncthreads();
+  for (int s = 1; s < kOpThreads; s <<= 1) {
+    const fr oa = t >= s ? lds_get2(la, t - s) : one;
+    const fr ob = t + s < kOpThreads ? lds_get2(lb, t + s) : one;
+    __syncthreads();
+    a = mul_pbd(a, oa);
+    b = mul_pbd(b, ob);
+    lds_put2(la, t, a);
+    lds_put2(lb, t, b);
+    __syncthreads();
+  }
+  const fr ea = t > 0 ? lds_get2(la, t - 1) : one;
+  const fr eb = t + 1 < kOpThreads ? lds_get2(lb, t + 1) : one;
+  __syncthreads();
+  a = ea;
+  b = eb;
+}
+
 // Phase 1: total[b] = product over block b of in[i] (zeros taken as one).
 // Blocks of 256 x 2^LG elements: LG = kOpLgPer in bulk levels; a single-block
 // top level takes the smallest LG that covers it.
@@ -420,39 +446,351 @@ struct RatioArgs {
 // permutation ratio.  Limbs: the inputs are balanced digits (<= 2^28); the
 // three-term sums <= 3 2^28 (products with balanced outputs), so a product of
 // two of them stays below 81 2^56 + 2^61 < 2^62.8 per column.
-__global__ void __launch_bounds__(256) perm_terms_kernel(RatioArgs a) {
-  uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
-  if (i >= a.n) return;
+// per-thread constants of the terms: B = beta R' (= 32 S(beta)), g = gamma
+struct TermConsts {
+  uint32_t B[9];
+  v9 g;
+};
+__device__ __forceinline__ TermConsts term_consts(const RatioArgs& a) {
+  TermConsts c;
   fr b32 = fr_load(a.beta);
 #pragma unroll
-  for (int d = 0; d < 5; d++) b32 = fr_add(b32, b32);  // beta R' = 32 S(beta)
-  uint32_t B[9];
-  unpack_ext(b32.l, B);
-  v9 g, w0, w1;
-  unpack_ext(fr_load(a.gamma).l, g.l);
+  for (int d = 0; d < 5; d++) b32 = fr_add(b32, b32);
+  unpack_ext(b32.l, c.B);
+  unpack_ext(fr_load(a.gamma).l, c.g.l);
+  return c;
+}
+
+// num and den of element i (before reduction)
+__device__ __forceinline__ void perm_terms_at(const RatioArgs& a, const TermConsts& c, uint64_t i, v9& num,
+                                              v9& den) {
+  v9 w0, w1;
   unpack_ext(fr_load(a.w0 + (i & 16383)).l, w0.l);
   unpack_ext(fr_load(a.w1 + (i >> 14)).l, w1.l);
   const v9 wi = v9_mul<false, false, false>(w0, w1.l);
-  const v9 bx = v9_mul<false, false, false>(wi, B);
-  v9 num, den;
+  const v9 bx = v9_mul<false, false, false>(wi, c.B);
 #pragma unroll
   for (int j = 0; j < 3; j++) {
     v9 wj, sj;
     unpack_ext(fr_load(a.w[j] + i).l, wj.l);
-    const v9 wg = v9_add(wj, g);
+    const v9 wg = v9_add(wj, c.g);
     uint32_t K[9];
     pbd_unpack(a.k29[j].l, K);
     const v9 nj = v9_add(wg, v9_mul<false, false, true>(bx, K));
     unpack_ext(fr_load(a.s[j] + i).l, sj.l);
-    const v9 dj = v9_add(wg, v9_mul<false, false, true>(sj, B));
+    const v9 dj = v9_add(wg, v9_mul<false, false, true>(sj, c.B));
     num = j == 0 ? nj : v9_mul<false, false, false>(num, nj.l);  // the first factor needs no product
     den = j == 0 ? dj : v9_mul<false, false, false>(den, dj.l);
   }
-  fr_store(a.num + i, v9_canonical(num));
-  fr_store(a.den + i, v9_canonical(den));
 }
 
-uint64_t ops_scratch_elems(uint64_t n) { return n + inv_scratch_elems(n); }
+// ZERO_DEN (grand product): a zero den is stored as num = 0, den = R' mod r,
+// and the products of num and den over each run of 8 consecutive elements go
+// to rn/rd: three butterfly steps across the run's lanes, each one product
+// per lane (step 1 pairs num on even lanes with den on odd lanes; after steps
+// 2 and 3, lane 8g holds the run's num product and lane 8g + 1 its den
+// product).  Lanes past n carry R' mod r.
+template <bool ZERO_DEN>
+__global__ void __launch_bounds__(256) perm_terms_kernel(RatioArgs a, fr* rn, fr* rd) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if constexpr (!ZERO_DEN) {
+    if (i >= a.n) return;
+    const TermConsts c = term_consts(a);
+    v9 num, den;
+    perm_terms_at(a, c, i, num, den);
+    fr_store(a.num + i, v9_canonical(num));
+    fr_store(a.den + i, v9_canonical(den));
+  } else {
+    fr nw = rp_one(), dw = rp_one();
+    if (i < a.n) {
+      const TermConsts c = term_consts(a);
+      v9 num, den;
+      perm_terms_at(a, c, i, num, den);
+      nw = v9_canonical(num);
+      dw = v9_canonical(den);
+      if (fr_is_zero(dw)) {
+        nw = fr_zero();
+        dw = rp_one();
+      }
+      fr_store(a.num + i, nw);
+      fr_store(a.den + i, dw);
+    }
+    const int lane = threadIdx.x & 7;
+    v9 p;
+    {
+      const fr send = (lane & 1) ? nw : dw;
+      v9 mine;
+      unpack_ext(((lane & 1) ? dw : nw).l, mine.l);
+      fr o;
+#pragma unroll
+      for (int j = 0; j < 8; j++) o.l[j] = (uint32_t)__shfl_xor((int)send.l[j], 1);
+      uint32_t other[9];
+      unpack_ext(o.l, other);
+      p = v9_mul<false, false, false>(mine, other);
+    }
+#pragma unroll
+    for (int m = 2; m <= 4; m <<= 1) {
+      uint32_t q[9];
+#pragma unroll
+      for (int j = 0; j < 9; j++) q[j] = (uint32_t)__shfl_xor((int)p.l[j], m);
+      p = v9_mul<false, false, false>(p, q);
+    }
+    if ((i & ~(uint64_t)7) < a.n && lane < 2) fr_store((lane ? rd : rn) + (i >> 3), v9_canonical(p));
+  }
+}
+
+// ---- permutation grand product without a ratio vector -----------------------
+// z_i = prod_{j<i} num_j / den_j (exclusive, z_0 = 1) as
+//   z_i = [prod_{j<i} num_j] [prod_{j>=i} den_j] / prod_j den_j,
+// so the one field inversion multiplies carries only and no element needs an
+// inverse of its own.  Up: per block of 2048 elements (256 threads x 8
+// consecutive) the thread products of num and den (kept for the down-sweep)
+// and the block products, recursively over the block products until one
+// block holds them.  Top (one block): the thread scans, the inversion of the
+// total den product, and the carries c_b = [prod_{b'<b} num] [prod_{b'>b} den]
+// / total for the level below.  Down: each level rescans its blocks from their
+// carries; level 0 takes its den suffix inclusive, upper levels exclusive.
+// A zero den_j (ratio 0 under x/0 = 0) is stored as num_j = 0, den_j = 1,
+// which gives the same z.
+// Words are read in the R'-view (the batch-inversion notes above): num_j and
+// den_j are stored words S(x), i.e. R'-view words of x / 32; the 32s cancel
+// between the two products and the total but for one, which the inverse
+// absorbs (32 fr_inv_single(W) is the R'-view word of 1 / (32 W/R')), so level
+// 0 yields the stored words S(z_i).
+
+// Up-sweep over one level: thread products of num and den over each
+// thread's 8 consecutive elements (tn, td: one per thread of every block, for
+// the down-sweep) and the block products (bn, bd).
+__global__ void __launch_bounds__(kOpThreads) gp_up_kernel(const fr* nv, const fr* dv, uint64_t n, fr* tn, fr* td,
+                                                            fr* bn, fr* bd) {
+  __shared__ uint4 ln[2][kOpThreads], ld[2][kOpThreads];
+  const int t = threadIdx.x;
+  const uint64_t tid = (uint64_t)blockIdx.x * kOpThreads + t;
+  const uint64_t base = tid * kOpPer;
+  v9 pn, pd;
+  static_for<0, kOpPer>([&](auto k_) {
+    constexpr int k = decltype(k_)::value;
+    const bool live = base + k < n;
+    const fr nw = live ? fr_load(nv + base + k) : rp_one();
+    const fr dw = live ? fr_load(dv + base + k) : rp_one();
+    v9 x, y;
+    unpack_ext(nw.l, x.l);
+    unpack_ext(dw.l, y.l);
+    pn = k == 0 ? x : v9_mul<false, false, false>(pn, x.l);
+    pd = k == 0 ? y : v9_mul<false, false, false>(pd, y.l);
+  });
+  fr cn = v9_canonical(pn), cd = v9_canonical(pd);
+  fr_store(tn + tid, cn);
+  fr_store(td + tid, cd);
+  lds_put2(ln, t, cn);
+  lds_put2(ld, t, cd);
+  __syncthreads();
+  for (int s = kOpThreads / 2; s > 0; s >>= 1) {
+    if (t < s) {
+      cn = mul_rp(cn, lds_get2(ln, t + s));
+      cd = mul_rp(cd, lds_get2(ld, t + s));
+      lds_put2(ln, t, cn);
+      lds_put2(ld, t, cd);
+    }
+    __syncthreads();
+  }
+  if (t == 0) {
+    fr_store(bn + blockIdx.x, cn);
+    fr_store(bd + blockIdx.x, cd);
+  }
+}
+
+// Down-sweep over one level (in place: out may be nv).  Thread t of block b
+// owns elements base = (256 b + t) 2^LG + k, k < 2^LG, and writes
+//   out = c_b pre_t suf_t prod_{k'<k} num [prod_{k'>=k} den (INCL) or k' > k]
+// with pre_t / suf_t the products over the block's threads before / after t.
+// TOP: a single block; the thread products are formed here, short scans, and
+// c = 32 fr_inv_single(total den).  Otherwise c_b = carry[b] and the thread
+// products come from the up-sweep (tn, td).
+template <int LG, bool INCL, bool TOP>
+__device__ __forceinline__ void gp_down_body(const fr* nv, const fr* dv, const fr* tn, const fr* td, uint64_t n,
+                                             const fr* carry, fr* out) {
+  constexpr int kPer = 1 << LG;
+  __shared__ uint4 l[2][kOpThreads], l2[2][kOpThreads];
+  __shared__ uint32_t cinv[8];
+  const int t = threadIdx.x;
+  const uint64_t tid = (uint64_t)blockIdx.x * kOpThreads + t;
+  const uint64_t base = tid * kPer;
+  fr one;
+  {
+    const fr o = rp_one();
+    to_pbd(o.l, one.l);
+  }
+  // local den suffixes S[k] (pbd words; a product of a value below 1.07 r and
+  // one below r lies in (-1.07 r, 0.07 r))
+  fr S[kPer];
+  v9 acc;
+  static_for<0, kPer>([&](auto k_) {
+    constexpr int k = kPer - 1 - decltype(k_)::value;
+    const fr dw = base + k < n ? fr_load(dv + base + k) : rp_one();
+    v9 y;
+    unpack_ext(dw.l, y.l);
+    if constexpr (INCL) {
+      acc = k == kPer - 1 ? y : v9_mul<false, false, true>(acc, y.l);
+      pbd_pack(acc.l, S[k].l);
+    } else {
+      if constexpr (k == kPer - 1) {
+        S[k] = one;
+        acc = y;
+      } else {
+        pbd_pack(acc.l, S[k].l);
+        acc = v9_mul<false, false, true>(acc, y.l);
+      }
+    }
+  });
+  // acc: the thread's den product
+  fr tnw, tdw;
+  if constexpr (TOP) {
+    v9 pn;
+    static_for<0, kPer>([&](auto k_) {
+      constexpr int k = decltype(k_)::value;
+      const fr nw = base + k < n ? fr_load(nv + base + k) : rp_one();
+      v9 x;
+      unpack_ext(nw.l, x.l);
+      pn = k == 0 ? x : v9_mul<false, false, false>(pn, x.l);
+    });
+    pbd_pack(acc.l, tdw.l);
+    if constexpr (kPer == 1) {
+      pbd_pack(pn.l, tnw.l);  // balanced digits of a canonical value
+    } else {
+      uint32_t b[9];
+      reduce_centered(pn.l, b);
+      pbd_pack(b, tnw.l);
+    }
+  } else {
+    to_pbd(fr_load(tn + tid).l, tnw.l);
+    to_pbd(fr_load(td + tid).l, tdw.l);
+  }
+  fr pre = tnw, suf = tdw;
+  block_scan2_short(pre, suf, l, l2, one);
+  fr cw;
+  if constexpr (TOP) {
+    if (t == 0) {
+      // total den product = suf_0 x (thread 0's product)
+      v9 a;
+      uint32_t b[9];
+      pbd_unpack(suf.l, a.l);
+      pbd_unpack(tdw.l, b);
+      fr z = fr_inv_single(v9_canonical(v9_mul<false, false, false>(a, b)));
+#pragma unroll
+      for (int d = 0; d < 5; d++) z = fr_add(z, z);
+#pragma unroll
+      for (int i = 0; i < 8; i++) cinv[i] = z.l[i];
+    }
+    __syncthreads();
+#pragma unroll
+    for (int i = 0; i < 8; i++) cw.l[i] = cinv[i];
+  } else {
+    cw = fr_load(carry + blockIdx.x);
+  }
+  v9 cur;
+  {
+    v9 a;
+    uint32_t b[9], c[9];
+    unpack_ext(cw.l, a.l);
+    pbd_unpack(pre.l, b);
+    pbd_unpack(suf.l, c);
+    cur = v9_mul<false, false, false>(v9_mul<false, false, false>(a, b), c);
+  }
+  static_for<0, kPer>([&](auto k_) {
+    constexpr int k = decltype(k_)::value;
+    const bool live = base + k < n;
+    const fr nw = live ? fr_load(nv + base + k) : rp_one();  // read before out[] is written
+    if (live) {
+      uint32_t sk[9];
+      pbd_unpack(S[k].l, sk);
+      fr_store(out + base + k, v9_canonical(v9_mul<false, false, false>(cur, sk)));
+    }
+    if constexpr (k < kPer - 1) {
+      uint32_t x[9];
+      unpack_ext(nw.l, x);
+      cur = v9_mul<false, false, false>(cur, x);
+    }
+  });
+}
+
+// bulk levels: three waves per SIMD (the local den suffixes stay in registers)
+template <bool INCL>
+__global__ void __launch_bounds__(kOpThreads) __attribute__((amdgpu_waves_per_eu(3)))
+gp_down_kernel(const fr* nv, const fr* dv, const fr* tn, const fr* td, uint64_t n, const fr* carry, fr* out) {
+  gp_down_body<kOpLgPer, INCL, false>(nv, dv, tn, td, n, carry, out);
+}
+
+// the single-block top level
+template <int LG, bool INCL>
+__global__ void __launch_bounds__(kOpThreads) gp_top_kernel(const fr* nv, const fr* dv, uint64_t n, fr* out) {
+  gp_down_body<LG, INCL, true>(nv, dv, nullptr, nullptr, n, nullptr, out);
+}
+
+// Level 0: thread = run of 8 consecutive elements with carry c (the num
+// products of the runs before it, the den products of those after it, the
+// inverse): z_k = c prod_{k'<k} num prod_{k'>=k} den.  No scans.
+__global__ void __launch_bounds__(256) gp_run_down_kernel(const fr* nv, const fr* dv, uint64_t n, const fr* carry,
+                                                          fr* out) {
+  const uint64_t run = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  const uint64_t base = run * kOpPer;
+  if (base >= n) return;
+  // local den suffixes (pbd words; products of a value below 1.07 r and one
+  // below r lie in (-1.07 r, 0.07 r))
+  fr S[kOpPer];
+  v9 acc;
+  static_for<0, kOpPer>([&](auto k_) {
+    constexpr int k = kOpPer - 1 - decltype(k_)::value;
+    const fr dw = base + k < n ? fr_load(dv + base + k) : rp_one();
+    v9 y;
+    unpack_ext(dw.l, y.l);
+    acc = k == kOpPer - 1 ? y : v9_mul<false, false, true>(acc, y.l);
+    pbd_pack(acc.l, S[k].l);
+  });
+  v9 cur;
+  unpack_ext(fr_load(carry + run).l, cur.l);
+  static_for<0, kOpPer>([&](auto k_) {
+    constexpr int k = decltype(k_)::value;
+    const bool live = base + k < n;
+    const fr nw = live ? fr_load(nv + base + k) : rp_one();  // read before out[] is written
+    if (live) {
+      uint32_t sk[9];
+      pbd_unpack(S[k].l, sk);
+      fr_store(out + base + k, v9_canonical(v9_mul<false, false, false>(cur, sk)));
+    }
+    if constexpr (k < kOpPer - 1) {
+      uint32_t x[9];
+      unpack_ext(nw.l, x);
+      cur = v9_mul<false, false, false>(cur, x);
+    }
+  });
+}
+
+// level sizes of the grand product: the runs of 8 elements (n_1 = ceil(n / 8)),
+// then n_{L+1} = ceil(n_L / 2048) while n_L > 2048; scratch holds den (n), the
+// run products (2 n_1), then per up-swept level the thread products (2 x 256
+// per block) and the next level's num and den
+static uint64_t gp_scratch_elems(uint64_t n) {
+  uint64_t m = (n + kOpPer - 1) / kOpPer;
+  uint64_t tot = n + 2 * m;
+  while (m > (uint64_t)kOpSeg) {
+    const uint64_t nb = (m + kOpSeg - 1) / kOpSeg;
+    tot += 2 * nb * kOpThreads + 2 * nb;
+    m = nb;
+  }
+  return tot;
+}
+
+template <bool INCL>
+static void gp_top(const fr* nv, const fr* dv, uint64_t n, fr* out, hipStream_t s) {
+  const dim3 g(1), b(kOpThreads);
+  if (n <= 256) hipLaunchKernelGGL((gp_top_kernel<0, INCL>), g, b, 0, s, nv, dv, n, out);
+  else if (n <= 512) hipLaunchKernelGGL((gp_top_kernel<1, INCL>), g, b, 0, s, nv, dv, n, out);
+  else if (n <= 1024) hipLaunchKernelGGL((gp_top_kernel<2, INCL>), g, b, 0, s, nv, dv, n, out);
+  else hipLaunchKernelGGL((gp_top_kernel<kOpLgPer, INCL>), g, b, 0, s, nv, dv, n, out);
+}
+
+uint64_t ops_scratch_elems(uint64_t n) { return std::max(n + inv_scratch_elems(n), gp_scratch_elems(n)); }
 
 hipError_t vec_op(int op, const fr* a, const fr* b, fr* out, uint64_t n, fr* scratch, hipStream_t s) {
   if (n == 0) return hipSuccess;
@@ -471,9 +809,8 @@ hipError_t vec_op(int op, const fr* a, const fr* b, fr* out, uint64_t n, fr* scr
   return hipGetLastError();
 }
 
-hipError_t perm_ratio(const fr* const w[3], const fr* const sg[3], const fr* beta, const fr* gamma,
-                      const fr* w0, const fr* w1, const host::fr64 k[3], fr* ratio, uint64_t n, fr* scratch,
-                      hipStream_t s) {
+static RatioArgs ratio_args(const fr* const w[3], const fr* const sg[3], const fr* beta, const fr* gamma,
+                            const fr* w0, const fr* w1, const host::fr64 k[3], uint64_t n) {
   RatioArgs a;
   for (int j = 0; j < 3; j++) {
     a.w[j] = w[j];
@@ -489,12 +826,66 @@ hipError_t perm_ratio(const fr* const w[3], const fr* const sg[3], const fr* bet
   a.gamma = gamma;
   a.w0 = w0;
   a.w1 = w1;
+  a.num = nullptr;
+  a.den = nullptr;
+  a.n = n;
+  return a;
+}
+
+hipError_t perm_ratio(const fr* const w[3], const fr* const sg[3], const fr* beta, const fr* gamma,
+                      const fr* w0, const fr* w1, const host::fr64 k[3], fr* ratio, uint64_t n, fr* scratch,
+                      hipStream_t s) {
+  RatioArgs a = ratio_args(w, sg, beta, gamma, w0, w1, k, n);
   a.num = ratio;
   a.den = scratch;
-  a.n = n;
-  hipLaunchKernelGGL(perm_terms_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a);
+  hipLaunchKernelGGL(perm_terms_kernel<false>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, nullptr,
+                     nullptr);
   // ratio = num / den in place over num
   return batch_inverse(scratch, ratio, ratio, n, scratch + n, s);
+}
+
+hipError_t perm_grand_product(const fr* const w[3], const fr* const sg[3], const fr* beta, const fr* gamma,
+                              const fr* w0, const fr* w1, const host::fr64 k[3], fr* z, uint64_t n, fr* scratch,
+                              hipStream_t s) {
+  RatioArgs a = ratio_args(w, sg, beta, gamma, w0, w1, k, n);
+  a.num = z;
+  a.den = scratch;
+  const uint64_t nr = (n + kOpPer - 1) / kOpPer;
+  fr* rn = scratch + n;
+  fr* rd = rn + nr;
+  hipLaunchKernelGGL(perm_terms_kernel<true>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, rn, rd);
+  // levels above the elements: the runs, then block products
+  struct Level {
+    fr *nv, *dv, *tn, *td;
+    uint64_t n;
+  } lv[8];
+  int top = 0;
+  lv[0] = {rn, rd, nullptr, nullptr, nr};
+  fr* p = rd + nr;
+  while (lv[top].n > (uint64_t)kOpSeg) {
+    Level& c = lv[top];
+    const uint64_t nb = (c.n + kOpSeg - 1) / kOpSeg;
+    c.tn = p;
+    c.td = p + nb * kOpThreads;
+    p += 2 * nb * kOpThreads;
+    fr* bn = p;
+    fr* bd = p + nb;
+    p += 2 * nb;
+    hipLaunchKernelGGL(gp_up_kernel, dim3((unsigned)nb), dim3(kOpThreads), 0, s, (const fr*)c.nv, (const fr*)c.dv,
+                       c.n, c.tn, c.td, bn, bd);
+    lv[++top] = {bn, bd, nullptr, nullptr, nb};
+  }
+  // carries in place over each level's num (the runs' carries over rn)
+  gp_top<false>(lv[top].nv, lv[top].dv, lv[top].n, lv[top].nv, s);
+  for (int L = top - 1; L >= 0; L--) {
+    const Level& c = lv[L];
+    const unsigned nb = (unsigned)((c.n + kOpSeg - 1) / kOpSeg);
+    hipLaunchKernelGGL(gp_down_kernel<false>, dim3(nb), dim3(kOpThreads), 0, s, c.nv, c.dv, c.tn, c.td, c.n,
+                       (const fr*)lv[L + 1].nv, c.nv);
+  }
+  hipLaunchKernelGGL(gp_run_down_kernel, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s, (const fr*)z,
+                     (const fr*)scratch, n, (const fr*)rn, z);
+  return hipGetLastError();
 }
 
 }  // namespace pl

@@ -106,6 +106,10 @@ hipError_t vec_op(int op, const fr* a, const fr* b, fr* out, uint64_t n, fr* scr
 hipError_t perm_ratio(const fr* const w[3], const fr* const sg[3], const fr* beta, const fr* gamma,
                       const fr* w0, const fr* w1, const host::fr64 k[3], fr* ratio, uint64_t n, fr* scratch,
                       hipStream_t s);
+// z = the exclusive grand product of the permutation ratio (no ratio vector)
+hipError_t perm_grand_product(const fr* const w[3], const fr* const sg[3], const fr* beta, const fr* gamma,
+                              const fr* w0, const fr* w1, const host::fr64 k[3], fr* z, uint64_t n, fr* scratch,
+                              hipStream_t s);
 
 // synthetic inputs and digests (util.hip)
 uint64_t uniform_work_bytes(uint64_t n);

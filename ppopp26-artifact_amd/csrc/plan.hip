@@ -91,7 +91,7 @@ hipError_t phase_plan(PhasePlan& ph, const std::vector<Item>& items, Bump& bump,
     if (it.op == IT_PERM_GP) {
       const Domain* d = cached_domain(it.lg, &e);
       if (!d) return e;
-      const uint64_t se = std::max(ops_scratch_elems(n), scan_scratch_elems(n));
+      const uint64_t se = ops_scratch_elems(n);
       fr* scr = (fr*)bump.get(se * sizeof(fr));
       ph.scans.push_back({IT_PERM_GP, (int)i, d, scr, n});
       continue;
@@ -195,9 +195,8 @@ static void launch_ragged(const RaggedGroup& g, hipStream_t s) {
 }
 
 static hipError_t perm_gp(const Item& it, const ScanJob& sc, hipStream_t s) {
-  hipError_t e = perm_ratio(it.w, it.sg, it.beta, it.gamma, sc.dom->pw0, sc.dom->pw1, it.k, it.out, sc.n, sc.scratch, s);
-  if (e == hipSuccess) e = grand_product(it.out, nullptr, sc.n, sc.scratch, s);
-  return e;
+  return perm_grand_product(it.w, it.sg, it.beta, it.gamma, sc.dom->pw0, sc.dom->pw1, it.k, it.out, sc.n, sc.scratch,
+                            s);
 }
 
 hipError_t phase_enqueue(const PhasePlan& ph, hipStream_t main, hipStream_t side, hipEvent_t fork, hipEvent_t join) {

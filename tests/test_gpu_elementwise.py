@@ -130,3 +130,64 @@ def test_perm_grand_product_vs_oracle(pl, lg):
     z = pl.perm_ratio(dev[:3], dev[3:], pl.to_dev(beta), pl.to_dev(gamma), np.concatenate(ks), dom,
                       grand_product=True)
     np.testing.assert_array_equal(pl.to_host(z), po.grand_product(exp_ratio))
+
+
+def _perm_inputs(n, lg, salt):
+    cols = [po.uniform(n, po.seed(po.OP_GP, lg, salt + j)) for j in range(6)]
+    beta, gamma = po.uniform(1, po.seed(po.OP_GP, lg, salt + 10)), po.uniform(1, po.seed(po.OP_GP, lg, salt + 11))
+    return cols, beta, gamma
+
+
+@pytest.mark.parametrize("lg", [1, 3, 8, 11, 12, 13, 14, 15, 21, 22])
+def test_perm_grand_product_levels(pl, lg):
+    """every top-level geometry (a single block of 256 x 1, 2, 4, 8 run
+    products, directly above the runs of 8 elements or above block products)
+    against the oracle's ratio and grand product"""
+    n = 1 << lg
+    print("perm gp levels", lg, flush=True)
+    cols, beta, gamma = _perm_inputs(n, lg, 40)
+    ks = [po.from_u64(1), po.from_u64(7), po.from_u64(13)]
+    dom = pl.Domain(lg)
+    dev = [pl.to_dev(c) for c in cols]
+    z = pl.perm_ratio(dev[:3], dev[3:], pl.to_dev(beta), pl.to_dev(gamma), np.concatenate(ks), dom,
+                      grand_product=True)
+    np.testing.assert_array_equal(pl.to_host(z), po.grand_product(po.perm_ratio(cols, beta, gamma, ks, lg)))
+
+
+@pytest.mark.parametrize("lg,i0", [(3, 0), (3, 7), (12, 4000), (13, 2047), (13, 2048), (15, 12345)])
+def test_perm_grand_product_zero_den(pl, lg, i0):
+    """a zero denominator (w_0 = -(beta s_0 + gamma) at i0) gives a zero ratio
+    under x/0 = 0: z is zero after i0"""
+    n = 1 << lg
+    print("perm gp zero den", lg, i0, flush=True)
+    cols, beta, gamma = _perm_inputs(n, lg, 60)
+    t = po.add(po.mul(beta, cols[3][i0:i0 + 1]), gamma)
+    cols[0][i0] = po.neg(t)[0]
+    ks = [po.from_u64(1), po.from_u64(7), po.from_u64(13)]
+    ratio = po.perm_ratio(cols, beta, gamma, ks, lg)
+    assert not ratio[i0].any()
+    exp = po.grand_product(ratio)
+    dom = pl.Domain(lg)
+    dev = [pl.to_dev(c) for c in cols]
+    z = pl.perm_ratio(dev[:3], dev[3:], pl.to_dev(beta), pl.to_dev(gamma), np.concatenate(ks), dom,
+                      grand_product=True)
+    np.testing.assert_array_equal(pl.to_host(z), exp)
+
+
+def test_perm_grand_product_deep(pl):
+    """2^26 elements (runs of 8, then block products over two levels): the
+    fused grand product equals the oracle's grand product of the ratio vector,
+    itself checked against the oracle at smaller sizes"""
+    import torch
+    lg = 26
+    n = 1 << lg
+    print("perm gp 2^26", flush=True)
+    dev = [pl.fill_uniform(torch.empty((n, 4), dtype=torch.int64, device="cuda"), 900 + j) for j in range(6)]
+    beta = pl.to_dev(po.uniform(1, 991))
+    gamma = pl.to_dev(po.uniform(1, 992))
+    ks = np.concatenate([po.from_u64(1), po.from_u64(7), po.from_u64(13)])
+    dom = pl.Domain(lg)
+    ratio = pl.to_host(pl.perm_ratio(dev[:3], dev[3:], beta, gamma, ks, dom))
+    z = pl.to_host(pl.perm_ratio(dev[:3], dev[3:], beta, gamma, ks, dom, grand_product=True))
+    del dev
+    np.testing.assert_array_equal(z, po.grand_product(ratio))
