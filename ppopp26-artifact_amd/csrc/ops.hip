@@ -427,6 +427,7 @@ struct RatioArgs {
   const fr* gamma;
   const fr* w0;  // w^i = w0[i & 16383] * w1[i >> 14]
   const fr* w1;
+  fr* tab;       // [3][16384] pbd words of (beta k_j w0[lo]) R' (perm_table_kernel)
   fr k[3];
   fr k29[3];  // pbd words of (32 k_j) R' = 1024 k_j R mod r
   fr* num;
@@ -436,9 +437,10 @@ struct RatioArgs {
 
 // Terms in signed radix-2^29 limbs (fr29.hpp).  With S(a) = a R (stored form)
 // and the product P(x, y) = x y / 2^261:
-//   P(S(w0), S(w1)) = S(w^i / 32)               (two stored-form windows)
-//   P(., beta R')   = S(beta w^i / 32)          (beta R' = 32 S(beta))
-//   P(., (32 k) R') = S(beta k w^i)             (host constant, pbd word)
+//   P(S(w0), beta R') = S(beta w0)                (beta R' = 32 S(beta))
+//   P(., (32 k) R')   = (beta k w0) R'            (host constant, pbd word)
+//     -- once per call for the 16384 low-window entries (perm_table_kernel) --
+//   P((beta k w0) R', S(w1)) = S(beta k w^i)      (w^i = w0[lo] w1[hi])
 //   P(S(s_j), beta R') = S(beta s_j)
 // so n_j = S(w_j + gamma + beta k_j w^i) and d_j = S(w_j + gamma + beta s_j)
 // exactly, and the two products of three terms each carry 2^-10:
@@ -461,22 +463,41 @@ __device__ __forceinline__ TermConsts term_consts(const RatioArgs& a) {
   return c;
 }
 
+constexpr int kWin = 16384;  // entries of the low window table
+
+// tab[j][lo] = P(P(S(w0[lo]), beta R'), (32 k_j) R') = (beta k_j w0[lo]) R', so
+// that P(tab[j][lo], S(w1[hi])) = S(beta k_j w^i)
+__global__ void __launch_bounds__(256) perm_table_kernel(RatioArgs a, uint32_t count) {
+  const uint32_t lo = blockIdx.x * blockDim.x + threadIdx.x;
+  if (lo >= count) return;
+  const TermConsts c = term_consts(a);
+  v9 w;
+  unpack_ext(fr_load(a.w0 + lo).l, w.l);
+  const v9 bw = v9_mul<false, false, false>(w, c.B);
+#pragma unroll
+  for (int j = 0; j < 3; j++) {
+    uint32_t K[9];
+    pbd_unpack(a.k29[j].l, K);
+    const v9 t = v9_mul<false, false, true>(bw, K);  // within (-1.02 r, 0.02 r): a pbd word
+    fr o;
+    pbd_pack(t.l, o.l);
+    fr_store(a.tab + j * kWin + lo, o);
+  }
+}
+
 // num and den of element i (before reduction)
 __device__ __forceinline__ void perm_terms_at(const RatioArgs& a, const TermConsts& c, uint64_t i, v9& num,
                                               v9& den) {
-  v9 w0, w1;
-  unpack_ext(fr_load(a.w0 + (i & 16383)).l, w0.l);
+  v9 w1;
   unpack_ext(fr_load(a.w1 + (i >> 14)).l, w1.l);
-  const v9 wi = v9_mul<false, false, false>(w0, w1.l);
-  const v9 bx = v9_mul<false, false, false>(wi, c.B);
 #pragma unroll
   for (int j = 0; j < 3; j++) {
     v9 wj, sj;
     unpack_ext(fr_load(a.w[j] + i).l, wj.l);
     const v9 wg = v9_add(wj, c.g);
     uint32_t K[9];
-    pbd_unpack(a.k29[j].l, K);
-    const v9 nj = v9_add(wg, v9_mul<false, false, true>(bx, K));
+    pbd_unpack(fr_load(a.tab + j * kWin + (i & (kWin - 1))).l, K);
+    const v9 nj = v9_add(wg, v9_mul<false, false, true>(w1, K));
     unpack_ext(fr_load(a.s[j] + i).l, sj.l);
     const v9 dj = v9_add(wg, v9_mul<false, false, true>(sj, c.B));
     num = j == 0 ? nj : v9_mul<false, false, false>(num, nj.l);  // the first factor needs no product
@@ -790,7 +811,9 @@ static void gp_top(const fr* nv, const fr* dv, uint64_t n, fr* out, hipStream_t 
   else hipLaunchKernelGGL((gp_top_kernel<kOpLgPer, INCL>), g, b, 0, s, nv, dv, n, out);
 }
 
-uint64_t ops_scratch_elems(uint64_t n) { return std::max(n + inv_scratch_elems(n), gp_scratch_elems(n)); }
+uint64_t ops_scratch_elems(uint64_t n) {
+  return 3 * kWin + std::max(n + inv_scratch_elems(n), gp_scratch_elems(n));
+}
 
 hipError_t vec_op(int op, const fr* a, const fr* b, fr* out, uint64_t n, fr* scratch, hipStream_t s) {
   if (n == 0) return hipSuccess;
@@ -826,16 +849,26 @@ static RatioArgs ratio_args(const fr* const w[3], const fr* const sg[3], const f
   a.gamma = gamma;
   a.w0 = w0;
   a.w1 = w1;
+  a.tab = nullptr;
   a.num = nullptr;
   a.den = nullptr;
   a.n = n;
   return a;
 }
 
+// the window tables at the front of scratch (3 kWin elements)
+static void perm_tables(RatioArgs& a, fr*& scratch, hipStream_t s) {
+  a.tab = scratch;
+  scratch += 3 * kWin;
+  const uint32_t count = (uint32_t)std::min<uint64_t>(a.n, kWin);
+  hipLaunchKernelGGL(perm_table_kernel, dim3((count + 255) / 256), dim3(256), 0, s, a, count);
+}
+
 hipError_t perm_ratio(const fr* const w[3], const fr* const sg[3], const fr* beta, const fr* gamma,
                       const fr* w0, const fr* w1, const host::fr64 k[3], fr* ratio, uint64_t n, fr* scratch,
                       hipStream_t s) {
   RatioArgs a = ratio_args(w, sg, beta, gamma, w0, w1, k, n);
+  perm_tables(a, scratch, s);
   a.num = ratio;
   a.den = scratch;
   hipLaunchKernelGGL(perm_terms_kernel<false>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, nullptr,
@@ -848,6 +881,7 @@ hipError_t perm_grand_product(const fr* const w[3], const fr* const sg[3], const
                               const fr* w0, const fr* w1, const host::fr64 k[3], fr* z, uint64_t n, fr* scratch,
                               hipStream_t s) {
   RatioArgs a = ratio_args(w, sg, beta, gamma, w0, w1, k, n);
+  perm_tables(a, scratch, s);
   a.num = z;
   a.den = scratch;
   const uint64_t nr = (n + kOpPer - 1) / kOpPer;
