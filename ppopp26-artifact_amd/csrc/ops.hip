@@ -621,21 +621,31 @@ __global__ void __launch_bounds__(kOpThreads) gp_up_kernel(const fr* nv, const f
   }
 }
 
+// 32 fr_inv_single(total): the R'-view word of 1 / (32 total / R')
+__device__ __forceinline__ fr gp_inverse(const fr& total) {
+  fr z = fr_inv_single(total);
+#pragma unroll
+  for (int d = 0; d < 5; d++) z = fr_add(z, z);
+  return z;
+}
+
 // Down-sweep over one level (in place: out may be nv).  Thread t of block b
 // owns elements base = (256 b + t) 2^LG + k, k < 2^LG, and writes
 //   out = c_b pre_t suf_t prod_{k'<k} num [prod_{k'>=k} den (INCL) or k' > k]
 // with pre_t / suf_t the products over the block's threads before / after t.
-// TOP: a single block; the thread products are formed here, short scans, and
-// c = 32 fr_inv_single(total den).  Otherwise c_b = carry[b] and the thread
-// products come from the up-sweep (tn, td).
+// TOP: a single block; the thread products are formed here and c =
+// gp_inverse(total den), or c = one with the total stored to *tot when a
+// later launch inverts it.  Otherwise c_b = carry[b] and the thread products
+// come from the up-sweep (tn, td).
 template <int LG, bool INCL, bool TOP>
 __device__ __forceinline__ void gp_down_body(const fr* nv, const fr* dv, const fr* tn, const fr* td, uint64_t n,
-                                             const fr* carry, fr* out) {
+                                             const fr* carry, fr* out, fr* tot,
+                                             uint32_t blk) {
   constexpr int kPer = 1 << LG;
   __shared__ uint4 l[2][kOpThreads], l2[2][kOpThreads];
   __shared__ uint32_t cinv[8];
   const int t = threadIdx.x;
-  const uint64_t tid = (uint64_t)blockIdx.x * kOpThreads + t;
+  const uint64_t tid = (uint64_t)blk * kOpThreads + t;
   const uint64_t base = tid * kPer;
   fr one;
   {
@@ -697,9 +707,14 @@ __device__ __forceinline__ void gp_down_body(const fr* nv, const fr* dv, const f
       uint32_t b[9];
       pbd_unpack(suf.l, a.l);
       pbd_unpack(tdw.l, b);
-      fr z = fr_inv_single(v9_canonical(v9_mul<false, false, false>(a, b)));
-#pragma unroll
-      for (int d = 0; d < 5; d++) z = fr_add(z, z);
+      const fr total = v9_canonical(v9_mul<false, false, false>(a, b));
+      fr z;
+      if (tot) {
+        fr_store(tot, total);  // inverted by gp_inverse beside the next sweep
+        z = rp_one();
+      } else {
+        z = gp_inverse(total);
+      }
 #pragma unroll
       for (int i = 0; i < 8; i++) cinv[i] = z.l[i];
     }
@@ -707,7 +722,7 @@ __device__ __forceinline__ void gp_down_body(const fr* nv, const fr* dv, const f
 #pragma unroll
     for (int i = 0; i < 8; i++) cw.l[i] = cinv[i];
   } else {
-    cw = fr_load(carry + blockIdx.x);
+    cw = fr_load(carry + blk);
   }
   v9 cur;
   {
@@ -736,23 +751,38 @@ __device__ __forceinline__ void gp_down_body(const fr* nv, const fr* dv, const f
 }
 
 // bulk levels: three waves per SIMD (the local den suffixes stay in registers)
+// inv != nullptr: one more block than the level needs; block 0 (dispatched
+// first) has thread 0 write gp_inverse(*tot) to inv while the other blocks
+// sweep (the inverse then multiplies the run carries, gp_run_down_kernel)
 template <bool INCL>
 __global__ void __launch_bounds__(kOpThreads) __attribute__((amdgpu_waves_per_eu(3)))
-gp_down_kernel(const fr* nv, const fr* dv, const fr* tn, const fr* td, uint64_t n, const fr* carry, fr* out) {
-  gp_down_body<kOpLgPer, INCL, false>(nv, dv, tn, td, n, carry, out);
+gp_down_kernel(const fr* nv, const fr* dv, const fr* tn, const fr* td, uint64_t n, const fr* carry, fr* out,
+               const fr* tot, fr* inv) {
+  uint32_t blk = blockIdx.x;
+  if (inv) {
+    if (blk == 0) {
+      if (threadIdx.x == 0) fr_store(inv, gp_inverse(fr_load(tot)));
+      return;
+    }
+    blk--;
+  }
+  gp_down_body<kOpLgPer, INCL, false>(nv, dv, tn, td, n, carry, out, nullptr, blk);
 }
 
 // the single-block top level
+// tot == nullptr: the inversion runs here (no level below the top but the
+// runs); otherwise the total is stored for gp_down_kernel's extra block
 template <int LG, bool INCL>
-__global__ void __launch_bounds__(kOpThreads) gp_top_kernel(const fr* nv, const fr* dv, uint64_t n, fr* out) {
-  gp_down_body<LG, INCL, true>(nv, dv, nullptr, nullptr, n, nullptr, out);
+__global__ void __launch_bounds__(kOpThreads) gp_top_kernel(const fr* nv, const fr* dv, uint64_t n, fr* out,
+                                                             fr* tot) {
+  gp_down_body<LG, INCL, true>(nv, dv, nullptr, nullptr, n, nullptr, out, tot, 0);
 }
 
 // Level 0: thread = run of 8 consecutive elements with carry c (the num
 // products of the runs before it, the den products of those after it, the
 // inverse): z_k = c prod_{k'<k} num prod_{k'>=k} den.  No scans.
 __global__ void __launch_bounds__(256) gp_run_down_kernel(const fr* nv, const fr* dv, uint64_t n, const fr* carry,
-                                                          fr* out) {
+                                                          const fr* inv, fr* out) {
   const uint64_t run = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   const uint64_t base = run * kOpPer;
   if (base >= n) return;
@@ -770,6 +800,11 @@ __global__ void __launch_bounds__(256) gp_run_down_kernel(const fr* nv, const fr
   });
   v9 cur;
   unpack_ext(fr_load(carry + run).l, cur.l);
+  if (inv) {
+    uint32_t w[9];
+    unpack_ext(fr_load(inv).l, w);
+    cur = v9_mul<false, false, false>(cur, w);
+  }
   static_for<0, kOpPer>([&](auto k_) {
     constexpr int k = decltype(k_)::value;
     const bool live = base + k < n;
@@ -790,10 +825,10 @@ __global__ void __launch_bounds__(256) gp_run_down_kernel(const fr* nv, const fr
 // level sizes of the grand product: the runs of 8 elements (n_1 = ceil(n / 8)),
 // then n_{L+1} = ceil(n_L / 2048) while n_L > 2048; scratch holds den (n), the
 // run products (2 n_1), then per up-swept level the thread products (2 x 256
-// per block) and the next level's num and den
+// per block) and the next level's num and den, then the total and its inverse
 static uint64_t gp_scratch_elems(uint64_t n) {
   uint64_t m = (n + kOpPer - 1) / kOpPer;
-  uint64_t tot = n + 2 * m;
+  uint64_t tot = n + 2 * m + 2;  // + the total den product and its inverse
   while (m > (uint64_t)kOpSeg) {
     const uint64_t nb = (m + kOpSeg - 1) / kOpSeg;
     tot += 2 * nb * kOpThreads + 2 * nb;
@@ -803,12 +838,12 @@ static uint64_t gp_scratch_elems(uint64_t n) {
 }
 
 template <bool INCL>
-static void gp_top(const fr* nv, const fr* dv, uint64_t n, fr* out, hipStream_t s) {
+static void gp_top(const fr* nv, const fr* dv, uint64_t n, fr* out, fr* tot, hipStream_t s) {
   const dim3 g(1), b(kOpThreads);
-  if (n <= 256) hipLaunchKernelGGL((gp_top_kernel<0, INCL>), g, b, 0, s, nv, dv, n, out);
-  else if (n <= 512) hipLaunchKernelGGL((gp_top_kernel<1, INCL>), g, b, 0, s, nv, dv, n, out);
-  else if (n <= 1024) hipLaunchKernelGGL((gp_top_kernel<2, INCL>), g, b, 0, s, nv, dv, n, out);
-  else hipLaunchKernelGGL((gp_top_kernel<kOpLgPer, INCL>), g, b, 0, s, nv, dv, n, out);
+  if (n <= 256) hipLaunchKernelGGL((gp_top_kernel<0, INCL>), g, b, 0, s, nv, dv, n, out, tot);
+  else if (n <= 512) hipLaunchKernelGGL((gp_top_kernel<1, INCL>), g, b, 0, s, nv, dv, n, out, tot);
+  else if (n <= 1024) hipLaunchKernelGGL((gp_top_kernel<2, INCL>), g, b, 0, s, nv, dv, n, out, tot);
+  else hipLaunchKernelGGL((gp_top_kernel<kOpLgPer, INCL>), g, b, 0, s, nv, dv, n, out, tot);
 }
 
 uint64_t ops_scratch_elems(uint64_t n) {
@@ -909,16 +944,21 @@ hipError_t perm_grand_product(const fr* const w[3], const fr* const sg[3], const
                        c.n, c.tn, c.td, bn, bd);
     lv[++top] = {bn, bd, nullptr, nullptr, nb};
   }
-  // carries in place over each level's num (the runs' carries over rn)
-  gp_top<false>(lv[top].nv, lv[top].dv, lv[top].n, lv[top].nv, s);
+  // carries in place over each level's num (the runs' carries over rn).  With
+  // a level between the top and the runs, the top stores the total den product
+  // and the first sweep below it inverts it in an extra block.
+  fr* tot = top > 0 ? p : nullptr;
+  fr* inv = top > 0 ? p + 1 : nullptr;
+  gp_top<false>(lv[top].nv, lv[top].dv, lv[top].n, lv[top].nv, tot, s);
   for (int L = top - 1; L >= 0; L--) {
     const Level& c = lv[L];
-    const unsigned nb = (unsigned)((c.n + kOpSeg - 1) / kOpSeg);
+    const bool first = L == top - 1;
+    const unsigned nb = (unsigned)((c.n + kOpSeg - 1) / kOpSeg) + (first ? 1 : 0);
     hipLaunchKernelGGL(gp_down_kernel<false>, dim3(nb), dim3(kOpThreads), 0, s, c.nv, c.dv, c.tn, c.td, c.n,
-                       (const fr*)lv[L + 1].nv, c.nv);
+                       (const fr*)lv[L + 1].nv, c.nv, (const fr*)tot, first ? inv : nullptr);
   }
   hipLaunchKernelGGL(gp_run_down_kernel, dim3((unsigned)((nr + 255) / 256)), dim3(256), 0, s, (const fr*)z,
-                     (const fr*)scratch, n, (const fr*)rn, z);
+                     (const fr*)scratch, n, (const fr*)rn, (const fr*)inv, z);
   return hipGetLastError();
 }
 
