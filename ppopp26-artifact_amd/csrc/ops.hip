@@ -427,7 +427,8 @@ struct RatioArgs {
   const fr* gamma;
   const fr* w0;  // w^i = w0[i & 16383] * w1[i >> 14]
   const fr* w1;
-  fr* tab;       // [3][16384] pbd words of (beta k_j w0[lo]) R' (perm_table_kernel)
+  fr* tab;       // [3][tstride] pbd words of (beta k_j w0[lo]) R' (perm_table_kernel)
+  uint32_t tstride;  // min(n, 16384): the low-window entries in use
   fr k[3];
   fr k29[3];  // pbd words of (32 k_j) R' = 1024 k_j R mod r
   fr* num;
@@ -464,6 +465,7 @@ __device__ __forceinline__ TermConsts term_consts(const RatioArgs& a) {
 }
 
 constexpr int kWin = 16384;  // entries of the low window table
+static uint64_t perm_table_elems(uint64_t n) { return 3 * std::min<uint64_t>(n, kWin); }
 
 // tab[j][lo] = P(P(S(w0[lo]), beta R'), (32 k_j) R') = (beta k_j w0[lo]) R', so
 // that P(tab[j][lo], S(w1[hi])) = S(beta k_j w^i)
@@ -481,7 +483,7 @@ __global__ void __launch_bounds__(256) perm_table_kernel(RatioArgs a, uint32_t c
     const v9 t = v9_mul<false, false, true>(bw, K);  // within (-1.02 r, 0.02 r): a pbd word
     fr o;
     pbd_pack(t.l, o.l);
-    fr_store(a.tab + j * kWin + lo, o);
+    fr_store(a.tab + j * a.tstride + lo, o);
   }
 }
 
@@ -496,7 +498,7 @@ __device__ __forceinline__ void perm_terms_at(const RatioArgs& a, const TermCons
     unpack_ext(fr_load(a.w[j] + i).l, wj.l);
     const v9 wg = v9_add(wj, c.g);
     uint32_t K[9];
-    pbd_unpack(fr_load(a.tab + j * kWin + (i & (kWin - 1))).l, K);
+    pbd_unpack(fr_load(a.tab + j * a.tstride + (i & (kWin - 1))).l, K);
     const v9 nj = v9_add(wg, v9_mul<false, false, true>(w1, K));
     unpack_ext(fr_load(a.s[j] + i).l, sj.l);
     const v9 dj = v9_add(wg, v9_mul<false, false, true>(sj, c.B));
@@ -847,7 +849,7 @@ static void gp_top(const fr* nv, const fr* dv, uint64_t n, fr* out, fr* tot, hip
 }
 
 uint64_t ops_scratch_elems(uint64_t n) {
-  return 3 * kWin + std::max(n + inv_scratch_elems(n), gp_scratch_elems(n));
+  return perm_table_elems(n) + std::max(n + inv_scratch_elems(n), gp_scratch_elems(n));
 }
 
 hipError_t vec_op(int op, const fr* a, const fr* b, fr* out, uint64_t n, fr* scratch, hipStream_t s) {
@@ -885,17 +887,19 @@ static RatioArgs ratio_args(const fr* const w[3], const fr* const sg[3], const f
   a.w0 = w0;
   a.w1 = w1;
   a.tab = nullptr;
+  a.tstride = 0;
   a.num = nullptr;
   a.den = nullptr;
   a.n = n;
   return a;
 }
 
-// the window tables at the front of scratch (3 kWin elements)
+// the window tables at the front of scratch (perm_table_elems(n) elements)
 static void perm_tables(RatioArgs& a, fr*& scratch, hipStream_t s) {
-  a.tab = scratch;
-  scratch += 3 * kWin;
   const uint32_t count = (uint32_t)std::min<uint64_t>(a.n, kWin);
+  a.tab = scratch;
+  a.tstride = count;
+  scratch += perm_table_elems(a.n);
   hipLaunchKernelGGL(perm_table_kernel, dim3((count + 255) / 256), dim3(256), 0, s, a, count);
 }
 
