@@ -195,9 +195,10 @@ int sim_column(int la, int last, int ext, const uint32_t* in, const uint32_t* w1
   return g_bad;
 }
 
-// The permutation-ratio terms of one element (ops.hip perm_terms_kernel) in
-// the same arithmetic; returns the number of violated bounds.  b32: 32 S(beta)
-// mod r; k29: three pbd words; the other inputs are stored-form words.
+// The permutation-ratio terms of one element (ops.hip perm_table_kernel and
+// perm_terms_kernel) in the same arithmetic; returns the number of violated
+// bounds.  b32: 32 S(beta) mod r; k29: three pbd words; w0, w1: the stored
+// window values (w^i = w0 w1); the other inputs are stored-form words.
 int sim_perm_terms(const uint32_t* w0, const uint32_t* w1, const uint32_t* b32, const uint32_t* gamma,
                    const uint32_t* k29, const uint32_t* w, const uint32_t* s, uint32_t* num_out,
                    uint32_t* den_out) {
@@ -206,18 +207,29 @@ int sim_perm_terms(const uint32_t* w0, const uint32_t* w1, const uint32_t* b32, 
   unpack_ext(b32, B);
   unpack_ext(gamma, b);
   const V g = from_u(b);
+  // the table entry of w0 per j: P(P(S(w0), beta R'), (32 k_j) R') as a pbd word
   unpack_ext(w0, b);
-  const V x0 = from_u(b);
+  const V bw = mul(from_u(b), B, false, false);
+  uint32_t T[3][8];
+  for (int j = 0; j < 3; j++) {
+    uint32_t K[9], u[9];
+    pbd_unpack(k29 + 8 * j, K);
+    const V t = mul(bw, K, false, true);
+    for (int i = 0; i < 8; i++)
+      if (t.l[i] < -(1 << 28) || t.l[i] >= (1 << 28)) g_bad++;
+    if (t.l[8] < -(1 << 23) || t.l[8] >= (1 << 23)) g_bad++;
+    to_u(t, u);
+    pbd_pack(u, T[j]);
+  }
   unpack_ext(w1, b);
-  const V wi = mul(x0, b, false, false);
-  const V bx = mul(wi, B, false, false);
+  const V x1 = from_u(b);
   V num, den;
   for (int j = 0; j < 3; j++) {
     unpack_ext(w + 8 * j, b);
     const V wg = add(from_u(b), g);
     uint32_t K[9];
-    pbd_unpack(k29 + 8 * j, K);
-    const V nj = add(wg, mul(bx, K, false, true));
+    pbd_unpack(T[j], K);
+    const V nj = add(wg, mul(x1, K, false, true));
     unpack_ext(s + 8 * j, b);
     const V dj = add(wg, mul(from_u(b), B, false, true));
     uint32_t u[9];

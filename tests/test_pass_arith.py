@@ -17,9 +17,10 @@ csrc/ntt_pass.hpp) on the host, against Python integers.
   integers (times the pass twiddles, or canonical in the last pass).  Inputs
   include the extremes: all 2^256 - 1 (the largest DC sums), all digits at
   -2^28 or 2^28 - 1, and chains of two passes;
-* the permutation-ratio terms (ops.hip perm_terms_kernel) of one element in
-  the same arithmetic: every bound holds and num / den carry exactly the
-  common factor 2^-10 (random and all-(2^256 - 1) inputs).
+* the permutation-ratio terms (ops.hip perm_table_kernel's table entry and
+  perm_terms_kernel) of one element in the same arithmetic: every bound holds
+  (the table entry fits a pbd word) and num / den carry exactly the common
+  factor 2^-10 (random and all-(2^256 - 1) inputs).
 """
 import ctypes
 import json
