@@ -192,8 +192,11 @@ int pl_vec_op(int op, const void* d_a, const void* d_b, void* d_out, uint64_t n,
  * for j = 0..2, w = omega_{2^lg} from the domain d (d->lg == lg), x/0 = 0.
  * d_w, d_s: three device columns each; beta, gamma device elements; h_k three
  * host Montgomery constants (k1, k2, k3).  d_work: NULL or
- * pl_vec_work_bytes(2^lg) bytes.  pl_perm_grand_product then runs the
- * exclusive grand product over the ratio in place (the PLONK z column). */
+ * pl_vec_work_bytes(2^lg) bytes.  pl_perm_grand_product writes the
+ * exclusive grand product of the ratio to d_z (the PLONK z column: z[0] =
+ * Mont(1), z[i] = prod_{j<i} ratio[j]) without forming the ratio vector:
+ * z[i] = prod_{j<i} num[j] * prod_{j>=i} den[j] / prod_j den[j], one field
+ * inversion per call; same arguments, same d_work size. */
 int pl_perm_ratio(const void* const* d_w, const void* const* d_s, const void* d_beta, const void* d_gamma,
                   const void* h_k, int lg, const pl_domain* d, void* d_ratio, void* d_work, pl_stream_t stream);
 int pl_perm_grand_product(const void* const* d_w, const void* const* d_s, const void* d_beta,
