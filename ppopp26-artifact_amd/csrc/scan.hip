@@ -216,6 +216,23 @@ __device__ fr block_suffix(fr v, uint4 (*l)[kScanThreads], const fr* pw) {
   return v;
 }
 
+// sum_k v_k w^k by Horner in signed limbs (fr29.hpp): h <- P(h, w R') + v_k,
+// limb-wise sums of a product (digits in [0, 2^29)) and balanced digits, so
+// every limb stays below 2^30 and the value within (-r, 1.02 r); canonical
+// once at the end.  w9: balanced digits of w R'.
+template <int K>
+__device__ __forceinline__ fr horner29(const fr (&v)[K], const uint32_t (&w9)[9]) {
+  v9 h;
+  unpack_ext(v[K - 1].l, h.l);
+  static_for<0, K - 1>([&](auto k_) {
+    constexpr int k = K - 2 - decltype(k_)::value;
+    v9 x;
+    unpack_ext(v[k].l, x.l);
+    h = v9_add(v9_mul<false, false, false>(h, w9), x);
+  });
+  return v9_canonical(h);
+}
+
 // partial[b] = sum over block b of a_k m^(k - start).  The sum is
 // order-free: thread t takes elements start + 256 j + t (coalesced loads),
 // H_t = sum_j a m^(256 j) by Horner in m^256, then m^t H_t; a plain sum over
@@ -227,18 +244,14 @@ __global__ void __launch_bounds__(kScanThreads) sfx_reduce_kernel(const fr* a, u
   const int t = threadIdx.x;
   const uint64_t b0 = (uint64_t)blockIdx.x << kLgSeg;
   uint32_t M[9];
-  split_fr(fr_load(pw + 8), M);  // m^256
+  unpack_ext(fr_load(pw + 8).l, M);  // m^256, balanced digits
   fr v[kPer];
   static_for<0, kPer>([&](auto k_) {
     constexpr int k = decltype(k_)::value;
     const uint64_t i = b0 + (uint64_t)k * kScanThreads + t;
     v[k] = i < n ? fr_load(a + i) : fr_zero();
   });
-  fr h = v[kPer - 1];
-  static_for<0, kPer - 1>([&](auto k_) {
-    constexpr int k = kPer - 2 - decltype(k_)::value;
-    h = fr_add(mulp(h, M), v[k]);
-  });
+  fr h = horner29<kPer>(v, M);
   // m^t R' from the powers m^(2^bit) R'
   fr mt;
 #pragma unroll
@@ -270,18 +283,14 @@ __global__ void __launch_bounds__(kScanThreads) sfx_down_kernel(const fr* a, fr*
   const int t = threadIdx.x;
   const uint64_t base = ((uint64_t)blockIdx.x << kLgSeg) + (uint64_t)t * kPer;
   uint32_t m[9];
-  split_fr(fr_load(pw), m);
+  unpack_ext(fr_load(pw).l, m);  // balanced digits
   fr e[kPer];
   static_for<0, kPer>([&](auto k_) {
     constexpr int k = decltype(k_)::value;
     e[k] = base + k < n ? fr_load(a + base + k) : fr_zero();
   });
   // local Horner: L = sum_k e_k m^k
-  fr h = e[kPer - 1];
-  static_for<0, kPer - 1>([&](auto k_) {
-    constexpr int k = kPer - 2 - decltype(k_)::value;
-    h = fr_add(mulp(h, m), e[k]);
-  });
+  fr h = horner29<kPer>(e, m);
   // fold the block carry into the last thread: s(start of thread 255) = L + m^kPer K
   fr K = carry ? fr_load(carry + blockIdx.x) : fr_zero();
   if (t == kScanThreads - 1) h = fr_add(h, mulp(K, fr_load(pw + kLgPer)));
@@ -289,12 +298,18 @@ __global__ void __launch_bounds__(kScanThreads) sfx_down_kernel(const fr* a, fr*
   // carry-in at this thread's end = s at the start of thread t+1
   fr cin = t + 1 < kScanThreads ? lds_get(l, t + 1) : K;
   if (total && blockIdx.x == 0 && t == 0) fr_store(total, v);
-  // rescan: s_k = e_k + m s_{k+1}, out[base + k] = s_{k+1}
-  fr sn = cin;
+  // rescan: s_k = e_k + m s_{k+1}, out[base + k] = s_{k+1} (signed limbs,
+  // canonical where stored)
+  v9 sn;
+  unpack_ext(cin.l, sn.l);
   static_for<0, kPer>([&](auto k_) {
     constexpr int k = kPer - 1 - decltype(k_)::value;
-    if (base + k < n) fr_store(out + base + k, sn);
-    sn = fr_add(e[k], mulp(sn, m));
+    if (base + k < n) fr_store(out + base + k, k == kPer - 1 ? cin : v9_canonical(sn));
+    if constexpr (k > 0) {
+      v9 x;
+      unpack_ext(e[k].l, x.l);
+      sn = v9_add(v9_mul<false, false, false>(sn, m), x);
+    }
   });
 }
 
