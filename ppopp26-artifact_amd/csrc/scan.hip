@@ -183,14 +183,12 @@ static hipError_t prefix_product(const fr* in, fr* out_excl, fr* out_incl, uint6
 // affine suffix scan  s_i = a_i + m * s_{i+1}
 // pw: device table of m^(2^j) R' mod r, j = 0..kLgSeg (m^kSeg last), R' = 2^261:
 // every product of the scan multiplies a value by a power of m, so it runs as
-// the radix-2^29 product (fr29.hpp) with the power in that form; results are
-// reduced to canonical form at once (the additions expect values below r).
+// the radix-2^29 product (fr29.hpp) with the power in that form: the Horner
+// steps in signed limbs (horner29), the block combines on canonical values.
 // ---------------------------------------------------------------------------
 
-// a * w R' / R' = a w (stored form), canonical; w9: the nine limbs of w R'
-__device__ __forceinline__ fr mulp(const fr& a, const uint32_t (&w9)[9]) { return fr_reduce_once(fr_mul29(a, w9)); }
+// a * w R' / R' = a w (stored form), canonical
 __device__ __forceinline__ fr mulp(const fr& a, const fr& w) { return fr_reduce_once(fr_mul29(a, w)); }
-__device__ __forceinline__ void split_fr(const fr& w, uint32_t (&w9)[9]) { split29(w.l, w9); }
 // R' mod r: the R'-form of one
 __host__ __device__ constexpr uint32_t rp_one_limb(int i) {
   return i == 0 ? 0xffffffbau : i == 1 ? 0x00000045u : i == 2 ? 0x0072d846u : i == 3 ? 0x1a25272eu
