@@ -271,11 +271,35 @@ __global__ void __launch_bounds__(kScanThreads) sfx_reduce_kernel(const fr* a, u
   if (t == 0) fr_store(partial + blockIdx.x, h);
 }
 
+// partial[b] as sfx_reduce_kernel computes it, from contiguous runs (thread t
+// owns the kPer elements the down-sweep gives it): the local Horner value L_t
+// goes to lt for the down-sweep (which then skips its own), then the block
+// suffix over the L_t gives the block's value at thread 0.
+__global__ void __launch_bounds__(kScanThreads) sfx_run_reduce_kernel(const fr* a, uint64_t n, const fr* pw,
+                                                                      fr* partial, fr* lt) {
+  __shared__ uint4 l[2][kScanThreads];
+  const int t = threadIdx.x;
+  const uint64_t tid = (uint64_t)blockIdx.x * kScanThreads + t;
+  const uint64_t base = tid * kPer;
+  uint32_t m[9];
+  unpack_ext(fr_load(pw).l, m);  // balanced digits
+  fr e[kPer];
+  static_for<0, kPer>([&](auto k_) {
+    constexpr int k = decltype(k_)::value;
+    e[k] = base + k < n ? fr_load(a + base + k) : fr_zero();
+  });
+  const fr h = horner29<kPer>(e, m);
+  fr_store(lt + tid, h);
+  const fr v = block_suffix<kLgPer>(h, l, pw);
+  if (t == 0) fr_store(partial + blockIdx.x, v);
+}
+
 // out[i] = s_{i+1} within each block given carry[b] = s at the block's end
 // (nullptr: zero); total (block 0) = s_0.  out may equal a.
+// lt: the local Horner values of sfx_run_reduce_kernel (nullptr: computed here)
 template <int LG>
 __global__ void __launch_bounds__(kScanThreads) sfx_down_kernel(const fr* a, fr* out, uint64_t n, const fr* pw,
-                                                                 const fr* carry, fr* total) {
+                                                                 const fr* carry, fr* total, const fr* lt) {
   constexpr int kPer = ScanGeom<LG>::kPer, kLgSeg = ScanGeom<LG>::kLgSeg, kLgPer = LG;
   __shared__ uint4 l[2][kScanThreads];
   const int t = threadIdx.x;
@@ -288,7 +312,7 @@ __global__ void __launch_bounds__(kScanThreads) sfx_down_kernel(const fr* a, fr*
     e[k] = base + k < n ? fr_load(a + base + k) : fr_zero();
   });
   // local Horner: L = sum_k e_k m^k
-  fr h = horner29<kPer>(e, m);
+  fr h = lt ? fr_load(lt + ((uint64_t)blockIdx.x * kScanThreads + t)) : horner29<kPer>(e, m);
   // fold the block carry into the last thread: s(start of thread 255) = L + m^kPer K
   fr K = carry ? fr_load(carry + blockIdx.x) : fr_zero();
   if (t == kScanThreads - 1) h = fr_add(h, mulp(K, fr_load(pw + kLgPer)));
@@ -353,7 +377,7 @@ static uint64_t sfx_scratch_elems(uint64_t n) {
   uint64_t tot = (uint64_t)sfx_pw_count(n);
   while (n > (uint64_t)kSeg) {
     n = (n + kSeg - 1) / kSeg;
-    tot += 2 * n;
+    tot += 2 * n + n * kScanThreads;  // block values, carries, local Horner values
   }
   return tot;
 }
@@ -367,10 +391,10 @@ static hipError_t suffix_scan(const fr* a, fr* out, uint64_t n, const fr* pw, fr
     if (out) {
       if (tiny)
         hipLaunchKernelGGL(sfx_down_kernel<0>, dim3(1), dim3(kScanThreads), 0, s, a, out, n, pw, (const fr*)nullptr,
-                           total);
+                           total, (const fr*)nullptr);
       else
         hipLaunchKernelGGL(sfx_down_kernel<kLgPer>, dim3(1), dim3(kScanThreads), 0, s, a, out, n, pw,
-                           (const fr*)nullptr, total);
+                           (const fr*)nullptr, total, (const fr*)nullptr);
     } else {
       // one block: the reduce kernel's tree value is s_0
       if (tiny)
@@ -383,12 +407,18 @@ static hipError_t suffix_scan(const fr* a, fr* out, uint64_t n, const fr* pw, fr
   const uint64_t nb = (n + kSeg - 1) / kSeg;
   fr* part = scratch;
   fr* carry = scratch + nb;
-  hipLaunchKernelGGL(sfx_reduce_kernel<kLgPer>, dim3(sgrid(n)), dim3(kScanThreads), 0, s, a, n, pw, part);
+  fr* lt = scratch + 2 * nb;
+  // with a down-sweep, the reduce runs over the down-sweep's runs and keeps
+  // their local Horner values; a reduction alone reads strided (coalesced)
+  if (out)
+    hipLaunchKernelGGL(sfx_run_reduce_kernel, dim3(sgrid(n)), dim3(kScanThreads), 0, s, a, n, pw, part, lt);
+  else
+    hipLaunchKernelGGL(sfx_reduce_kernel<kLgPer>, dim3(sgrid(n)), dim3(kScanThreads), 0, s, a, n, pw, part);
   // carry[b] = s at the end of block b = sum_{b' > b} part[b'] (m^4096)^(b'-b-1)
-  hipError_t e = suffix_scan(part, out ? carry : nullptr, nb, pw + kLgSeg, total, scratch + 2 * nb, s);
+  hipError_t e = suffix_scan(part, out ? carry : nullptr, nb, pw + kLgSeg, total, lt + nb * kScanThreads, s);
   if (e != hipSuccess || !out) return e;
   hipLaunchKernelGGL(sfx_down_kernel<kLgPer>, dim3(sgrid(n)), dim3(kScanThreads), 0, s, a, out, n, pw,
-                     (const fr*)carry, (fr*)nullptr);
+                     (const fr*)carry, (fr*)nullptr, (const fr*)lt);
   return hipGetLastError();
 }
 
