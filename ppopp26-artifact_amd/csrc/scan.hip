@@ -296,8 +296,10 @@ __global__ void __launch_bounds__(kScanThreads) sfx_run_reduce_kernel(const fr* 
 
 // out[i] = s_{i+1} within each block given carry[b] = s at the block's end
 // (nullptr: zero); total (block 0) = s_0.  out may equal a.
-// lt: the local Horner values of sfx_run_reduce_kernel (nullptr: computed here)
-template <int LG>
+// LT: the local Horner values come from sfx_run_reduce_kernel (lt); the
+// elements are then loaded only for the rescan, in two halves, so that fewer
+// values are live (more waves per SIMD).  Otherwise computed here.
+template <int LG, bool LT>
 __global__ void __launch_bounds__(kScanThreads) sfx_down_kernel(const fr* a, fr* out, uint64_t n, const fr* pw,
                                                                  const fr* carry, fr* total, const fr* lt) {
   constexpr int kPer = ScanGeom<LG>::kPer, kLgSeg = ScanGeom<LG>::kLgSeg, kLgPer = LG;
@@ -306,13 +308,18 @@ __global__ void __launch_bounds__(kScanThreads) sfx_down_kernel(const fr* a, fr*
   const uint64_t base = ((uint64_t)blockIdx.x << kLgSeg) + (uint64_t)t * kPer;
   uint32_t m[9];
   unpack_ext(fr_load(pw).l, m);  // balanced digits
-  fr e[kPer];
-  static_for<0, kPer>([&](auto k_) {
-    constexpr int k = decltype(k_)::value;
-    e[k] = base + k < n ? fr_load(a + base + k) : fr_zero();
-  });
   // local Horner: L = sum_k e_k m^k
-  fr h = lt ? fr_load(lt + ((uint64_t)blockIdx.x * kScanThreads + t)) : horner29<kPer>(e, m);
+  fr h;
+  if constexpr (LT) {
+    h = fr_load(lt + ((uint64_t)blockIdx.x * kScanThreads + t));
+  } else {
+    fr e[kPer];
+    static_for<0, kPer>([&](auto k_) {
+      constexpr int k = decltype(k_)::value;
+      e[k] = base + k < n ? fr_load(a + base + k) : fr_zero();
+    });
+    h = horner29<kPer>(e, m);
+  }
   // fold the block carry into the last thread: s(start of thread 255) = L + m^kPer K
   fr K = carry ? fr_load(carry + blockIdx.x) : fr_zero();
   if (t == kScanThreads - 1) h = fr_add(h, mulp(K, fr_load(pw + kLgPer)));
@@ -321,17 +328,28 @@ __global__ void __launch_bounds__(kScanThreads) sfx_down_kernel(const fr* a, fr*
   fr cin = t + 1 < kScanThreads ? lds_get(l, t + 1) : K;
   if (total && blockIdx.x == 0 && t == 0) fr_store(total, v);
   // rescan: s_k = e_k + m s_{k+1}, out[base + k] = s_{k+1} (signed limbs,
-  // canonical where stored)
+  // canonical where stored); out may equal a: each half's elements are read
+  // before its outputs are written, and the low half's outputs only overwrite
+  // elements already read
   v9 sn;
   unpack_ext(cin.l, sn.l);
-  static_for<0, kPer>([&](auto k_) {
-    constexpr int k = kPer - 1 - decltype(k_)::value;
-    if (base + k < n) fr_store(out + base + k, k == kPer - 1 ? cin : v9_canonical(sn));
-    if constexpr (k > 0) {
-      v9 x;
-      unpack_ext(e[k].l, x.l);
-      sn = v9_add(v9_mul<false, false, false>(sn, m), x);
-    }
+  constexpr int G = kPer > 1 ? 2 : 1, Q = kPer / G;
+  static_for<0, G>([&](auto g_) {
+    constexpr int top = kPer - 1 - decltype(g_)::value * Q;  // highest index of this half
+    fr e[Q];
+    static_for<0, Q>([&](auto j_) {
+      constexpr int k = top - decltype(j_)::value;
+      e[top - k] = k > 0 && base + k < n ? fr_load(a + base + k) : fr_zero();
+    });
+    static_for<0, Q>([&](auto j_) {
+      constexpr int k = top - decltype(j_)::value;
+      if (base + k < n) fr_store(out + base + k, k == kPer - 1 ? cin : v9_canonical(sn));
+      if constexpr (k > 0) {
+        v9 x;
+        unpack_ext(e[top - k].l, x.l);
+        sn = v9_add(v9_mul<false, false, false>(sn, m), x);
+      }
+    });
   });
 }
 
@@ -390,10 +408,10 @@ static hipError_t suffix_scan(const fr* a, fr* out, uint64_t n, const fr* pw, fr
     const bool tiny = n <= (uint64_t)kScanThreads;
     if (out) {
       if (tiny)
-        hipLaunchKernelGGL(sfx_down_kernel<0>, dim3(1), dim3(kScanThreads), 0, s, a, out, n, pw, (const fr*)nullptr,
+        hipLaunchKernelGGL((sfx_down_kernel<0, false>), dim3(1), dim3(kScanThreads), 0, s, a, out, n, pw, (const fr*)nullptr,
                            total, (const fr*)nullptr);
       else
-        hipLaunchKernelGGL(sfx_down_kernel<kLgPer>, dim3(1), dim3(kScanThreads), 0, s, a, out, n, pw,
+        hipLaunchKernelGGL((sfx_down_kernel<kLgPer, false>), dim3(1), dim3(kScanThreads), 0, s, a, out, n, pw,
                            (const fr*)nullptr, total, (const fr*)nullptr);
     } else {
       // one block: the reduce kernel's tree value is s_0
@@ -417,7 +435,7 @@ static hipError_t suffix_scan(const fr* a, fr* out, uint64_t n, const fr* pw, fr
   // carry[b] = s at the end of block b = sum_{b' > b} part[b'] (m^4096)^(b'-b-1)
   hipError_t e = suffix_scan(part, out ? carry : nullptr, nb, pw + kLgSeg, total, lt + nb * kScanThreads, s);
   if (e != hipSuccess || !out) return e;
-  hipLaunchKernelGGL(sfx_down_kernel<kLgPer>, dim3(sgrid(n)), dim3(kScanThreads), 0, s, a, out, n, pw,
+  hipLaunchKernelGGL((sfx_down_kernel<kLgPer, true>), dim3(sgrid(n)), dim3(kScanThreads), 0, s, a, out, n, pw,
                      (const fr*)carry, (fr*)nullptr, (const fr*)lt);
   return hipGetLastError();
 }
