@@ -807,20 +807,30 @@ __global__ void __launch_bounds__(256) gp_run_down_kernel(const fr* nv, const fr
     unpack_ext(fr_load(inv).l, w);
     cur = v9_mul<false, false, false>(cur, w);
   }
-  static_for<0, kOpPer>([&](auto k_) {
-    constexpr int k = decltype(k_)::value;
-    const bool live = base + k < n;
-    const fr nw = live ? fr_load(nv + base + k) : rp_one();  // read before out[] is written
-    if (live) {
-      uint32_t sk[9];
-      pbd_unpack(S[k].l, sk);
-      fr_store(out + base + k, v9_canonical(v9_mul<false, false, false>(cur, sk)));
-    }
-    if constexpr (k < kOpPer - 1) {
-      uint32_t x[9];
-      unpack_ext(nw.l, x);
-      cur = v9_mul<false, false, false>(cur, x);
-    }
+  // four quarters: each quarter's num values are loaded before its outputs
+  // are written (out may equal nv), so a thread waits for four loads, not
+  // eight, with few values live
+  constexpr int Q = kOpPer / 4;
+  static_for<0, 4>([&](auto g_) {
+    constexpr int k0 = decltype(g_)::value * Q;
+    fr nw[Q];
+    static_for<0, Q>([&](auto j_) {
+      constexpr int j = decltype(j_)::value;
+      nw[j] = base + k0 + j < n ? fr_load(nv + base + k0 + j) : rp_one();
+    });
+    static_for<0, Q>([&](auto j_) {
+      constexpr int j = decltype(j_)::value, k = k0 + j;
+      if (base + k < n) {
+        uint32_t sk[9];
+        pbd_unpack(S[k].l, sk);
+        fr_store(out + base + k, v9_canonical(v9_mul<false, false, false>(cur, sk)));
+      }
+      if constexpr (k < kOpPer - 1) {
+        uint32_t x[9];
+        unpack_ext(nw[j].l, x);
+        cur = v9_mul<false, false, false>(cur, x);
+      }
+    });
   });
 }
 
