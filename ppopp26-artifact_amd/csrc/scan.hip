@@ -231,16 +231,41 @@ __device__ __forceinline__ fr horner29(const fr (&v)[K], const uint32_t (&w9)[9]
   return v9_canonical(h);
 }
 
+// The powers table beyond level 0: pw[from + j] = pw[from + j - 1]^2 for
+// j < count - from.  A multi-level scan's first reduce launch runs it in one
+// extra block (block 0, dispatched first) beside the reduction; only the
+// levels above read those entries.
+struct PwExt {
+  fr* pw;
+  int from, count;  // count == 0: no extension
+};
+__device__ void pw_extend(const PwExt& e) {
+  fr m = fr_load(e.pw + e.from - 1);
+  for (int j = e.from; j < e.count; j++) {
+    m = mulp(m, m);
+    fr_store(e.pw + j, m);
+  }
+}
+
 // partial[b] = sum over block b of a_k m^(k - start).  The sum is
 // order-free: thread t takes elements start + 256 j + t (coalesced loads),
 // H_t = sum_j a m^(256 j) by Horner in m^256, then m^t H_t; a plain sum over
 // the block finishes it.
 template <int LG>
-__global__ void __launch_bounds__(kScanThreads) sfx_reduce_kernel(const fr* a, uint64_t n, const fr* pw, fr* partial) {
+__global__ void __launch_bounds__(kScanThreads) sfx_reduce_kernel(const fr* a, uint64_t n, const fr* pw, fr* partial,
+                                                                   PwExt ext) {
   constexpr int kPer = ScanGeom<LG>::kPer, kLgSeg = ScanGeom<LG>::kLgSeg;
   __shared__ uint4 l[2][kScanThreads];
   const int t = threadIdx.x;
-  const uint64_t b0 = (uint64_t)blockIdx.x << kLgSeg;
+  uint32_t blk = blockIdx.x;
+  if (ext.count) {
+    if (blk == 0) {
+      if (t == 0) pw_extend(ext);
+      return;
+    }
+    blk--;
+  }
+  const uint64_t b0 = (uint64_t)blk << kLgSeg;
   uint32_t M[9];
   unpack_ext(fr_load(pw + 8).l, M);  // m^256, balanced digits
   fr v[kPer];
@@ -268,7 +293,7 @@ __global__ void __launch_bounds__(kScanThreads) sfx_reduce_kernel(const fr* a, u
     }
     __syncthreads();
   }
-  if (t == 0) fr_store(partial + blockIdx.x, h);
+  if (t == 0) fr_store(partial + blk, h);
 }
 
 // partial[b] as sfx_reduce_kernel computes it, from contiguous runs (thread t
@@ -276,10 +301,18 @@ __global__ void __launch_bounds__(kScanThreads) sfx_reduce_kernel(const fr* a, u
 // goes to lt for the down-sweep (which then skips its own), then the block
 // suffix over the L_t gives the block's value at thread 0.
 __global__ void __launch_bounds__(kScanThreads) sfx_run_reduce_kernel(const fr* a, uint64_t n, const fr* pw,
-                                                                      fr* partial, fr* lt) {
+                                                                      fr* partial, fr* lt, PwExt ext) {
   __shared__ uint4 l[2][kScanThreads];
   const int t = threadIdx.x;
-  const uint64_t tid = (uint64_t)blockIdx.x * kScanThreads + t;
+  uint32_t blk = blockIdx.x;
+  if (ext.count) {
+    if (blk == 0) {
+      if (t == 0) pw_extend(ext);
+      return;
+    }
+    blk--;
+  }
+  const uint64_t tid = (uint64_t)blk * kScanThreads + t;
   const uint64_t base = tid * kPer;
   uint32_t m[9];
   unpack_ext(fr_load(pw).l, m);  // balanced digits
@@ -291,7 +324,7 @@ __global__ void __launch_bounds__(kScanThreads) sfx_run_reduce_kernel(const fr* 
   const fr h = horner29<kPer>(e, m);
   fr_store(lt + tid, h);
   const fr v = block_suffix<kLgPer>(h, l, pw);
-  if (t == 0) fr_store(partial + blockIdx.x, v);
+  if (t == 0) fr_store(partial + blk, v);
 }
 
 // out[i] = s_{i+1} within each block given carry[b] = s at the block's end
@@ -390,6 +423,8 @@ static int sfx_levels(uint64_t n) {
   return L;
 }
 static int sfx_pw_count(uint64_t n) { return kPw + kLgSeg * (sfx_levels(n) - 1); }
+// the entries beyond level 0's, for the first reduce launch (none for one level)
+static PwExt pw_ext(fr* pw, int np) { return np > kPw ? PwExt{pw, kPw, np} : PwExt{nullptr, 0, 0}; }
 
 static uint64_t sfx_scratch_elems(uint64_t n) {
   uint64_t tot = (uint64_t)sfx_pw_count(n);
@@ -403,7 +438,8 @@ static uint64_t sfx_scratch_elems(uint64_t n) {
 // suffix scan with powers table pw (this level's m): out[i] = s_{i+1} (if out),
 // *total = s_0 (if total).  out == nullptr: reduction only.  A top level of at
 // most 256 values runs one element per thread.
-static hipError_t suffix_scan(const fr* a, fr* out, uint64_t n, const fr* pw, fr* total, fr* scratch, hipStream_t s) {
+static hipError_t suffix_scan(const fr* a, fr* out, uint64_t n, const fr* pw, fr* total, fr* scratch, hipStream_t s,
+                              PwExt ext = PwExt{nullptr, 0, 0}) {
   if (n <= (uint64_t)kSeg) {
     const bool tiny = n <= (uint64_t)kScanThreads;
     if (out) {
@@ -415,10 +451,11 @@ static hipError_t suffix_scan(const fr* a, fr* out, uint64_t n, const fr* pw, fr
                            (const fr*)nullptr, total, (const fr*)nullptr);
     } else {
       // one block: the reduce kernel's tree value is s_0
+      const PwExt none{nullptr, 0, 0};
       if (tiny)
-        hipLaunchKernelGGL(sfx_reduce_kernel<0>, dim3(1), dim3(kScanThreads), 0, s, a, n, pw, total);
+        hipLaunchKernelGGL(sfx_reduce_kernel<0>, dim3(1), dim3(kScanThreads), 0, s, a, n, pw, total, none);
       else
-        hipLaunchKernelGGL(sfx_reduce_kernel<kLgPer>, dim3(1), dim3(kScanThreads), 0, s, a, n, pw, total);
+        hipLaunchKernelGGL(sfx_reduce_kernel<kLgPer>, dim3(1), dim3(kScanThreads), 0, s, a, n, pw, total, none);
     }
     return hipGetLastError();
   }
@@ -428,10 +465,11 @@ static hipError_t suffix_scan(const fr* a, fr* out, uint64_t n, const fr* pw, fr
   fr* lt = scratch + 2 * nb;
   // with a down-sweep, the reduce runs over the down-sweep's runs and keeps
   // their local Horner values; a reduction alone reads strided (coalesced)
+  const unsigned grid = sgrid(n) + (ext.count ? 1 : 0);
   if (out)
-    hipLaunchKernelGGL(sfx_run_reduce_kernel, dim3(sgrid(n)), dim3(kScanThreads), 0, s, a, n, pw, part, lt);
+    hipLaunchKernelGGL(sfx_run_reduce_kernel, dim3(grid), dim3(kScanThreads), 0, s, a, n, pw, part, lt, ext);
   else
-    hipLaunchKernelGGL(sfx_reduce_kernel<kLgPer>, dim3(sgrid(n)), dim3(kScanThreads), 0, s, a, n, pw, part);
+    hipLaunchKernelGGL(sfx_reduce_kernel<kLgPer>, dim3(grid), dim3(kScanThreads), 0, s, a, n, pw, part, ext);
   // carry[b] = s at the end of block b = sum_{b' > b} part[b'] (m^4096)^(b'-b-1)
   hipError_t e = suffix_scan(part, out ? carry : nullptr, nb, pw + kLgSeg, total, lt + nb * kScanThreads, s);
   if (e != hipSuccess || !out) return e;
@@ -456,15 +494,15 @@ hipError_t grand_product(fr* inout, fr* incl_out, uint64_t n, fr* scratch, hipSt
 hipError_t div_linear(fr* inout, uint64_t n, const fr* c, fr* rem, fr* scratch, hipStream_t s) {
   fr* pw = scratch;
   const int np = sfx_pw_count(n);
-  hipLaunchKernelGGL(sfx_powers_kernel, dim3(1), dim3(64), 0, s, c, 1, pw, np);
-  return suffix_scan(inout, inout, n, pw, rem, scratch + np, s);
+  hipLaunchKernelGGL(sfx_powers_kernel, dim3(1), dim3(64), 0, s, c, 1, pw, std::min(np, kPw));
+  return suffix_scan(inout, inout, n, pw, rem, scratch + np, s, pw_ext(pw, np));
 }
 
 hipError_t evaluate(const fr* coeffs, uint64_t n, const fr* x, fr* y, fr* scratch, hipStream_t s) {
   fr* pw = scratch;
   const int np = sfx_pw_count(n);
-  hipLaunchKernelGGL(sfx_powers_kernel, dim3(1), dim3(64), 0, s, x, 0, pw, np);
-  return suffix_scan(coeffs, nullptr, n, pw, y, scratch + np, s);
+  hipLaunchKernelGGL(sfx_powers_kernel, dim3(1), dim3(64), 0, s, x, 0, pw, std::min(np, kPw));
+  return suffix_scan(coeffs, nullptr, n, pw, y, scratch + np, s, pw_ext(pw, np));
 }
 
 // one scan for an evaluation + division pair: q(X) = (p(X) - p(z)) / (X - z)
@@ -472,8 +510,8 @@ hipError_t evaluate(const fr* coeffs, uint64_t n, const fr* x, fr* y, fr* scratc
 hipError_t eval_div(const fr* in, fr* q, uint64_t n, const fr* z, fr* y, fr* scratch, hipStream_t s) {
   fr* pw = scratch;
   const int np = sfx_pw_count(n);
-  hipLaunchKernelGGL(sfx_powers_kernel, dim3(1), dim3(64), 0, s, z, 0, pw, np);
-  return suffix_scan(in, q, n, pw, y, scratch + np, s);
+  hipLaunchKernelGGL(sfx_powers_kernel, dim3(1), dim3(64), 0, s, z, 0, pw, std::min(np, kPw));
+  return suffix_scan(in, q, n, pw, y, scratch + np, s, pw_ext(pw, np));
 }
 
 }  // namespace pl
