@@ -69,6 +69,8 @@ int pl_ntt(const void* d_in, void* d_out, int lg, int inverse, const pl_domain* 
 /* npoly contiguous polynomials of size 2^lg in one launch sequence */
 int pl_ntt_batch(const void* d_in, void* d_out, int lg, uint32_t npoly, int inverse,
                  const pl_domain* d, void* d_work, pl_stream_t stream);
+/* workspace for pl_ntt_batch: npoly * 2^lg elements for lg >= 6 (any pass
+ * split of such a size may need it), 0 below */
 size_t pl_ntt_work_bytes(int lg, uint32_t npoly);
 
 /* Pass-level access to the same transform (the intermediate layout is this
@@ -76,10 +78,14 @@ size_t pl_ntt_work_bytes(int lg, uint32_t npoly);
  * words congruent to the field elements, not canonical Montgomery values;
  * the reference's per-chunk step with its own layout is
  * pl_ntt_stage_no_rotate below).  Running passes
- * 0 .. pl_ntt_num_passes(lg)-1 in order, each reading the previous pass's
+ * 0 .. pl_domain_num_passes(d)-1 in order, each reading the previous pass's
  * output, equals pl_ntt_batch.  Intermediate passes may run in place; the
- * last pass writes natural order and must not (d_in != d_out). */
+ * last pass writes natural order and must not (d_in != d_out).
+ * pl_ntt_num_passes(lg) is the pass count of a domain created now (the
+ * split in use, see pl_plan_set); a domain keeps the split it was built
+ * with. */
 int pl_ntt_num_passes(int lg);
+int pl_domain_num_passes(const pl_domain* d);
 int pl_ntt_pass(const void* d_in, void* d_out, int lg, uint32_t npoly, int pass, int inverse,
                 const pl_domain* d, pl_stream_t stream);
 
@@ -141,7 +147,7 @@ int pl_div_linear(void* d_inout, uint64_t n, const void* d_c, void* d_rem, void*
 /* ---- pass-split autotuner ----------------------------------------------------
  * (replaces the reference's manual segment sweep, NTT/LDE --bench-sweep,
  *  ntt.cu:1494-1500, lde.cu:1834-1846)
- * A size-2^lg transform runs as P passes of 2^m[p]-point columns (4 <= m <= 8,
+ * A size-2^lg transform runs as P passes of 2^m[p]-point columns (3 <= m <= 6,
  * sum = lg); every split gives the same exact result.  pl_autotune times the
  * candidate splits of each lg in [lg_lo, lg_hi] on the current device (a
  * batch of max(1, 2^22 / N) polynomials per call), keeps the fastest and
@@ -220,7 +226,11 @@ int pl_perm_grand_product(const void* const* d_w, const void* const* d_s, const 
  *               p = in (n = 2^lg coefficients), z a device element; out != in
  * pl_batch_run_plain runs the same items as one launch sequence per
  * polynomial on `stream` (the per-call entry points' kernels), for
- * comparison. */
+ * comparison.  Both read the items' inputs and points when they run, so the
+ * caller may rewrite them (a new challenge z per proof) between runs.  Runs
+ * of one batch share its scratch: the caller orders them (one stream, or
+ * events).  pl_batch_destroy waits for the batch's last run and its own
+ * streams, not for the rest of the device. */
 enum pl_batch_op { PL_BATCH_NTT = 0, PL_BATCH_INTT = 1, PL_BATCH_LDE = 2, PL_BATCH_EVAL_DIV = 3 };
 enum pl_batch_flags { PL_BATCH_GRAPH = 1 };
 typedef struct pl_batch_item {

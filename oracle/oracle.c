@@ -8,9 +8,12 @@
  * deliberately a different limb size and algorithm from the GPU's 8 x 32-bit
  * FIPS product so that the two cannot share a mistake.
  */
+#define _GNU_SOURCE
 #include "oracle.h"
+#include <pthread.h>
 #include <stdlib.h>
 #include <string.h>
+#include <unistd.h>
 
 typedef unsigned __int128 u128;
 
@@ -215,42 +218,166 @@ uint64_t or_seed(int op, int lg, int aux) {
   return 0x504C4E4B00000000ull ^ ((uint64_t)op << 40) ^ ((uint64_t)lg << 8) ^ (uint64_t)aux;
 }
 
+/* ---- host threads ------------------------------------------------------
+ * The transforms below split their loops over host threads (pthreads).  The
+ * arithmetic is exact, so the result does not depend on the thread count. */
+static int g_threads = 0;
+
+void or_set_threads(int t) { g_threads = t < 1 ? 1 : (t > 64 ? 64 : t); }
+
+int or_get_threads(void) {
+  if (g_threads > 0) return g_threads;
+  const char* e = getenv("OR_THREADS");
+  int t = e ? atoi(e) : (int)sysconf(_SC_NPROCESSORS_ONLN);
+  if (t < 1) t = 1;
+  if (t > 16) t = 16;
+  return t;
+}
+
+typedef void (*range_fn)(void* ctx, uint64_t lo, uint64_t hi);
+struct par_job {
+  range_fn fn;
+  void* ctx;
+  uint64_t lo, hi;
+};
+
+static void* par_entry(void* p) {
+  struct par_job* j = (struct par_job*)p;
+  if (j->lo < j->hi) j->fn(j->ctx, j->lo, j->hi);
+  return NULL;
+}
+
+/* fn(ctx, lo, hi) over [0, n) in contiguous ranges, one per thread; ranges
+ * are multiples of `grain` except the last.  `cost`: elements touched per
+ * index, so that small loops stay on the calling thread. */
+static void par_for_cost(uint64_t n, uint64_t grain, uint64_t cost, range_fn fn, void* ctx) {
+  int T = or_get_threads();
+  if (grain == 0) grain = 1;
+  uint64_t units = (n + grain - 1) / grain;
+  if ((uint64_t)T > units) T = (int)units;
+  if (T <= 1 || n * cost < 8192) {
+    if (n) fn(ctx, 0, n);
+    return;
+  }
+  pthread_t th[64];
+  struct par_job jobs[64];
+  for (int t = 0; t < T; t++) {
+    uint64_t lo = units * t / T * grain, hi = units * (t + 1) / T * grain;
+    if (hi > n) hi = n;
+    jobs[t].fn = fn;
+    jobs[t].ctx = ctx;
+    jobs[t].lo = lo;
+    jobs[t].hi = hi;
+  }
+  int started[64] = {0};
+  for (int t = 1; t < T; t++) started[t] = pthread_create(&th[t], NULL, par_entry, &jobs[t]) == 0;
+  par_entry(&jobs[0]);
+  for (int t = 1; t < T; t++) {
+    if (started[t])
+      pthread_join(th[t], NULL);
+    else
+      par_entry(&jobs[t]); /* could not start a thread: run the range here */
+  }
+}
+
+static void par_for(uint64_t n, uint64_t grain, range_fn fn, void* ctx) { par_for_cost(n, grain, 1, fn, ctx); }
+
 /* ---- NTT: bit-reverse, then radix-2 DIT (natural in, natural out) ------ */
-static void bitrev_permute(uint64_t* d, int lg) {
-  uint64_t n = 1ull << lg;
-  for (uint64_t i = 0; i < n; i++) {
-    uint64_t j = or_bitrev(i, lg);
+struct bitrev_ctx {
+  uint64_t* d;
+  int lg;
+};
+static void bitrev_range(void* c, uint64_t lo, uint64_t hi) {
+  struct bitrev_ctx* x = (struct bitrev_ctx*)c;
+  for (uint64_t i = lo; i < hi; i++) {
+    uint64_t j = or_bitrev(i, x->lg);
     if (j > i) {
       uint64_t t[4];
-      memcpy(t, d + 4 * i, 32);
-      memcpy(d + 4 * i, d + 4 * j, 32);
-      memcpy(d + 4 * j, t, 32);
+      memcpy(t, x->d + 4 * i, 32);
+      memcpy(x->d + 4 * i, x->d + 4 * j, 32);
+      memcpy(x->d + 4 * j, t, 32);
     }
+  }
+}
+
+/* o[i] = scale * base^(i + start) for i in [lo, hi) (each range starts from
+ * one exponentiation, then steps by products) */
+struct powers_ctx {
+  uint64_t* o;
+  const uint64_t* base;
+  const uint64_t* scale; /* NULL: 1 */
+  uint64_t start;
+  int multiply; /* 1: o[i] *= scale * base^(i+start) in place */
+};
+static void powers_range(void* c, uint64_t lo, uint64_t hi) {
+  struct powers_ctx* x = (struct powers_ctx*)c;
+  uint64_t p[4], e[4] = {lo + x->start, 0, 0, 0};
+  or_pow(p, x->base, e);
+  if (x->scale) or_mul(p, p, x->scale);
+  for (uint64_t i = lo; i < hi; i++) {
+    if (x->multiply)
+      or_mul(x->o + 4 * i, x->o + 4 * i, p);
+    else
+      memcpy(x->o + 4 * i, p, 32);
+    or_mul(p, p, x->base);
+  }
+}
+
+struct stage_ctx {
+  uint64_t* d;
+  const uint64_t* tw;
+  uint64_t half;
+  int s;    /* stage (half block 2^s) for stage_range */
+  int lgB;  /* block_range: stages 0..lgB-1 on blocks of 2^lgB elements */
+};
+static void butterfly(uint64_t* a, uint64_t* b, const uint64_t* w) {
+  uint64_t t[4];
+  or_mul(t, b, w);
+  or_sub(b, a, t);
+  or_add(a, a, t);
+}
+/* stages 0 .. lgB-1 on each block of 2^lgB elements in [lo, hi) (block units) */
+static void block_range(void* c, uint64_t lo, uint64_t hi) {
+  struct stage_ctx* x = (struct stage_ctx*)c;
+  const uint64_t B = 1ull << x->lgB;
+  for (uint64_t b = lo; b < hi; b++) {
+    uint64_t* d = x->d + 4 * b * B;
+    for (int s = 0; s < x->lgB; s++) {
+      uint64_t h = 1ull << s, stride = x->half >> s;
+      for (uint64_t blk = 0; blk < B; blk += 2 * h)
+        for (uint64_t j = 0; j < h; j++) butterfly(d + 4 * (blk + j), d + 4 * (blk + j + h), x->tw + 4 * (j * stride));
+    }
+  }
+}
+/* butterflies [lo, hi) of stage s (butterfly i pairs blk + j and blk + j + h) */
+static void stage_range(void* c, uint64_t lo, uint64_t hi) {
+  struct stage_ctx* x = (struct stage_ctx*)c;
+  const int s = x->s;
+  const uint64_t h = 1ull << s, stride = x->half >> s;
+  for (uint64_t i = lo; i < hi; i++) {
+    uint64_t blk = (i >> s) << (s + 1), j = i & (h - 1);
+    butterfly(x->d + 4 * (blk + j), x->d + 4 * (blk + j + h), x->tw + 4 * (j * stride));
   }
 }
 
 static void ntt_with_root(uint64_t* d, int lg, const uint64_t* w) {
   uint64_t n = 1ull << lg;
   if (lg == 0) return;
-  bitrev_permute(d, lg);
+  struct bitrev_ctx bc = {d, lg};
+  par_for(n, 1, bitrev_range, &bc);
   /* tw[j] = w^j, j < n/2 */
   uint64_t half = n >> 1;
   uint64_t* tw = (uint64_t*)malloc(32 * half);
-  memcpy(tw, ONE, 32);
-  for (uint64_t j = 1; j < half; j++) or_mul(tw + 4 * j, tw + 4 * (j - 1), w);
-  for (int s = 0; s < lg; s++) {
-    uint64_t h = 1ull << s;           /* half block */
-    uint64_t stride = half >> s;      /* w_{2h}^j = w^(j*stride) */
-    for (uint64_t blk = 0; blk < n; blk += 2 * h) {
-      for (uint64_t j = 0; j < h; j++) {
-        uint64_t* a = d + 4 * (blk + j);
-        uint64_t* b = d + 4 * (blk + j + h);
-        uint64_t t[4];
-        or_mul(t, b, tw + 4 * (j * stride));
-        or_sub(b, a, t);
-        or_add(a, a, t);
-      }
-    }
+  struct powers_ctx pc = {tw, w, NULL, 0, 0};
+  par_for(half, 1, powers_range, &pc);
+  /* the first stages block by block (2^12 elements = 128 KiB stay in cache),
+   * the rest one stage at a time over all butterflies */
+  int lgB = lg < 12 ? lg : 12;
+  struct stage_ctx sc = {d, tw, half, 0, lgB};
+  par_for_cost(n >> lgB, 1, 1ull << lgB, block_range, &sc);
+  for (int s = lgB; s < lg; s++) {
+    sc.s = s;
+    par_for(half, 1, stage_range, &sc);
   }
   free(tw);
 }
@@ -261,22 +388,36 @@ void or_ntt(uint64_t* data, int lg, int inverse) {
   if (inverse) or_inv(w, w);
   ntt_with_root(data, lg, w);
   if (inverse) {
-    uint64_t ninv[4];
+    uint64_t ninv[4], one[4];
     or_domain_size_inverse(ninv, lg);
-    uint64_t n = 1ull << lg;
-    for (uint64_t i = 0; i < n; i++) or_mul(data + 4 * i, data + 4 * i, ninv);
+    memcpy(one, ONE, 32);
+    /* data[i] *= ninv * 1^i */
+    struct powers_ctx pc = {data, one, ninv, 0, 1};
+    par_for(1ull << lg, 1, powers_range, &pc);
   }
 }
 
 void or_coset_ntt(uint64_t* data, int lg, const uint64_t* shift) {
-  uint64_t n = 1ull << lg;
-  uint64_t p[4];
-  memcpy(p, ONE, 32);
-  for (uint64_t i = 0; i < n; i++) {
-    or_mul(data + 4 * i, data + 4 * i, p);
-    or_mul(p, p, shift);
-  }
+  /* data[i] *= shift^i */
+  struct powers_ctx pc = {data, shift, NULL, 0, 1};
+  par_for(1ull << lg, 1, powers_range, &pc);
   or_ntt(data, lg, 0);
+}
+
+struct strided_ctx {
+  uint64_t* out;
+  const uint64_t* in;
+  uint64_t B, t, mask;
+};
+/* out[j*B + t] = in[j] */
+static void interleave_range(void* c, uint64_t lo, uint64_t hi) {
+  struct strided_ctx* x = (struct strided_ctx*)c;
+  for (uint64_t j = lo; j < hi; j++) memcpy(x->out + 4 * (j * x->B + x->t), x->in + 4 * j, 32);
+}
+/* out[k] = in[k & mask] */
+static void periodic_range(void* c, uint64_t lo, uint64_t hi) {
+  struct strided_ctx* x = (struct strided_ctx*)c;
+  for (uint64_t k = lo; k < hi; k++) memcpy(x->out + 4 * k, x->in + 4 * (k & x->mask), 32);
 }
 
 /* out[j*B + t] = coset_NTT_N(c; shift*w_M^t)[j], B = 2^lg_blowup */
@@ -291,7 +432,8 @@ void or_lde(const uint64_t* coeffs, uint64_t* out, int lg_n, int lg_blowup, cons
   for (uint64_t t = 0; t < B; t++) {
     memcpy(tmp, coeffs, 32 * n);
     or_coset_ntt(tmp, lg_n, st);
-    for (uint64_t j = 0; j < n; j++) memcpy(out + 4 * (j * B + t), tmp + 4 * j, 32);
+    struct strided_ctx ic = {out, tmp, B, t, 0};
+    par_for(n, 1, interleave_range, &ic);
     or_mul(st, st, wM);
   }
   free(tmp);
@@ -303,7 +445,8 @@ void or_naive_lde(const uint64_t* coeffs, uint64_t* out, int lg_n, int lg_blowup
   uint64_t* tmp = (uint64_t*)malloc(32 * n);
   memcpy(tmp, coeffs, 32 * n);
   or_ntt(tmp, lg_n, 0);
-  for (uint64_t k = 0; k < M; k++) memcpy(out + 4 * k, tmp + 4 * (k & (n - 1)), 32);
+  struct strided_ctx pc = {out, tmp, 0, 0, n - 1};
+  par_for(M, 1, periodic_range, &pc);
   free(tmp);
 }
 

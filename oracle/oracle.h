@@ -68,6 +68,12 @@ uint64_t or_bitrev(uint64_t v, int bits);
 void or_fill_uniform(uint64_t* out, uint64_t n, uint64_t seed);
 uint64_t or_seed(int op, int lg, int aux);
 
+/* host threads used by the transforms (or_ntt, or_coset_ntt, or_lde,
+ * or_naive_lde): default min(16, online CPUs) or $OR_THREADS; the results do
+ * not depend on it */
+void or_set_threads(int t);
+int or_get_threads(void);
+
 /* NTT, natural order in and out (Ntt::forward / Intt::forward,
  * PLONK/utils/function.cu:280-332).  In place. */
 void or_ntt(uint64_t* data, int lg, int inverse);

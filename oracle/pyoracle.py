@@ -49,6 +49,8 @@ def lib():
         L.or_fill_uniform.argtypes = [_u64p, ctypes.c_uint64, ctypes.c_uint64]
         L.or_seed.argtypes = [ctypes.c_int, ctypes.c_int, ctypes.c_int]
         L.or_seed.restype = ctypes.c_uint64
+        L.or_set_threads.argtypes = [ctypes.c_int]
+        L.or_get_threads.restype = ctypes.c_int
         L.or_ntt.argtypes = [_u64p, ctypes.c_int, ctypes.c_int]
         L.or_coset_ntt.argtypes = [_u64p, ctypes.c_int, _u64p]
         L.or_lde.argtypes = [_u64p, _u64p, ctypes.c_int, ctypes.c_int, _u64p]
@@ -163,6 +165,14 @@ def domain_size_inverse(lg):
     o = fr_array(1)
     lib().or_domain_size_inverse(P(o), lg)
     return o
+
+
+def set_threads(t):
+    lib().or_set_threads(int(t))
+
+
+def get_threads():
+    return lib().or_get_threads()
 
 
 def ntt(x, lg, inverse=False):

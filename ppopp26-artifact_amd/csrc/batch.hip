@@ -18,6 +18,8 @@ struct pl_batch {
   int n_kernels = 0;
   hipStream_t s_main = nullptr, s_side = nullptr;
   hipEvent_t ev_fork = nullptr, ev_join = nullptr, ev_a = nullptr, ev_b = nullptr;
+  hipEvent_t ev_done = nullptr;  // recorded on the caller's stream after each run
+  bool ran = false;
   hipGraph_t graph = nullptr;
   hipGraphExec_t exec = nullptr;
 };
@@ -25,9 +27,14 @@ struct pl_batch {
 namespace pl {
 
 static void batch_free(pl_batch& b) {
+  // the batch's own work: its streams and the last run on a caller's stream
+  // (not the whole device, which other callers' streams share)
+  if (b.ran && b.ev_done) (void)hipEventSynchronize(b.ev_done);
+  if (b.s_main) (void)hipStreamSynchronize(b.s_main);
+  if (b.s_side) (void)hipStreamSynchronize(b.s_side);
   if (b.exec) (void)hipGraphExecDestroy(b.exec);
   if (b.graph) (void)hipGraphDestroy(b.graph);
-  for (hipEvent_t ev : {b.ev_fork, b.ev_join, b.ev_a, b.ev_b})
+  for (hipEvent_t ev : {b.ev_fork, b.ev_join, b.ev_a, b.ev_b, b.ev_done})
     if (ev) (void)hipEventDestroy(ev);
   if (b.s_main) (void)hipStreamDestroy(b.s_main);
   if (b.s_side) (void)hipStreamDestroy(b.s_side);
@@ -54,12 +61,15 @@ static hipError_t batch_build(pl_batch& b, const pl_batch_item* items, int n, bo
   real.base = (char*)b.block;
   real.cap = b.bytes;
   if ((e = phase_plan(b.plan, its, real)) != hipSuccess) return e;
-  if ((e = phase_upload(b.plan, nullptr)) != hipSuccess) return e;
   if ((e = hipStreamCreateWithFlags(&b.s_main, hipStreamNonBlocking)) != hipSuccess) return e;
   if ((e = hipStreamCreateWithFlags(&b.s_side, hipStreamNonBlocking)) != hipSuccess) return e;
-  for (hipEvent_t* ev : {&b.ev_fork, &b.ev_join, &b.ev_a, &b.ev_b})
+  for (hipEvent_t* ev : {&b.ev_fork, &b.ev_join, &b.ev_a, &b.ev_b, &b.ev_done})
     if ((e = hipEventCreateWithFlags(ev, hipEventDisableTiming)) != hipSuccess) return e;
-  if ((e = hipDeviceSynchronize()) != hipSuccess) return e;
+  // the launch tables (and the domain tables the planner built on the null
+  // stream) are in place before the first launch
+  if ((e = phase_upload(b.plan, b.s_main)) != hipSuccess) return e;
+  if ((e = hipStreamSynchronize(b.s_main)) != hipSuccess) return e;
+  if ((e = hipStreamSynchronize(nullptr)) != hipSuccess) return e;
   // capture once: the node count is reported, and kept as the run form when
   // a graph was asked for
   if ((e = hipStreamBeginCapture(b.s_main, hipStreamCaptureModeThreadLocal)) != hipSuccess) return e;
@@ -82,13 +92,26 @@ static hipError_t batch_build(pl_batch& b, const pl_batch_item* items, int n, bo
 }
 
 static hipError_t batch_run(pl_batch& b, hipStream_t s) {
-  if (b.exec) return hipGraphLaunch(b.exec, s);
-  // the same launches, ordered after and before the caller's stream
-  hipError_t e = hipEventRecord(b.ev_a, s);
-  if (e == hipSuccess) e = hipStreamWaitEvent(b.s_main, b.ev_a, 0);
-  if (e == hipSuccess) e = phase_enqueue(b.plan, b.s_main, b.s_side, b.ev_fork, b.ev_join);
-  if (e == hipSuccess) e = hipEventRecord(b.ev_b, b.s_main);
-  if (e == hipSuccess) e = hipStreamWaitEvent(s, b.ev_b, 0);
+  hipError_t e;
+  if (b.exec) {
+    e = hipGraphLaunch(b.exec, s);
+  } else {
+    // the same launches, ordered after and before the caller's stream
+    e = hipEventRecord(b.ev_a, s);
+    if (e == hipSuccess) e = hipStreamWaitEvent(b.s_main, b.ev_a, 0);
+    if (e == hipSuccess) e = phase_enqueue(b.plan, b.s_main, b.s_side, b.ev_fork, b.ev_join);
+    if (e == hipSuccess) e = hipEventRecord(b.ev_b, b.s_main);
+    if (e == hipSuccess) e = hipStreamWaitEvent(s, b.ev_b, 0);
+  }
+  if (e == hipSuccess) e = hipEventRecord(b.ev_done, s);
+  if (e == hipSuccess) b.ran = true;
+  return e;
+}
+
+static hipError_t batch_run_plain(pl_batch& b, hipStream_t s) {
+  hipError_t e = phase_enqueue_plain(b.plan, s);
+  if (e == hipSuccess) e = hipEventRecord(b.ev_done, s);
+  if (e == hipSuccess) b.ran = true;
   return e;
 }
 
@@ -133,7 +156,6 @@ int pl_batch_create(const pl_batch_item* items, int n_items, int flags, pl_batch
   (void)hipGetDevice(&b->device);
   hipError_t e = pl::batch_build(*b, items, n_items, (flags & PL_BATCH_GRAPH) != 0);
   if (e != hipSuccess) {
-    (void)hipDeviceSynchronize();
     pl::batch_free(*b);
     delete b;
     return pl::hip_status(e);
@@ -150,7 +172,7 @@ int pl_batch_run(pl_batch* b, pl_stream_t stream) {
 
 int pl_batch_run_plain(pl_batch* b, pl_stream_t stream) {
   if (!b || !device_matches(b)) return PL_ERR_ARG;
-  hipError_t e = pl::phase_enqueue_plain(b->plan, (hipStream_t)stream);
+  hipError_t e = pl::batch_run_plain(*b, (hipStream_t)stream);
   return e == hipSuccess ? PL_OK : pl::hip_status(e);
 }
 
@@ -163,7 +185,6 @@ int pl_batch_info(const pl_batch* b, int* n_launches, size_t* bytes) {
 
 void pl_batch_destroy(pl_batch* b) {
   if (!b) return;
-  (void)hipDeviceSynchronize();
   pl::batch_free(*b);
   delete b;
 }

@@ -3,7 +3,7 @@
 
 #include <new>
 
-#include "pl_internal.hpp"
+#include "ntt_pass.hpp"
 
 struct pl_domain {
   pl::Domain d;
@@ -83,10 +83,13 @@ int pl_domain_lg(const pl_domain* d) { return d ? d->d.lg : PL_ERR_ARG; }
 
 size_t pl_domain_bytes(const pl_domain* d) { return d ? d->d.bytes : 0; }
 
+// Every split of lg >= 6 may have two or more passes (columns of 2^3..2^6
+// points), and such a transform needs an intermediate buffer; the size does
+// not depend on the split in use, so a workspace sized here fits a domain
+// built under any split.
 size_t pl_ntt_work_bytes(int lg, uint32_t npoly) {
   if (lg < 1 || lg > 28) return 0;
-  pl::NttPlan p = pl::make_plan(lg);
-  return p.P >= 2 ? ((size_t)npoly << lg) * sizeof(pl::fr) : 0;
+  return lg >= 2 * pl::kLgPerT ? ((size_t)npoly << lg) * sizeof(pl::fr) : 0;
 }
 
 int pl_ntt_batch(const void* d_in, void* d_out, int lg, uint32_t npoly, int inverse, const pl_domain* d,
@@ -95,7 +98,8 @@ int pl_ntt_batch(const void* d_in, void* d_out, int lg, uint32_t npoly, int inve
   if (d->d.lg != lg) return PL_ERR_ARG;
   if (!aligned16(d_in) || !aligned16(d_out) || !aligned16(d_work)) return PL_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
-  size_t wb = pl_ntt_work_bytes(lg, npoly);
+  // the domain's own split decides whether the passes need the buffer
+  size_t wb = d->d.plan.P >= 2 ? ((size_t)npoly << lg) * sizeof(pl::fr) : 0;
   pl::fr* tmp = (pl::fr*)d_work;
   bool own = false;
   if (wb && !tmp) {
@@ -119,6 +123,11 @@ int pl_ntt_num_passes(int lg) {
   if (lg < 1 || lg > 28) return PL_ERR_UNSUPPORTED;
   pl::NttPlan p = pl::make_plan(lg);
   return p.P == 0 ? 1 : p.P;
+}
+
+int pl_domain_num_passes(const pl_domain* d) {
+  if (!d) return PL_ERR_ARG;
+  return d->d.plan.P == 0 ? 1 : d->d.plan.P;
 }
 
 int pl_ntt_pass(const void* d_in, void* d_out, int lg, uint32_t npoly, int pass, int inverse, const pl_domain* d,

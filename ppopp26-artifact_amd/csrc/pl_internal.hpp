@@ -12,7 +12,7 @@ namespace pl {
 #define PL_MAX_PASSES 8
 
 // Four-step DIF decomposition of a size-2^lg NTT into passes of 2^m[p]
-// points per column (4 <= m <= 8).  Pass p works on sub-problems of size
+// points per column (3 <= m <= 6).  Pass p works on sub-problems of size
 // 2^lgM[p] (contiguous), doing column DFTs of size 2^m[p] at stride
 // 2^lgS[p] = 2^(lgM[p] - m[p]); the last pass has lgS = 0 and scatters its
 // output to the digit-reversed position.

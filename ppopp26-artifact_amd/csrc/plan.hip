@@ -172,15 +172,7 @@ hipError_t phase_upload(PhasePlan& ph, hipStream_t s) {
       e = hipMemcpyAsync(g.d_seg_block0, g.seg_block0.data(), g.seg_block0.size() * sizeof(uint32_t),
                          hipMemcpyHostToDevice, s);
   }
-  return e == hipSuccess ? phase_prepare(ph, s) : e;
-}
-
-hipError_t phase_prepare(const PhasePlan& ph, hipStream_t s) {
-  int k = 0;
-  for (auto& sc : ph.scans)
-    if (sc.op == IT_EVAL_DIV)
-      hipLaunchKernelGGL(neg_copy_kernel, dim3(1), dim3(64), 0, s, ph.items[sc.item].z, ph.negz_all + k++);
-  return hipGetLastError();
+  return e;
 }
 
 static void launch_ragged(const RaggedGroup& g, hipStream_t s) {
@@ -232,9 +224,15 @@ hipError_t phase_enqueue_plain(const PhasePlan& ph, hipStream_t s) {
       if ((e = perm_gp(it, sc, s)) != hipSuccess) return e;
       continue;
     }
-    // the reference's call pair: evaluate, then poly_div_poly by (X + c), c = -z
+    // the reference's call pair: evaluate, then poly_div_poly by (X + c),
+    // c = -z read from the item's point at run time (the caller may change
+    // it between runs)
     e = evaluate(it.in, sc.n, it.z, it.y, sc.scratch, s);
     if (e == hipSuccess) e = hipMemcpyAsync(it.out, it.in, sc.n * sizeof(fr), hipMemcpyDeviceToDevice, s);
+    if (e == hipSuccess) {
+      hipLaunchKernelGGL(neg_copy_kernel, dim3(1), dim3(64), 0, s, it.z, ph.negz_all + k);
+      e = hipGetLastError();
+    }
     if (e == hipSuccess) e = div_linear(it.out, sc.n, ph.negz_all + k, nullptr, sc.scratch, s);
     k++;
     if (e != hipSuccess) return e;

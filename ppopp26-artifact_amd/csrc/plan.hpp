@@ -79,18 +79,16 @@ struct PhasePlan {
   std::vector<std::pair<const Domain*, NttCall>> irregular;  // tiny / single-pass premultiplied forms
   std::vector<ScanJob> scans;
   std::vector<std::pair<const Domain*, NttCall>> plain_calls;  // per-item form of the transforms
-  fr* negz_all = nullptr;  // -z per EVAL_DIV item (plain form)
+  fr* negz_all = nullptr;  // -z per EVAL_DIV item (plain form; written at run time from the item's z)
 };
 
 // Plan one phase; with bump.base == nullptr only sizes (bump.off = bytes
 // needed).  Launch tables come from `tables` when given, else from `bump`.
 // Domains are looked up (and built) in the shared cache.
 hipError_t phase_plan(PhasePlan& ph, const std::vector<Item>& items, Bump& bump, Bump* tables = nullptr);
-// Copy the plan's launch tables to the device (stream-ordered on s) and
-// enqueue phase_prepare; the host copies stay in the plan.
+// Copy the plan's launch tables to the device (stream-ordered on s); the
+// host copies stay in the plan.
 hipError_t phase_upload(PhasePlan& ph, hipStream_t s);
-// Fill the plain form's -z table (one small kernel per evaluation item).
-hipError_t phase_prepare(const PhasePlan& ph, hipStream_t s);
 // Enqueue the batched form: ragged launches on `main`, scans on `side`
 // (forked and joined with the two events); no allocation.
 hipError_t phase_enqueue(const PhasePlan& ph, hipStream_t main, hipStream_t side, hipEvent_t fork, hipEvent_t join);

@@ -93,6 +93,7 @@ struct Job {
   double t_arr = 0, t_admit = -1, t_done = -1;
   int status = PL_OK;
   bool done = false;
+  bool holds_out = false;  // the output block [out_off, +out_bytes) is this job's until released
   size_t out_off = 0, out_bytes = 0;
   fr *coeffs = nullptr, *lde = nullptr, *z = nullptr, *y = nullptr, *q = nullptr;
   uint64_t n() const { return 1ull << d.lg; }
@@ -298,8 +299,7 @@ static hipError_t slot_launch(pl_server& S, Slot& s) {
   for (auto& ph : s.phases) {
     if (e != hipSuccess) break;
     if (plain) {
-      e = phase_prepare(ph, s.main);
-      if (e == hipSuccess) e = phase_enqueue_plain(ph, s.main);
+      e = phase_enqueue_plain(ph, s.main);
     } else {
       e = phase_enqueue(ph, s.main, s.side, s.fork, s.join);
     }
@@ -312,6 +312,17 @@ static hipError_t slot_launch(pl_server& S, Slot& s) {
 // free slot.  Called with the lock held.  Returns false when nothing started.
 static bool try_admit(pl_server& S) {
   if (S.queue.empty()) return false;
+  if (S.fatal != PL_OK) {
+    // the context reported an error: fail what is queued instead of launching
+    for (Job* j : S.queue) {
+      j->status = S.fatal;
+      j->done = true;
+      j->t_done = S.now();
+    }
+    S.queue.clear();
+    S.cv_done.notify_all();
+    return false;
+  }
   Slot* slot = nullptr;
   for (auto& s : S.slots)
     if (!s.busy) {
@@ -374,7 +385,10 @@ static bool try_admit(pl_server& S) {
       if (e == hipSuccess) {
         stage_tables(*slot);
         const double t = S.now();
-        for (Job* j : cand) j->t_admit = t;
+        for (Job* j : cand) {
+          j->t_admit = t;
+          j->holds_out = true;
+        }
         e = slot_launch(S, *slot);
       }
       if (e != hipSuccess) {
@@ -383,6 +397,7 @@ static bool try_admit(pl_server& S) {
         for (Job* j : cand) {
           j->status = S.fatal;
           j->done = true;
+          j->holds_out = false;  // released just below
           j->t_done = S.now();
         }
         S.arena.release(blk_off, blk_bytes);
@@ -439,6 +454,9 @@ static void worker_main(pl_server* Sp) {
       }
       const double t = S.now();
       const int st = q == hipSuccess ? PL_OK : hip_status(q);
+      // an error reported at completion leaves the context unusable: admit
+      // nothing more (submit and later admissions report it)
+      if (st != PL_OK && S.fatal == PL_OK) S.fatal = st;
       for (Job* j : s.jobs) {
         j->t_done = t;
         j->status = st;
@@ -582,7 +600,7 @@ int pl_server_release(pl_server* S, uint64_t job_id) {
   auto it = S->jobs.find(job_id);
   if (it == S->jobs.end() || !it->second->done) return PL_ERR_ARG;
   pl::Job* j = it->second;
-  if (j->status == PL_OK) S->arena.release(j->out_off, j->out_bytes);
+  if (j->holds_out) S->arena.release(j->out_off, j->out_bytes);
   S->jobs.erase(it);
   delete j;
   S->cv_work.notify_one();  // memory freed: admission may proceed
@@ -611,7 +629,12 @@ void pl_server_destroy(pl_server* S) {
     S->cv_work.notify_all();
   }
   S->worker.join();
-  (void)hipDeviceSynchronize();
+  // the worker leaves only when no slot is busy; wait for the slots' own
+  // streams, not the whole device
+  for (auto& s : S->slots) {
+    if (s.main) (void)hipStreamSynchronize(s.main);
+    if (s.side) (void)hipStreamSynchronize(s.side);
+  }
   for (auto& kv : S->jobs) delete kv.second;
   for (auto& s : S->slots) pl::slot_free(s);
   (void)hipFree(S->arena_base);

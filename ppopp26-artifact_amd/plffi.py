@@ -45,6 +45,7 @@ SIGNATURES = {
     "pl_ntt_batch": (_i, [_vp, _vp, _i, _u32, _i, _vp, _vp, _vp]),
     "pl_ntt_work_bytes": (ctypes.c_size_t, [_i, _u32]),
     "pl_ntt_num_passes": (_i, [_i]),
+    "pl_domain_num_passes": (_i, [_vp]),
     "pl_ntt_pass": (_i, [_vp, _vp, _i, _u32, _i, _i, _vp, _vp]),
     "pl_ntt_stage_no_rotate": (_i, [_vp, _u64, _i, _i, _i, _u64, _vp, _vp]),
     "pl_lde_domain_create": (_i, [_i, _i, _vp, ctypes.POINTER(_vp)]),
@@ -488,6 +489,28 @@ def stage_no_rotate(chunk, lg_domain, iterations, stage, chunk_id, params_buf, s
 
 def ntt_num_passes(lg):
     return lib().pl_ntt_num_passes(lg)
+
+
+def domain_num_passes(dom):
+    return lib().pl_domain_num_passes(dom.h)
+
+
+def ntt_naive(x, dom, out=None, stream=None):
+    """The workload baseline NTT (bit reversal + one radix-2 launch per stage)."""
+    import torch
+    if out is None:
+        out = torch.empty_like(x)
+    check("pl_ntt_naive", lib().pl_ntt_naive(_ptr(x), _ptr(out), dom.lg, dom.h, _stream(stream)))
+    return out
+
+
+def lde_naive(c, dom, lg_blowup, out=None, stream=None):
+    """naive_lde: out[k] = NTT_N(c)[k mod N] for k < N << lg_blowup (dom of size N)."""
+    import torch
+    if out is None:
+        out = torch.empty((c.shape[0] << lg_blowup, 4), dtype=torch.int64, device=c.device)
+    check("pl_lde_naive", lib().pl_lde_naive(_ptr(c), _ptr(out), dom.lg, lg_blowup, dom.h, _stream(stream)))
+    return out
 
 
 def ntt_work_bytes(lg, npoly=1):

@@ -20,3 +20,17 @@ def golden():
     def load(name):
         return np.load(os.path.join(GOLDEN, name), allow_pickle=False)
     return load
+
+
+def assert_fr_equal(got, want, what=""):
+    """Element-for-element equality of two (n, 4) uint64 arrays; on a
+    mismatch, names the first differing element and the count."""
+    got = np.asarray(got)
+    want = np.asarray(want)
+    assert got.shape == want.shape, "%s: shape %s vs %s" % (what, got.shape, want.shape)
+    bad = np.any(got != want, axis=-1) if got.ndim > 1 else (got != want)
+    if bad.any():
+        i = int(np.argmax(bad))
+        raise AssertionError("%s: %d of %d elements differ, first at %d: got %s want %s"
+                             % (what, int(bad.sum()), bad.size, i, [hex(int(v)) for v in np.atleast_1d(got[i])],
+                                [hex(int(v)) for v in np.atleast_1d(want[i])]))

@@ -48,7 +48,8 @@ def test_library_loads_and_host_functions():
     assert [L.pl_ntt_num_passes(lg) for lg in (1, 3, 4, 6, 7, 12, 13, 18, 19, 24, 25, 28)] == \
         [1, 1, 1, 1, 2, 2, 3, 3, 4, 4, 5, 5]
     assert L.pl_ntt_num_passes(29) == -3 and L.pl_ntt_num_passes(0) == -3
-    assert L.pl_ntt_work_bytes(6, 1) == 0          # single pass runs in place
+    assert L.pl_ntt_work_bytes(5, 1) == 0          # every split of 2^5 is one pass, in place
+    assert L.pl_ntt_work_bytes(6, 1) == 32 << 6    # 2^6 may be split 3+3 (size fixed by lg alone)
     assert L.pl_ntt_work_bytes(24, 1) == 32 << 24  # one intermediate buffer
     assert L.pl_ntt_work_bytes(12, 3) == 3 * (32 << 12)
     assert L.pl_lde_work_bytes(24, 2) == 32 << 26

@@ -176,3 +176,34 @@ def test_batch_argument_errors(pl):
     assert L.pl_batch_create(ctypes.cast(bad, ctypes.c_void_p), 1, 1, ctypes.byref(h)) == -3
     assert L.pl_batch_create(None, 1, 1, ctypes.byref(h)) == -1
     assert not h.value
+
+
+def test_batch_point_rewritten_between_runs(pl):
+    """Both forms read the evaluation point when they run: a new challenge
+    written after pl_batch_create gives p(z_new) and the quotient by
+    (X - z_new), against the oracle."""
+    import torch
+    lg = 10
+    a = synth.uniform(1 << lg, 321)
+    z0 = synth.uniform(1, 322)
+    z1 = synth.uniform(1, 323)
+    inp = pl.to_dev(a)
+    zt = pl.to_dev(z0)
+    items = [{"op": "eval_div", "lg": lg, "lg_blowup": 0, "inp": inp,
+              "out": torch.zeros_like(inp), "z": zt, "y": torch.zeros((1, 4), dtype=torch.int64, device="cuda")}]
+    for graph in (True, False):
+        b = pl.Batch(items, graph=graph)
+        for z in (z0, z1, z0):
+            zt.copy_(pl.to_dev(z))
+            y_ref = po.eval_(a, z)
+            q_ref, _ = po.div(a, po.neg(z))
+            for plain in (False, True):
+                print("rewritten point graph=%s plain=%s" % (graph, plain), flush=True)
+                items[0]["out"].zero_()
+                items[0]["y"].zero_()
+                torch.cuda.synchronize()
+                (b.run_plain if plain else b.run)()
+                torch.cuda.synchronize()
+                np.testing.assert_array_equal(pl.to_host(items[0]["y"]), y_ref)
+                np.testing.assert_array_equal(pl.to_host(items[0]["out"]), q_ref)
+        b.close()

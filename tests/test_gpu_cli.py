@@ -94,4 +94,4 @@ def test_ntt_cli_autotune(tmp_path):
     assert [int(r[0]) for r in rows] == [10, 11, 12]
     for r in rows:
         m = [int(v) for v in r[2:2 + int(r[1])]]
-        assert sum(m) == int(r[0]) and all(4 <= v <= 8 for v in m)
+        assert sum(m) == int(r[0]) and all(3 <= v <= 6 for v in m)

@@ -83,6 +83,25 @@ def test_ntt_roundtrip_large(pl, lg):
         np.testing.assert_array_equal(yh[k:k + 1], po.eval_(x, pt))
 
 
+@pytest.mark.parametrize("lg", [22, 24])
+def test_ntt_full_vs_oracle_large(pl, lg):
+    """Bit-exact over the whole vector at the benchmark sizes, forward and
+    inverse (the oracle's radix-2 transform runs on the host's threads)."""
+    from conftest import assert_fr_equal
+    import torch
+    dom = pl.Domain(lg)
+    x = po.uniform(1 << lg, po.seed(po.OP_NTT, lg))
+    xd = pl.to_dev(x)
+    print("ntt full lg=%d: GPU" % lg, flush=True)
+    y = pl.to_host(pl.ntt(xd, dom))
+    yi = pl.to_host(pl.ntt(xd, dom, inverse=True))
+    del xd
+    torch.cuda.empty_cache()
+    print("ntt full lg=%d: oracle on %d threads" % (lg, po.get_threads()), flush=True)
+    assert_fr_equal(y, po.ntt(x, lg), "ntt lg=%d" % lg)
+    assert_fr_equal(yi, po.ntt(x, lg, inverse=True), "intt lg=%d" % lg)
+
+
 def test_ntt_in_place(pl):
     lg = 14
     dom = pl.Domain(lg)

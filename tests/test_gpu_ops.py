@@ -29,13 +29,47 @@ def test_lde_golden_small(pl, golden):
             np.testing.assert_array_equal(pl.to_host(pl.lde(c, g)), k["lde_g_%d_%d" % (lg_n, lb)])
 
 
-@pytest.mark.parametrize("lg_n,lb", [(8, 1), (9, 2), (12, 2), (12, 3), (16, 2), (17, 1), (20, 2)])
+@pytest.mark.parametrize("lg_n,lb", [(8, 1), (9, 2), (12, 2), (12, 3), (16, 2), (17, 1), (20, 2), (20, 3),
+                                     (22, 2), (22, 3), (24, 2), (24, 3)])
 def test_lde_vs_oracle(pl, lg_n, lb):
-    print("lde", lg_n, lb, flush=True)
+    """our_lde bit-exact over the whole output, up to the benchmark sizes
+    (2^24 -> 2^26 and 2^27; the oracle's coset transforms run on the host's
+    threads)."""
+    from conftest import assert_fr_equal
+    import torch
+    print("lde %d %d: GPU" % (lg_n, lb), flush=True)
     c = po.uniform(1 << lg_n, po.seed(po.OP_LDE, lg_n, lb))
     dom = pl.LdeDomain(lg_n, lb)
     out = pl.to_host(pl.lde(pl.to_dev(c), dom))
-    np.testing.assert_array_equal(out, po.lde(c, lg_n, lb, po.ref_lde_shift(lb)))
+    dom.close()
+    torch.cuda.empty_cache()
+    print("lde %d %d: oracle" % (lg_n, lb), flush=True)
+    assert_fr_equal(out, po.lde(c, lg_n, lb, po.ref_lde_shift(lb)), "lde %d %d" % (lg_n, lb))
+
+
+@pytest.mark.parametrize("lg_n,lb", [(10, 1), (12, 3), (16, 2), (20, 2)])
+def test_lde_naive_vs_oracle(pl, lg_n, lb):
+    """naive_lde (lde.cu:1367-1490): out[k] = NTT_N(c)[k mod N]"""
+    c = po.uniform(1 << lg_n, po.seed(po.OP_LDE, lg_n, 100 + lb))
+    print("lde naive %d %d" % (lg_n, lb), flush=True)
+    out = pl.to_host(pl.lde_naive(pl.to_dev(c), pl.Domain(lg_n), lb))
+    np.testing.assert_array_equal(out, po.naive_lde(c, lg_n, lb))
+
+
+@pytest.mark.parametrize("lg_n,lb", [(12, 1), (20, 1), (20, 2), (22, 1), (22, 2)])
+def test_evals_to_coset_lde_pipeline(pl, lg_n, lb):
+    """The prover pipeline on evaluations: iNTT -> multiply coefficient i by
+    7^i -> zero-pad to 2^lb N -> NTT of size 2^lb N, i.e. pl_ntt(inverse)
+    then pl_lde with shift 7, against the oracle's composition."""
+    from conftest import assert_fr_equal
+    import torch
+    e = po.uniform(1 << lg_n, po.seed(po.OP_LDE, lg_n, 200 + lb))
+    print("evals -> coset lde %d %d" % (lg_n, lb), flush=True)
+    coeffs = pl.ntt(pl.to_dev(e), pl.Domain(lg_n), inverse=True)
+    out = pl.to_host(pl.lde(coeffs, pl.LdeDomain(lg_n, lb, shift=po.from_u64(7))))
+    torch.cuda.empty_cache()
+    want = po.lde(po.ntt(e, lg_n, inverse=True), lg_n, lb, po.from_u64(7))
+    assert_fr_equal(out, want, "evals->lde %d %d" % (lg_n, lb))
 
 
 @pytest.mark.parametrize("lg_n,lb", [(12, 2), (16, 3)])

@@ -55,3 +55,27 @@ def test_autotune_applies_and_caches(pl, tmp_path):
     assert time.time() - t0 < 5.0
     for lg, m in got.items():
         assert pl.plan_get(lg) == m
+
+
+def test_domain_keeps_its_split(pl):
+    """A domain built under one split keeps it after the process-wide split
+    changes: its pass count, the workspace size and a call with no workspace
+    (the library allocates by the domain's own split) stay consistent."""
+    lg = 6
+    x = po.uniform(1 << lg, po.seed(po.OP_NTT, lg, 5))
+    pl.plan_set(lg, [3, 3])
+    dom = pl.Domain(lg)
+    pl.plan_set(lg, [])
+    assert pl.ntt_num_passes(lg) == 1 and pl.domain_num_passes(dom) == 2
+    assert pl.ntt_work_bytes(lg) == 32 << lg
+    print("split kept lg=6", flush=True)
+    np.testing.assert_array_equal(pl.to_host(pl.ntt(pl.to_dev(x), dom)), po.ntt(x, lg))
+    np.testing.assert_array_equal(pl.to_host(pl.ntt(pl.to_dev(x), dom, inverse=True)), po.ntt(x, lg, inverse=True))
+    # passes 0 .. domain_num_passes - 1 equal the whole transform
+    import torch
+    xd = pl.to_dev(x)
+    mid = torch.empty_like(xd)
+    out = torch.empty_like(xd)
+    pl.ntt_pass(xd, mid, dom, 0)
+    pl.ntt_pass(mid, out, dom, 1)
+    np.testing.assert_array_equal(pl.to_host(out), po.ntt(x, lg))
