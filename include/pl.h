@@ -103,6 +103,63 @@ int pl_ntt_pass(const void* d_in, void* d_out, int lg, uint32_t npoly, int pass,
 int pl_ntt_stage_no_rotate(void* d_chunk, uint64_t chunk_elems, int lg_domain, int iterations, int stage,
                            uint64_t chunk_id, const void* d_params, pl_stream_t stream);
 
+/* The chunked flow's index algebra [ct_stage_indices, ntt.cu:168-178;
+ * rotate_stage_output_index, ntt.cu:190-195] (host functions): thread tid
+ * of a step (stage, iterations) owns the global pair (*idx0, *idx1 =
+ * *idx0 + 2^stage); a step's output element g belongs at g with bits
+ * [stage, stage + iterations) rotated right by one. */
+void pl_ct_stage_indices(uint64_t tid, int stage, int iterations, uint64_t* idx0, uint64_t* idx1);
+uint64_t pl_rotate_stage_output_index(uint64_t global_idx, int stage, int iterations);
+
+/* The chunk permutations as device kernels (chunk = chunk_elems elements,
+ * a power of two; buffers distinct):
+ *   pl_stage_gather      chunk[2t+b] = src[idx_b(chunk_id*chunk_elems/2 + t)]
+ *                        [permute_stage_input_chunk_range, ntt.cu:335-354]
+ *   pl_stage_scatter     dst[rotate(idx_b(...)) - (chunk_local ? chunk_id*chunk_elems : 0)] = chunk[2t+b]
+ *                        [scatter_stage_output_chunk_range, ntt.cu:593-614;
+ *                         scatter_stage_kernel, ntt.cu:647-700]
+ *   pl_stage_transition  dst[2t+b] = src[rotate_prev^-1(idx_b^next(...)) - chunk_id*chunk_elems]:
+ *                        a chunk left in one step's rotated order regathered
+ *                        for the next step [transition_stage_kernel, ntt.cu:702-756]
+ *   pl_bit_reverse       out[rev_lg(i)] = in[i], in != out [the host bit
+ *                        reversal of our_ntt: calculate_map1 +
+ *                        bit_rev_step1_parallel, function.cu:503-509,
+ *                        function.cuh:221-244] */
+int pl_stage_gather(const void* d_src, void* d_chunk, uint64_t chunk_elems, uint64_t chunk_id, int stage,
+                    int iterations, pl_stream_t stream);
+int pl_stage_scatter(const void* d_chunk, void* d_dst, uint64_t chunk_elems, uint64_t chunk_id, int stage,
+                     int iterations, int chunk_local, pl_stream_t stream);
+int pl_stage_transition(const void* d_src, void* d_dst, uint64_t chunk_elems, uint64_t chunk_id, int prev_stage,
+                        int prev_iterations, int next_stage, int next_iterations, pl_stream_t stream);
+int pl_bit_reverse(const void* d_in, void* d_out, int lg, pl_stream_t stream);
+
+/* The reference's segmented NTT flow, device-resident [our_ntt,
+ * ntt.cu:979-1037 with run_first_ntt_pass, ntt.cu:801-859, and
+ * run_second_ntt_pass_async_scatter, ntt.cu:861-977]: bit reversal, pass 1
+ * over chunks of 2^(lg_i + lg_k) consecutive elements (stage_no_rotate of
+ * lg_i/2 and lg_i - lg_i/2 stages with the transition between them and a
+ * chunk-local scatter), pass 2 per chunk (gather for stage lg_i, one step of
+ * lg_n - lg_i stages, scatter).  Forward NTT, natural order in and out, in
+ * place; every permutation is a device kernel.  The same split rules as the
+ * reference: lg_i/2, lg_i - lg_i/2 and lg_n - lg_i in 1..8 or 10.
+ * d_params: pl_params_fill; d_work: NULL or pl_ntt_segmented_work_bytes. */
+size_t pl_ntt_segmented_work_bytes(int lg_n, int lg_i, int lg_k);
+int pl_ntt_segmented(void* d_inout, int lg_n, int lg_i, int lg_k, const void* d_params, void* d_work,
+                     pl_stream_t stream);
+
+/* Host-to-host NTT [our_ntt's scope, ntt.cu:979-1037]: npoly polynomials of
+ * 2^lg elements in host memory (pinned for overlap) in, results out
+ * (h_out may equal h_in).  Two device slots and two copy streams beside
+ * `stream`: each polynomial's first pass runs on column groups as they
+ * arrive (2^lg_k groups, each a strided 2-D copy), its last pass on line
+ * groups whose natural-order outputs leave at once, and the next
+ * polynomial's copy-in overlaps the current one's transform and copy-out.
+ * Blocks until h_out holds every result.  d_work: NULL or
+ * pl_ntt_host_work_bytes(lg, npoly) bytes. */
+size_t pl_ntt_host_work_bytes(int lg, uint32_t npoly);
+int pl_ntt_host(const void* h_in, void* h_out, int lg, uint32_t npoly, int inverse, const pl_domain* d, int lg_k,
+                void* d_work, pl_stream_t stream);
+
 /* ---- coset low-degree extension ---------------------------------------
  * [our_lde, lde.cu:1267-1365 (lde_spread_powers_kernel lde.cu:623-648,
  *  make_lde_plan lde.cu:288-302)]

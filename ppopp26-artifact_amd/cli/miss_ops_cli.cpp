@@ -1,9 +1,13 @@
 // miss_ops command line (the reference's miss_ops.cu): grand product,
 // evaluation and division by (X + c), CPU baselines beside the GPU operators.
 //   miss_ops                      reference workload: N = 2^22, ../../input.bin
-//   miss_ops --lg <n> [--input <path>] [--perm]
+//   miss_ops --lg <n> [--input <path>] [--perm] [--dump <dir>]
 // --perm adds the PLONK permutation grand product (ratio over 3 witness and
 // 3 sigma columns, then the exclusive product), seeded inputs.
+// --dump writes the input and every operator's output of the last run to
+// <dir> as raw 32-byte elements (input.bin, naive_gp.bin, ours_gp.bin,
+// ours_eval.bin, naive_div.bin, ours_div.bin; with --perm also perm_cols.bin
+// (a, b, c, s1, s2, s3), perm_bg.bin (beta, gamma) and perm_z.bin).
 // The CPU baselines restate the reference's serial loops (miss_ops.cu:34-87)
 // over this library's host field arithmetic.
 #include <functional>
@@ -64,6 +68,14 @@ static SyncedMemory naive_div(int lg_N, SyncedMemory input, SyncedMemory divisor
   return quotient;
 }
 
+static void dump(const std::string& dir, const char* name, const void* p, size_t bytes) {
+  if (dir.empty()) return;
+  const std::string path = dir + "/" + name;
+  std::ofstream f(path, std::ios::binary);
+  f.write((const char*)p, (std::streamsize)bytes);
+  if (!f) throw std::runtime_error("cannot write " + path);
+}
+
 static float gpu_time(const std::function<void()>& fn) {
   cli::EventTimer t;
   t.start();
@@ -76,13 +88,15 @@ int main(int argc, char** argv) {
     int lg_N = 22;
     std::string file_path = "../../input.bin";
     bool perm = false;
+    std::string dump_dir;
     for (int i = 1; i < argc; i++) {
       std::string a(argv[i]);
       if (a == "--lg" && i + 1 < argc) lg_N = std::atoi(argv[++i]);
       else if (a == "--input" && i + 1 < argc) file_path = argv[++i];
       else if (a == "--perm") perm = true;
+      else if (a == "--dump" && i + 1 < argc) dump_dir = argv[++i];
       else {
-        std::cerr << "Usage: " << argv[0] << " [--lg <n>] [--input <path>] [--perm]\n";
+        std::cerr << "Usage: " << argv[0] << " [--lg <n>] [--input <path>] [--perm] [--dump <dir>]\n";
         return 1;
       }
     }
@@ -101,6 +115,10 @@ int main(int argc, char** argv) {
       auto t1 = std::chrono::high_resolution_clock::now();
       std::cout << "[naive_gp] execution time: "
                 << std::chrono::duration_cast<std::chrono::milliseconds>(t1 - t0).count() << " ms" << std::endl;
+      if (rep == 2) {
+        dump(dump_dir, "input.bin", input.cpu_data(), N * 32);
+        dump(dump_dir, "naive_gp.bin", out.cpu_data(), N * 32);
+      }
     }
     for (int rep = 0; rep < 3; rep++) {
       cli::read_file(file_path.c_str(), inp_, N * 32, seed);
@@ -108,6 +126,7 @@ int main(int argc, char** argv) {
       std::cout << "[ours] run " << (rep + 1) << "/3" << std::endl;
       float ms = gpu_time([&]() { accumulate_mul_poly(input); });
       std::cout << "[ours_gp] execution time: " << ms << " ms" << std::endl;
+      if (rep == 2) dump(dump_dir, "ours_gp.bin", input.cpu_data(), N * 32);
     }
 
     if (perm) {
@@ -142,6 +161,15 @@ int main(int argc, char** argv) {
         });
         std::cout << "[ours_perm_gp] execution time: " << ms << " ms" << std::endl;
       }
+      if (!dump_dir.empty()) {
+        std::vector<uint64_t> h(4 * N * 6);
+        for (int j = 0; j < 6; j++)
+          cli::hipc(hipMemcpy(h.data() + 4 * N * j, cols[j], N * 32, hipMemcpyDeviceToHost), "D2H");
+        dump(dump_dir, "perm_cols.bin", h.data(), 6 * N * 32);
+        dump(dump_dir, "perm_bg.bin", bg, 64);
+        cli::hipc(hipMemcpy(h.data(), d_z, N * 32, hipMemcpyDeviceToHost), "D2H");
+        dump(dump_dir, "perm_z.bin", h.data(), N * 32);
+      }
       pl_domain_destroy(dom);
       for (void* p : cols) (void)hipFree(p);
       (void)hipFree(d_beta);
@@ -160,8 +188,10 @@ int main(int argc, char** argv) {
       input.mutable_gpu_data();
       point.mutable_gpu_data();
       std::cout << "[ours] run " << (rep + 1) << "/3" << std::endl;
-      float ms = gpu_time([&]() { SyncedMemory out = evaluate(input, point); });
+      SyncedMemory out(32);
+      float ms = gpu_time([&]() { out = evaluate(input, point); });
       std::cout << "[ours_eval] execution time: " << ms << " ms" << std::endl;
+      if (rep == 2) dump(dump_dir, "ours_eval.bin", out.cpu_data(), 32);
     }
 
     // division by X + c (miss_ops.cu:161-200)
@@ -179,6 +209,7 @@ int main(int argc, char** argv) {
       auto t1 = std::chrono::high_resolution_clock::now();
       std::cout << "[naive_div] execution time: "
                 << std::chrono::duration_cast<std::chrono::milliseconds>(t1 - t0).count() << " ms" << std::endl;
+      if (rep == 2) dump(dump_dir, "naive_div.bin", out.cpu_data(), (N - 1) * 32);
     }
     SyncedMemory divisor_gpu(32);
     std::memcpy(divisor_gpu.mutable_cpu_data(), divisorr, 32);
@@ -187,8 +218,10 @@ int main(int argc, char** argv) {
       cli::read_file(file_path.c_str(), inp_, N * 32, seed);
       input.mutable_gpu_data();
       std::cout << "[ours] run " << (rep + 1) << "/3" << std::endl;
-      float ms = gpu_time([&]() { SyncedMemory out = poly_div_poly(input, divisor_gpu); });
+      SyncedMemory out(32);
+      float ms = gpu_time([&]() { out = poly_div_poly(input, divisor_gpu); });
       std::cout << "[ours_div] execution time: " << ms << " ms" << std::endl;
+      if (rep == 2) dump(dump_dir, "ours_div.bin", out.cpu_data(), N * 32);
     }
     return 0;
   } catch (const std::exception& e) {

@@ -2,8 +2,11 @@
 //   NTT --bench-split <lg_N> | --bench-baseline <lg_N> | --check-split <lg_N>
 //       | --check-baseline <lg_N> | --bench-sweep <lg_N> [reps]
 // lg_N in {22, 24, 26}.  The input lives in pinned host memory as in the
-// reference; `our_ntt` copies it to HBM (2^lg_k chunks), runs the device
-// NTT and copies the result back, and the printed time covers all of it.
+// reference.  `our_ntt` is the host-to-host pipelined transform
+// (pl_ntt_host: 2^lg_k copy groups overlapping the first and last passes);
+// --check-split compares it with the reference's own chunked flow run on the
+// device (pl_ntt_segmented: bit reversal, stage_no_rotate steps, transition,
+// gathers and scatters), an independent set of kernels.
 #include <vector>
 
 #include "common.hpp"
@@ -24,28 +27,58 @@ static NttScaleConfig ntt_scale_config(int lg_N) {
   }
 }
 
-static void print_workspace() { std::cout << "[workspace] mode=device-resident, full-GPU vector buffer" << std::endl; }
+static void print_workspace() {
+  std::cout << "[workspace] mode=host-to-host pipeline, two device slots, grouped 2-D copies" << std::endl;
+}
 
 // our_ntt (ntt.cu:979-1037): natural order in and out, in place in the host
-// buffer of `input`.  lg_i / lg_j only shape the printed split; lg_k sets the
-// number of copy chunks.  Prints host-to-host and device-only times.
-static float our_ntt(SyncedMemory input, int lg_N, int lg_i, int lg_j, int lg_k, Ntt& ntt, void* d_buf,
-                     void* d_work, hipStream_t s) {
+// buffer of `input`; the printed time covers copies in, transform and copies
+// out, the scope of the reference's timer.  lg_i / lg_j only shape the
+// printed split; lg_k sets the copy groups.
+static float our_ntt(SyncedMemory input, int lg_N, int lg_i, int lg_j, int lg_k, Ntt& ntt, void* d_work,
+                     hipStream_t s) {
   (void)lg_i;
   (void)lg_j;
+  void* h = input.mutable_cpu_data();
+  cli::hipc(hipStreamSynchronize(s), "sync");
+  const auto t0 = std::chrono::steady_clock::now();
+  cli::check(pl_ntt_host(h, h, lg_N, 1, 0, ntt.domain(lg_N), lg_k, d_work, (pl_stream_t)s), "pl_ntt_host");
+  const float ms = std::chrono::duration<float, std::milli>(std::chrono::steady_clock::now() - t0).count();
+  std::cout << "our-ntt execution time: " << ms << " ms" << std::endl;
+  return ms;
+}
+
+// the reference's chunked flow on the device (pl_ntt_segmented), host to
+// host: the expected output of --check-split
+static void segmented_reference(SyncedMemory input, int lg_N, int lg_i, int lg_k, hipStream_t s) {
   const size_t bytes = (size_t)32 << lg_N;
   void* h = input.mutable_cpu_data();
-  cli::EventTimer all(s), dev(s);
-  cli::hipc(hipDeviceSynchronize(), "sync");
-  all.start();
-  cli::copy_chunked(d_buf, h, bytes, 1 << lg_k, hipMemcpyHostToDevice, s);
-  dev.start();
-  cli::check(pl_ntt(d_buf, d_buf, lg_N, 0, ntt.domain(lg_N), d_work, (pl_stream_t)s), "pl_ntt");
-  float dev_ms = dev.stop_ms();
-  cli::copy_chunked(h, d_buf, bytes, 1 << lg_k, hipMemcpyDeviceToHost, s);
-  float ms = all.stop_ms();
-  std::cout << "our-ntt execution time: " << ms << " ms" << std::endl;
-  std::cout << "[our_ntt] device-resident NTT: " << dev_ms << " ms" << std::endl;
+  void *d = nullptr, *params = nullptr;
+  cli::hipc(hipMalloc(&d, bytes), "hipMalloc");
+  cli::hipc(hipMalloc(&params, 4 * 16384 * 32), "hipMalloc");
+  cli::check(pl_params_fill(params, (pl_stream_t)s), "pl_params_fill");
+  cli::hipc(hipMemcpyAsync(d, h, bytes, hipMemcpyHostToDevice, s), "hipMemcpyAsync");
+  cli::check(pl_ntt_segmented(d, lg_N, lg_i, lg_k, params, nullptr, (pl_stream_t)s), "pl_ntt_segmented");
+  cli::hipc(hipMemcpyAsync(h, d, bytes, hipMemcpyDeviceToHost, s), "hipMemcpyAsync");
+  cli::hipc(hipStreamSynchronize(s), "sync");
+  (void)hipFree(d);
+  (void)hipFree(params);
+  std::cout << "[check] expected: the reference's chunked flow on the device, split " << lg_i << "+"
+            << (lg_N - lg_i) << ", 2^" << lg_k << " segments" << std::endl;
+}
+
+// device-resident NTT of the same size (HBM in, HBM out), for comparison
+static float device_resident_ms(int lg_N, Ntt& ntt, void* d_work, hipStream_t s) {
+  const size_t bytes = (size_t)32 << lg_N;
+  void* d = nullptr;
+  cli::hipc(hipMalloc(&d, bytes), "hipMalloc");
+  cli::check(pl_fill_uniform(d, 1ull << lg_N, cli::seed_of(1, lg_N, 0), nullptr, (pl_stream_t)s), "fill");
+  cli::check(pl_ntt(d, d, lg_N, 0, ntt.domain(lg_N), d_work, (pl_stream_t)s), "pl_ntt");
+  cli::EventTimer t(s);
+  t.start();
+  cli::check(pl_ntt(d, d, lg_N, 0, ntt.domain(lg_N), d_work, (pl_stream_t)s), "pl_ntt");
+  const float ms = t.stop_ms();
+  (void)hipFree(d);
   return ms;
 }
 
@@ -84,16 +117,24 @@ static int run_check(const NttScaleConfig& cfg, bool baseline) {
   SyncedMemory reference_input(N * 32), ours_input(N * 32);
   cli::read_file(cfg.input_file, reference_input.mutable_cpu_data(), N * 32, cli::seed_of(1, cfg.lg_N, 0));
   std::memcpy(ours_input.mutable_cpu_data(), reference_input.cpu_data(), N * 32);
-  SyncedMemory expected = ntt.forward(reference_input);
-  cli::hipc(hipDeviceSynchronize(), "sync");
-  expected.mutable_cpu_data();
-  DevBufs buf(N * 32);
+  SyncedMemory expected(N * 32);
   if (baseline) {
+    // baseline check: the library's NTT against the naive baseline
+    expected = ntt.forward(reference_input);
+    cli::hipc(hipDeviceSynchronize(), "sync");
+    expected.mutable_cpu_data();
+    DevBufs buf(N * 32);
     std::cout << "[workspace] mode=naive, bit reversal + one radix-2 launch per stage" << std::endl;
     baseline_ntt(ours_input, cfg.lg_N, cfg.ours_lg_k, ntt, buf.a, buf.b, nullptr);
   } else {
+    // split check: the reference's chunked flow (expected) against our_ntt
+    std::memcpy(expected.mutable_cpu_data(), reference_input.cpu_data(), N * 32);
+    segmented_reference(expected, cfg.lg_N, cfg.lg_i, cfg.ours_lg_k, nullptr);
     print_workspace();
-    our_ntt(ours_input, cfg.lg_N, cfg.lg_i, cfg.lg_N - cfg.lg_i, cfg.ours_lg_k, ntt, buf.a, buf.b, nullptr);
+    void* w = nullptr;
+    cli::hipc(hipMalloc(&w, pl_ntt_host_work_bytes(cfg.lg_N, 1)), "hipMalloc");
+    our_ntt(ours_input, cfg.lg_N, cfg.lg_i, cfg.lg_N - cfg.lg_i, cfg.ours_lg_k, ntt, w, nullptr);
+    (void)hipFree(w);
   }
   cli::hipc(hipDeviceSynchronize(), "sync");
   return cli::compare_outputs(expected.cpu_data(), ours_input.cpu_data(), N) ? 0 : 1;
@@ -106,14 +147,17 @@ static int run_bench(const NttScaleConfig& cfg, int k_begin, int k_end, int reps
   SyncedMemory input(N * 32);
   void* h = input.mutable_cpu_data();
   DevBufs buf(N * 32);
-  if (baseline)
+  void* w = nullptr;  // the pipeline's device slot (data + scratch)
+  if (baseline) {
     std::cout << "[workspace] mode=naive, bit reversal + one radix-2 launch per stage" << std::endl;
-  else
+  } else {
     print_workspace();
+    cli::hipc(hipMalloc(&w, pl_ntt_host_work_bytes(cfg.lg_N, 1)), "hipMalloc");
+  }
   for (int lg_k = k_begin; lg_k <= k_end; lg_k++) {
     const size_t chunk_bytes = ((size_t)1 << (cfg.lg_i + lg_k)) * 32;
     std::cout << "=== lg_N = " << cfg.lg_N << ", split = " << cfg.lg_i << "+" << lg_j
-              << ", mode = " << (baseline ? "baseline-naive" : "device-resident")
+              << ", mode = " << (baseline ? "baseline-naive" : "host-to-host pipeline")
               << ", chunk_size = " << cli::format_bytes(chunk_bytes) << " ===" << std::endl;
     for (int rep = 0; rep < reps; rep++) {
       cli::read_file(cfg.input_file, h, N * 32, cli::seed_of(1, cfg.lg_N, 0));
@@ -122,9 +166,14 @@ static int run_bench(const NttScaleConfig& cfg, int k_begin, int k_end, int reps
         baseline_ntt(input, cfg.lg_N, lg_k, ntt, buf.a, buf.b, nullptr);
       } else {
         std::cout << "[our_ntt] run " << (rep + 1) << "/" << reps << std::endl;
-        our_ntt(input, cfg.lg_N, cfg.lg_i, lg_j, lg_k, ntt, buf.a, buf.b, nullptr);
+        our_ntt(input, cfg.lg_N, cfg.lg_i, lg_j, lg_k, ntt, w, nullptr);
       }
     }
+  }
+  if (!baseline) {
+    (void)hipFree(w);
+    std::cout << "[our_ntt] device-resident NTT (HBM in, HBM out): " << device_resident_ms(cfg.lg_N, ntt, buf.b, nullptr)
+              << " ms" << std::endl;
   }
   return 0;
 }

@@ -5,10 +5,6 @@
 
 #include "ntt_pass.hpp"
 
-struct pl_domain {
-  pl::Domain d;
-};
-
 struct pl_lde_domain {
   pl::LdeDomain d;
 };

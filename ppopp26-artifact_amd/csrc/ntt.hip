@@ -376,6 +376,37 @@ hipError_t ntt_launch_one(const Domain& d, const NttCall& c, int p, const fr* in
   return hipGetLastError();
 }
 
+// Pass p of a single-polynomial transform on `nlines` of its lines: the
+// in/out pointers advanced by in_off/out_off elements (the pass twiddles
+// follow the output column).  Pass 0's lines are columns j at stride
+// N >> m[0], so a group of consecutive columns is in_off = out_off = j0; the
+// last pass's line lp writes lp + (N >> m[P-1]) k and reads sub-row
+// digit-reversed(lp), a bit permutation, so a group of lines g*W .. g*W+W-1
+// (W a power of two) is out_off = g*W, in_off = pi(g*W) << m[P-1].
+hipError_t ntt_launch_part(const Domain& d, const NttCall& c, int p, const fr* in, fr* out, uint64_t in_off,
+                           uint64_t out_off, uint64_t nlines, hipStream_t s) {
+  const NttPlan& P = d.plan;
+  if (P.P < 2 || c.premul || c.npoly != 1 || p < 0 || p >= P.P) return hipErrorInvalidValue;
+  PassArgs a;
+  fill_pass_args(d, c, p, in + in_off, 1ull << d.lg, nullptr, out + out_off, a);
+  if (a.tw) a.tw += out_off;
+  a.nlines = nlines;
+  launch_pass_la(P.m[p] - kLgPerT, c.inverse, p == P.P - 1, a, s);
+  return hipGetLastError();
+}
+
+// the last pass's sub-row of line lp (the digit reversal of the earlier
+// passes' radices, as the kernel undoes it)
+uint64_t last_pass_row(const NttPlan& P, uint64_t lp) {
+  uint64_t r = 0, q = lp;
+  for (int p = 0; p + 1 < P.P; p++) {
+    const int lr = P.m[p];
+    r = (r << lr) | (q & ((1ull << lr) - 1));
+    q >>= lr;
+  }
+  return r;
+}
+
 bool pass_args_regular(const Domain& d, const NttCall& c, int p, PassArgs& a) {
   const NttPlan& P = d.plan;
   if (P.P == 0 || (c.premul && P.P == 1) || p < 0 || p >= P.P) return false;

@@ -83,6 +83,12 @@ struct NttCall {
 hipError_t ntt_launch(const Domain& d, const NttCall& c, hipStream_t s);
 hipError_t ntt_launch_one(const Domain& d, const NttCall& c, int p, const fr* in, uint64_t in_pstride,
                           const fr* premul, fr* out, hipStream_t s);
+// pass p of a one-polynomial call over nlines of its lines, pointers advanced
+// by in_off / out_off elements (column groups of pass 0, line groups of the
+// last pass; see ntt.hip)
+hipError_t ntt_launch_part(const Domain& d, const NttCall& c, int p, const fr* in, fr* out, uint64_t in_off,
+                           uint64_t out_off, uint64_t nlines, hipStream_t s);
+uint64_t last_pass_row(const NttPlan& P, uint64_t lp);
 hipError_t domain_build(Domain& d, int lg, hipStream_t s);
 // maps a HIP error to a pl_status and records its text for pl_last_hip_error
 int hip_status(hipError_t e);
@@ -121,3 +127,8 @@ hipError_t mad_probe(double* tmad_per_s, hipStream_t s);
 hipError_t fill_pow_table(fr* out, uint64_t n, const host::fr64& base, const host::fr64& scale, hipStream_t s);
 
 }  // namespace pl
+
+// the C-ABI's opaque domain handle (include/pl.h)
+struct pl_domain {
+  pl::Domain d;
+};

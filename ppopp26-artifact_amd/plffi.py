@@ -48,6 +48,16 @@ SIGNATURES = {
     "pl_domain_num_passes": (_i, [_vp]),
     "pl_ntt_pass": (_i, [_vp, _vp, _i, _u32, _i, _i, _vp, _vp]),
     "pl_ntt_stage_no_rotate": (_i, [_vp, _u64, _i, _i, _i, _u64, _vp, _vp]),
+    "pl_ct_stage_indices": (None, [_u64, _i, _i, ctypes.POINTER(_u64), ctypes.POINTER(_u64)]),
+    "pl_rotate_stage_output_index": (_u64, [_u64, _i, _i]),
+    "pl_stage_gather": (_i, [_vp, _vp, _u64, _u64, _i, _i, _vp]),
+    "pl_stage_scatter": (_i, [_vp, _vp, _u64, _u64, _i, _i, _i, _vp]),
+    "pl_stage_transition": (_i, [_vp, _vp, _u64, _u64, _i, _i, _i, _i, _vp]),
+    "pl_bit_reverse": (_i, [_vp, _vp, _i, _vp]),
+    "pl_ntt_segmented_work_bytes": (ctypes.c_size_t, [_i, _i, _i]),
+    "pl_ntt_segmented": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp]),
+    "pl_ntt_host_work_bytes": (ctypes.c_size_t, [_i, _u32]),
+    "pl_ntt_host": (_i, [_vp, _vp, _i, _u32, _i, _vp, _i, _vp, _vp]),
     "pl_lde_domain_create": (_i, [_i, _i, _vp, ctypes.POINTER(_vp)]),
     "pl_lde_domain_destroy": (_i, [_vp]),
     "pl_lde_domain_bytes": (ctypes.c_size_t, [_vp]),
@@ -485,6 +495,63 @@ def stage_no_rotate(chunk, lg_domain, iterations, stage, chunk_id, params_buf, s
           lib().pl_ntt_stage_no_rotate(_ptr(chunk), chunk.shape[0], lg_domain, iterations, stage, chunk_id,
                                        _ptr(params_buf), _stream(stream)))
     return chunk
+
+
+def ct_stage_indices(tid, stage, iterations):
+    """The chunked flow's element pair of thread tid (ntt.cu:168-178)."""
+    a, b = _u64(), _u64()
+    lib().pl_ct_stage_indices(tid, stage, iterations, ctypes.byref(a), ctypes.byref(b))
+    return a.value, b.value
+
+
+def rotate_stage_output_index(g, stage, iterations):
+    return lib().pl_rotate_stage_output_index(g, stage, iterations)
+
+
+def stage_gather(src, chunk, chunk_id, stage, iterations, stream=None):
+    check("pl_stage_gather", lib().pl_stage_gather(_ptr(src), _ptr(chunk), chunk.shape[0], chunk_id, stage,
+                                                   iterations, _stream(stream)))
+    return chunk
+
+
+def stage_scatter(chunk, dst, chunk_id, stage, iterations, chunk_local=False, stream=None):
+    check("pl_stage_scatter", lib().pl_stage_scatter(_ptr(chunk), _ptr(dst), chunk.shape[0], chunk_id, stage,
+                                                     iterations, 1 if chunk_local else 0, _stream(stream)))
+    return dst
+
+
+def stage_transition(src, dst, chunk_id, prev_stage, prev_it, next_stage, next_it, stream=None):
+    check("pl_stage_transition", lib().pl_stage_transition(_ptr(src), _ptr(dst), src.shape[0], chunk_id, prev_stage,
+                                                           prev_it, next_stage, next_it, _stream(stream)))
+    return dst
+
+
+def bit_reverse(x, out=None, stream=None):
+    import torch
+    if out is None:
+        out = torch.empty_like(x)
+    lg = x.shape[0].bit_length() - 1
+    check("pl_bit_reverse", lib().pl_bit_reverse(_ptr(x), _ptr(out), lg, _stream(stream)))
+    return out
+
+
+def ntt_segmented(x, lg_i, lg_k, params_buf, stream=None):
+    """The reference's chunked flow on the device (forward NTT, in place)."""
+    lg = x.shape[0].bit_length() - 1
+    check("pl_ntt_segmented", lib().pl_ntt_segmented(_ptr(x), lg, lg_i, lg_k, _ptr(params_buf), None,
+                                                     _stream(stream)))
+    return x
+
+
+def ntt_host(h, dom, lg_k=4, npoly=1, inverse=False, out=None, stream=None):
+    """Host-to-host NTT of npoly polynomials held in a host tensor (pinned for
+    overlap), shape (npoly * 2^lg, 4) int64; returns the output host tensor."""
+    import torch
+    if out is None:
+        out = torch.empty_like(h)
+    check("pl_ntt_host", lib().pl_ntt_host(ctypes.c_void_p(h.data_ptr()), ctypes.c_void_p(out.data_ptr()), dom.lg,
+                                           npoly, 1 if inverse else 0, dom.h, lg_k, None, _stream(stream)))
+    return out
 
 
 def ntt_num_passes(lg):

@@ -95,3 +95,67 @@ def test_ntt_cli_autotune(tmp_path):
     for r in rows:
         m = [int(v) for v in r[2:2 + int(r[1])]]
         assert sum(m) == int(r[0]) and all(3 <= v <= 6 for v in m)
+
+
+def test_ntt_check_split_uses_the_reference_flow():
+    """--check-split's expected output comes from the reference's chunked
+    flow on the device (pl_ntt_segmented), a different set of kernels from
+    our_ntt's pipelined passes, so a match says something."""
+    p = run(["NTT", "--check-split", "24"])
+    assert p.returncode == 0
+    assert "[check] expected: the reference's chunked flow on the device, split 14+10, 2^4 segments" in p.stdout
+    assert "outputs match for 16777216 elements" in p.stdout
+
+
+def test_miss_ops_cli_outputs_vs_oracle(tmp_path):
+    """Every miss_ops output, the CPU loops included (naive_gp, naive_div),
+    against the oracle, on the reference's own input file (the first 4096
+    elements of input.bin, read through the CLI's loader)."""
+    from conftest import GOLDEN
+    src = os.path.join(GOLDEN, "input_prefix.bin")
+    p = run(["miss_ops", "--lg", "12", "--perm", "--input", src, "--dump", str(tmp_path)])
+    assert p.returncode == 0
+    f = lambda n: load(os.path.join(tmp_path, n))  # noqa: E731
+    x = f("input.bin")
+    np.testing.assert_array_equal(x, load(src))
+    np.testing.assert_array_equal(f("naive_gp.bin"), po.naive_gp(x))
+    np.testing.assert_array_equal(f("ours_gp.bin"), po.grand_product(x))
+    point = np.array([[123456789, 987654321, 1122334455, 6677889900]], dtype=np.uint64)
+    np.testing.assert_array_equal(f("ours_eval.bin"), po.eval_(x, point))
+    c = np.array([[123456789123456789, 987654321987654321, 1122334455667788, 1122334455667788]], dtype=np.uint64)
+    np.testing.assert_array_equal(f("naive_div.bin"), po.naive_div(x, c, po.one()))
+    np.testing.assert_array_equal(f("ours_div.bin"), po.div(x, c)[0])
+    cols = f("perm_cols.bin").reshape(6, -1, 4)
+    bg = f("perm_bg.bin")
+    ks = [po.from_u64(1), po.from_u64(7), po.from_u64(13)]
+    ratio = po.perm_ratio([np.ascontiguousarray(cols[j]) for j in range(6)], bg[0:1], bg[1:2], ks, 12)
+    np.testing.assert_array_equal(f("perm_z.bin"), po.grand_product(ratio))
+    # the reference's own blst loops agree with the CLI's CPU loops too
+    ref = po.ref_lib()
+    if ref is not None:
+        out = po.fr_array(len(x))
+        ref.ref_naive_gp(po.P(x), po.P(out), len(x))
+        np.testing.assert_array_equal(f("naive_gp.bin"), out)
+
+
+def test_input_loader_short_file(tmp_path):
+    """A file shorter than the buffer is read and the rest zero-filled, with
+    a message (the reference reads whatever the file holds)."""
+    from conftest import GOLDEN
+    src = os.path.join(GOLDEN, "input_prefix.bin")
+    p = run(["miss_ops", "--lg", "13", "--input", src, "--dump", str(tmp_path)])
+    assert p.returncode == 0
+    assert "rest zero-filled" in p.stderr
+    x = load(os.path.join(tmp_path, "input.bin"))
+    np.testing.assert_array_equal(x[:4096], load(src))
+    assert not x[4096:].any()
+    np.testing.assert_array_equal(load(os.path.join(tmp_path, "naive_gp.bin")), po.naive_gp(x))
+
+
+def test_input_loader_missing_file(tmp_path):
+    """A missing file is reported and replaced by the seeded generator."""
+    p = run(["miss_ops", "--lg", "10", "--input", str(tmp_path / "absent.bin"), "--dump", str(tmp_path)])
+    assert p.returncode == 0
+    assert "Error opening file" in p.stderr and "seeded synthetic data" in p.stderr
+    x = load(os.path.join(tmp_path, "input.bin"))
+    np.testing.assert_array_equal(x, po.uniform(1 << 10, po.seed(po.OP_GP, 10)))

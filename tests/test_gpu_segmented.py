@@ -69,3 +69,112 @@ def test_argument_errors(pl, params):
     assert L.pl_ntt_stage_no_rotate(ptr, 32, 10, 6, 0, 0, pp, None) == -1   # chunk < 2^it
     assert L.pl_ntt_stage_no_rotate(ptr, 48, 10, 4, 0, 0, pp, None) == -1   # not a power of two
     assert L.pl_ntt_stage_no_rotate(ptr, 64, 10, 6, 0, 0, None, None) == -1
+
+
+def _rot(g, stage, it, left=False):
+    mask = np.uint64(((1 << it) - 1) << stage)
+    r = g & mask
+    if left:
+        r = ((r << np.uint64(1)) | (r >> np.uint64(it - 1))) & mask
+    else:
+        r = ((r >> np.uint64(1)) | (r << np.uint64(it - 1))) & mask
+    return (g & ~mask) | r
+
+
+@pytest.mark.parametrize("lg,lgc,cid,stage,it", [(12, 8, 3, 0, 6), (12, 8, 5, 4, 4), (14, 10, 7, 8, 6),
+                                                 (16, 12, 2, 6, 10)])
+def test_chunk_permutations_vs_index_algebra(pl, lg, lgc, cid, stage, it):
+    """pl_stage_gather / pl_stage_scatter / pl_stage_transition against
+    numpy indexing by the oracle's ct_stage_indices and rotation."""
+    import torch
+    n, C = 1 << lg, 1 << lgc
+    x = po.uniform(n, po.seed(po.OP_NTT, lg, 300 + stage))
+    xd = pl.to_dev(x)
+    half = C // 2
+    tids = np.arange(cid * half, (cid + 1) * half, dtype=np.uint64)
+    g0, g1 = po.ct_stage_indices(tids, stage, it)
+    print("chunk permutations lg=%d chunk 2^%d id %d step (%d,%d)" % (lg, lgc, cid, stage, it), flush=True)
+    # gather
+    ch = torch.empty((C, 4), dtype=torch.int64, device="cuda")
+    pl.stage_gather(xd, ch, cid, stage, it)
+    want = np.empty((C, 4), dtype=np.uint64)
+    want[0::2] = x[g0.astype(np.int64)]
+    want[1::2] = x[g1.astype(np.int64)]
+    np.testing.assert_array_equal(pl.to_host(ch), want)
+    # scatter, global and chunk-local
+    chunk = po.uniform(C, 9)
+    out = torch.zeros_like(xd)
+    pl.stage_scatter(pl.to_dev(chunk), out, cid, stage, it)
+    want = np.zeros_like(x)
+    want[_rot(g0, stage, it).astype(np.int64)] = chunk[0::2]
+    want[_rot(g1, stage, it).astype(np.int64)] = chunk[1::2]
+    np.testing.assert_array_equal(pl.to_host(out), want)
+    if stage + it <= lgc:
+        loc = torch.zeros((C, 4), dtype=torch.int64, device="cuda")
+        pl.stage_scatter(pl.to_dev(chunk), loc, cid, stage, it, chunk_local=True)
+        np.testing.assert_array_equal(pl.to_host(loc), want[cid * C:(cid + 1) * C])
+        # transition from the step (0, stage) to (stage, it) within the chunk
+        if stage >= 1:
+            h0, h1 = po.ct_stage_indices(tids, stage, it)
+            tr = torch.empty((C, 4), dtype=torch.int64, device="cuda")
+            pl.stage_transition(pl.to_dev(chunk), tr, cid, 0, stage, stage, it)
+            want_t = np.empty((C, 4), dtype=np.uint64)
+            want_t[0::2] = chunk[_rot(h0, 0, stage, left=True).astype(np.int64) - cid * C]
+            want_t[1::2] = chunk[_rot(h1, 0, stage, left=True).astype(np.int64) - cid * C]
+            np.testing.assert_array_equal(pl.to_host(tr), want_t)
+
+
+def test_bit_reverse(pl):
+    lg = 13
+    x = po.uniform(1 << lg, 11)
+    rev = np.array([po.lib().or_bitrev(i, lg) for i in range(1 << lg)], dtype=np.int64)
+    want = np.empty_like(x)
+    want[rev] = x
+    np.testing.assert_array_equal(pl.to_host(pl.bit_reverse(pl.to_dev(x))), want)
+
+
+@pytest.mark.parametrize("lg,lg_i,lg_k", [(12, 6, 1), (16, 8, 2), (20, 10, 2), (22, 12, 4), (24, 14, 4)])
+def test_segmented_flow_vs_oracle(pl, params, lg, lg_i, lg_k):
+    """The reference's chunked flow (bit reversal, pass 1 with the
+    transition, pass 2 with gather/scatter) run on the device equals the
+    NTT, over the whole vector, at the reference's splits (22: 12+10, lg_k 4;
+    24: 14+10, lg_k 4)."""
+    from conftest import assert_fr_equal
+    x = po.uniform(1 << lg, po.seed(po.OP_NTT, lg, 400))
+    print("segmented flow lg=%d lg_i=%d lg_k=%d" % (lg, lg_i, lg_k), flush=True)
+    y = pl.to_host(pl.ntt_segmented(pl.to_dev(x), lg_i, lg_k, params))
+    assert_fr_equal(y, po.ntt(x, lg), "segmented %d" % lg)
+
+
+def test_segmented_flow_rejects_reference_invalid_splits(pl, params):
+    import torch
+    t = torch.zeros((1 << 12, 4), dtype=torch.int64, device="cuda")
+    L = pl.lib()
+    import ctypes
+    p = ctypes.c_void_p(t.data_ptr())
+    q = ctypes.c_void_p(params.data_ptr())
+    assert L.pl_ntt_segmented(p, 12, 2, 1, q, None, None) == -3   # pass 2 of 10 stages > chunk of 2^3
+    assert L.pl_ntt_segmented(p, 12, 3, 9, q, None, None) == -3   # 12 - 3 = 9 stages: rejected as by the reference
+    assert L.pl_ntt_segmented(p, 12, 6, 7, q, None, None) == -3   # chunk larger than the vector
+
+
+@pytest.mark.parametrize("lg,npoly,lg_k,inverse", [(6, 1, 0, False), (10, 3, 2, False), (16, 1, 4, True),
+                                                   (20, 2, 4, False), (20, 3, 8, True), (24, 1, 4, False)])
+def test_host_to_host_ntt_vs_oracle(pl, lg, npoly, lg_k, inverse):
+    """pl_ntt_host (pinned host in and out, grouped copies overlapping the
+    first and last passes, two device slots) against the oracle."""
+    import torch
+    from conftest import assert_fr_equal
+    n = 1 << lg
+    x = po.uniform(npoly * n, po.seed(po.OP_NTT, lg, 500 + npoly))
+    h = torch.from_numpy(x.view(np.int64)).pin_memory()
+    print("host ntt lg=%d npoly=%d lg_k=%d inv=%s" % (lg, npoly, lg_k, inverse), flush=True)
+    dom = pl.Domain(lg)
+    out = pl.ntt_host(h, dom, lg_k=lg_k, npoly=npoly, inverse=inverse)
+    got = out.numpy().view(np.uint64)
+    for p in range(npoly):
+        assert_fr_equal(got[p * n:(p + 1) * n], po.ntt(x[p * n:(p + 1) * n], lg, inverse=inverse),
+                        "host ntt poly %d" % p)
+    # in place in the host buffer, as our_ntt leaves its result in `input`
+    pl.ntt_host(h, dom, lg_k=lg_k, npoly=npoly, inverse=inverse, out=h)
+    assert_fr_equal(h.numpy().view(np.uint64), got, "host ntt in place")
