@@ -23,7 +23,9 @@ launch stream.  Beside it, in the same line:
            stream, against the measured v_mad_u64_u32 peak
 Each timed section first runs its own step, untimed, for at least 300 ms of
 device time, so that the clocks have settled under that load.
-  cpu_baseline  the oracle's radix-2 NTT 2^24 on one host core
+  cpu_baseline  the oracle's radix-2 NTT and LDE of 2^24 on all host cores (at
+           most 16 threads), and the miss_ops workload at 2^20 over the
+           reference's own blst field on one pinned core
 """
 import argparse
 import json
@@ -415,17 +417,93 @@ def bench_served(pl, synth, n_jobs, slots, large_lg):
     }
 
 
+def _cpu_model():
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
+
+
 def cpu_baseline(lg):
+    """The CPU baselines of the plan, timed on this host (reported only):
+      * NTT 2^lg and LDE 2^lg -> 2^(lg+2) by the oracle's radix-2 transform
+        on all host cores available to this process (at most 16 threads);
+      * the miss_ops workload at 2^20 (the reference's CPU path: its serial
+        naive_gp and naive_div loops, plus the permutation ratio composed
+        element by element and a Horner evaluation) over the reference's own
+        blst field build (oracle/_ref), on one pinned core."""
+    import ctypes
+    import numpy as np
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import pyoracle as po  # the CPU baseline leg only
+    cores = min(16, len(os.sched_getaffinity(0)))
+    po.set_threads(cores)
     n = 1 << lg
     xh = po.uniform(n, po.seed(po.OP_NTT, lg))
     t0 = time.perf_counter()
     po.ntt(xh, lg)
     dt = time.perf_counter() - t0
-    return {"value": round(n / dt, 1), "unit": "Fr elems/s", "cores": 1, "kind": "port",
-            "sample": "one full NTT 2^%d by the oracle's radix-2 restatement (%.1f s), %s" % (
-                lg, dt, platform.processor() or platform.machine())}
+    del xh
+    c = po.uniform(n, po.seed(po.OP_LDE, lg, 2))
+    t0 = time.perf_counter()
+    po.lde(c, lg, 2, po.ref_lde_shift(2))
+    dt_lde = time.perf_counter() - t0
+    del c
+    out = {"value": round(n / dt, 1), "unit": "Fr elems/s", "cores": cores, "kind": "port",
+           "sample": "one NTT 2^%d by the oracle's radix-2 restatement on %d threads (%.2f s)" % (lg, cores, dt),
+           "cpu_model": _cpu_model(), "nproc": os.cpu_count(), "threads_used": cores,
+           "lde": {"workload": "LDE 2^%d -> 2^%d (lambda=2, shift 7^4), oracle on %d threads" % (lg, lg + 2, cores),
+                   "s": round(dt_lde, 3), "value": round(n / dt_lde, 1), "unit": "Fr elems/s (input N)",
+                   "kind": "port", "cores": cores}}
+    ref = po.ref_lib()
+    if ref is None:
+        out["miss_ops"] = "reference blst build (oracle/_ref) absent"
+        return out
+    lgm = 20
+    nm = 1 << lgm
+    cols = [po.uniform(nm, po.seed(po.OP_GP, lgm, 10 + j)) for j in range(6)]
+    bg = po.uniform(2, po.seed(po.OP_GP, lgm, 20))
+    ks = np.concatenate([po.from_u64(1), po.from_u64(7), po.from_u64(13)])
+    poly = po.uniform(nm, po.seed(po.OP_EVAL, lgm))
+    pt = po.uniform(1, po.seed(po.OP_EVAL, lgm, 1))
+    cdiv = po.neg(pt)
+    gp_out = po.fr_array(nm)
+    q = po.fr_array(nm - 1)
+    y = po.fr_array(1)
+    prev = os.sched_getaffinity(0)
+    core = min(prev)
+    os.sched_setaffinity(0, {core})
+    try:
+        t0 = time.perf_counter()
+        z = po.ref_perm_grand_product(cols, bg[0:1], bg[1:2], ks, po.root_of_unity(lgm))
+        t_perm = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ref.ref_naive_gp(po.P(cols[0]), po.P(gp_out), nm)
+        t_gp = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ref.ref_naive_div(po.P(poly), nm, po.P(cdiv), po.P(po.one()), po.P(q))
+        t_div = time.perf_counter() - t0
+        t0 = time.perf_counter()
+        ref.ref_eval(po.P(poly), nm, po.P(pt), po.P(y))
+        t_ev = time.perf_counter() - t0
+    finally:
+        os.sched_setaffinity(0, prev)
+    del z, cols, poly, gp_out, q
+    ms = lambda t: round(t * 1e3, 3)  # noqa: E731
+    out["miss_ops"] = {
+        "workload": "miss_ops 2^20: permutation grand product (ratio over a,b,c / s1,s2,s3, k = 1, 7, 13, "
+                    "then z), naive_gp scan, naive_div by (X - z), Horner evaluation",
+        "kind": "reference", "cores": 1, "pinned_core": core,
+        "perm_grand_product_ms": ms(t_perm), "naive_gp_ms": ms(t_gp), "naive_div_ms": ms(t_div),
+        "eval_ms": ms(t_ev),
+        "perm_grand_product_elems_per_s": round(nm / t_perm, 1), "eval_elems_per_s": round(nm / t_ev, 1),
+        "div_elems_per_s": round(nm / t_div, 1),
+        "field": "the reference's blst (PLONK/blst/src/assembly.S + cpu/ff/bls12-381.hpp) built from source",
+    }
+    return out
 
 
 def main():
@@ -503,8 +581,13 @@ def main():
         result["served"] = bench_served(pl, synth, args.serve_jobs, args.serve_slots, args.serve_large)
         torch.cuda.empty_cache()
     if not args.no_cpu:
-        progress("cpu baseline (one core, ~15 s)")
+        progress("cpu baseline (NTT/LDE on all cores, miss_ops loops on one core, ~20 s)")
         result["cpu_baseline"] = cpu_baseline(lg)
+        if "miss_ops" in result and isinstance(result["cpu_baseline"].get("miss_ops"), dict):
+            g, c = result["miss_ops"]["2^20"], result["cpu_baseline"]["miss_ops"]
+            result["cpu_baseline"]["miss_ops"]["gpu_speedup"] = {
+                "perm_grand_product": round(c["perm_grand_product_ms"] / g["perm_grand_product"]["ms"], 1),
+                "eval": round(c["eval_ms"] / g["eval"]["ms"], 1), "div": round(c["naive_div_ms"] / g["div"]["ms"], 1)}
     print(json.dumps(result), flush=True)
 
 

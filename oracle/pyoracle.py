@@ -80,6 +80,9 @@ def ref_lib():
         _ref.ref_inv.argtypes = [_u64p, _u64p]
         _ref.ref_naive_gp.argtypes = [_u64p, _u64p, ctypes.c_uint64]
         _ref.ref_naive_div.argtypes = [_u64p, ctypes.c_uint64, _u64p, _u64p, _u64p]
+        _ref.ref_eval.argtypes = [_u64p, ctypes.c_uint64, _u64p, _u64p]
+        _ref.ref_perm_grand_product.argtypes = [ctypes.POINTER(_u64p), ctypes.POINTER(_u64p), _u64p, _u64p, _u64p,
+                                                _u64p, ctypes.c_uint64, _u64p]
     return _ref
 
 
@@ -273,3 +276,19 @@ def rotate_stage_output_index(g, stage, iterations):
     rot = g & mask
     rot = (rot >> np.uint64(1)) | (rot << np.uint64(iterations - 1))
     return (g & ~mask) | (rot & mask)
+
+
+def ref_perm_grand_product(cols, beta, gamma, ks, omega):
+    """The permutation grand product over the reference's own field (oracle/_ref);
+    cols = [a, b, c, s1, s2, s3], ks = (3, 4) Montgomery constants."""
+    ref = ref_lib()
+    n = len(cols[0])
+    cols = [np.ascontiguousarray(c) for c in cols]
+    arr = _u64p * 3
+    w = arr(*[P(cols[j]) for j in range(3)])
+    s = arr(*[P(cols[3 + j]) for j in range(3)])
+    z = fr_array(n)
+    kk = np.ascontiguousarray(ks, dtype=np.uint64).reshape(3, 4)
+    ref.ref_perm_grand_product(w, s, P(np.ascontiguousarray(beta)), P(np.ascontiguousarray(gamma)), P(kk),
+                               P(np.ascontiguousarray(omega)), n, P(z))
+    return z

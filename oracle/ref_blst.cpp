@@ -60,3 +60,45 @@ void ref_naive_div(const uint64_t* in, uint64_t n, const uint64_t* c, const uint
   delete[] rem;
 }
 }
+
+extern "C" {
+// Horner evaluation sum c_i x^i over the reference's field type (the
+// reference has no CPU evaluation; its GPU evaluate is function.cu:170-181):
+// the one-core CPU baseline of the evaluation.
+void ref_eval(const uint64_t* c, uint64_t n, const uint64_t* x, uint64_t* y) {
+  rfr acc;
+  std::memset(&acc, 0, 32);
+  const rfr xx = ld(x);
+  for (uint64_t i = n; i-- > 0;) acc = acc * xx + ld(c + 4 * i);
+  st(y, acc);
+}
+
+// The PLONK permutation grand product of BASELINE's miss_ops workload over
+// the reference's field type, element by element as the reference's
+// element-wise operators compose it (function.cu:41-152; the quotient term
+// shape of zksnark_compute_permutation_constraint.cu:13-14, div_mod as a
+// reciprocal per element, x/0 = 0), then the exclusive product z:
+//   p_i = prod_j (w_j + beta k_j omega^i + gamma) / prod_j (w_j + beta s_j + gamma)
+//   z_0 = 1, z_{i+1} = z_i p_i.
+void ref_perm_grand_product(const uint64_t* const* w, const uint64_t* const* s, const uint64_t* beta,
+                            const uint64_t* gamma, const uint64_t* k, const uint64_t* omega, uint64_t n,
+                            uint64_t* z) {
+  static const uint64_t one[4] = {8589934590UL, 6378425256633387010UL, 11064306276430008309UL,
+                                  1739710354780652911UL};
+  const rfr b = ld(beta), g = ld(gamma), om = ld(omega);
+  rfr bk[3];
+  for (int j = 0; j < 3; j++) bk[j] = b * ld(k + 4 * j);
+  rfr wi = ld(one), acc = ld(one);
+  for (uint64_t i = 0; i < n; i++) {
+    st(z + 4 * i, acc);
+    rfr num = ld(one), den = ld(one);
+    for (int j = 0; j < 3; j++) {
+      const rfr wj = ld(w[j] + 4 * i);
+      num = num * (wj + bk[j] * wi + g);
+      den = den * (wj + b * ld(s[j] + 4 * i) + g);
+    }
+    acc = acc * (num * (1 / den));
+    wi = wi * om;
+  }
+}
+}

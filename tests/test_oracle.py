@@ -256,3 +256,25 @@ def test_reference_input_prefix_shape():
     vals = to_int(raw)
     assert all(v < R_MOD for v in vals)
     assert vals[0] == 0 and vals[1] == to_int(po.one())[0]
+
+
+def test_reference_field_eval_and_perm_product_against_the_goldens(golden):
+    """The reference-field Horner evaluation and permutation grand product
+    (the timed CPU baseline of the miss_ops workload) against the Python
+    big-int fixtures."""
+    ref = po.ref_lib()
+    if ref is None:
+        pytest.skip("reference blst not built here")
+    k = golden("scan_small.npz")
+    for n in (1, 2, 3, 64, 100):
+        for tag in ("r", "0", "1"):
+            y = po.fr_array(1)
+            c = np.ascontiguousarray(k["eval_in_%d" % n])
+            ref.ref_eval(po.P(c), n, po.P(np.ascontiguousarray(k["eval_x_%s_%d" % (tag, n)])), po.P(y))
+            np.testing.assert_array_equal(y, k["eval_y_%s_%d" % (tag, n)])
+    ks = np.concatenate([po.from_u64(1), po.from_u64(7), po.from_u64(13)])
+    for lg in (3, 6):
+        cols = [k["perm_%s_%d" % (nm, lg)] for nm in ("a", "b", "c", "s1", "s2", "s3")]
+        z = po.ref_perm_grand_product(cols, k["perm_beta_%d" % lg], k["perm_gamma_%d" % lg], ks,
+                                      po.root_of_unity(lg))
+        np.testing.assert_array_equal(z, po.grand_product(k["perm_ratio_%d" % lg]))
