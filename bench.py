@@ -9,6 +9,8 @@ NTT of 2^24 elements (the reference's `NTT --bench-split 24` workload with the
 input already in HBM): K back-to-back NTTs bracketed by HIP events on the
 launch stream.  Beside it, in the same line:
 
+  host_to_host  NTT 2^24 from pinned host memory back to pinned host memory
+           (the reference's `our-ntt execution time` scope), 1 and 4 polynomials
   lde      coset LDE 2^24 -> 2^26 (lambda = 2, shift 7^4), device-resident
   lde_l3   coset LDE 2^24 -> 2^27 (lambda = 3, shift 7^8)
   batched  the ragged proof batch (64 polynomials, lg 12..20: 40 NTT, 16 LDE
@@ -59,6 +61,7 @@ def parse():
     ap.add_argument("--lg", type=int, default=24)
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline")
     ap.add_argument("--no-lde", action="store_true")
+    ap.add_argument("--no-h2h", action="store_true", help="skip the host-to-host NTT (our_ntt's scope)")
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-miss", action="store_true", help="skip grand product / evaluation / division")
@@ -187,6 +190,41 @@ def bench_ntt(pl, synth, lg, steps, warmup, stream, peak):
     }
     del x, y, work
     return {"value": n / (ms * 1e-3), "ms": ms, "verified": verified, "npass": npass}, roof
+
+
+def bench_host_to_host(pl, synth, lg, stream, reps=5):
+    """our_ntt's scope (ntt.cu:979-1037, the reference's `our-ntt execution
+    time`): pinned host buffer in, NTT, pinned host buffer out, through
+    pl_ntt_host (grouped 2-D copies overlapping the first and last passes;
+    with several polynomials the next one's copy-in overlaps the current
+    one's transform and copy-out).  Wall time of the blocking call."""
+    import torch
+    n = 1 << lg
+    dom = pl.Domain(lg)
+    out = {}
+    for npoly in (1, 4):
+        h = torch.empty((npoly * n, 4), dtype=torch.int64).pin_memory()
+        h.copy_(dev_uniform(pl, npoly * n, synth.seed(synth.OP_NTT, lg, 7)).cpu())
+        o = torch.empty_like(h).pin_memory()
+        best = {}
+        for lg_k in (0, 2, 4, 6):
+            pl.ntt_host(h, dom, lg_k=lg_k, npoly=npoly, out=o)
+            ts = []
+            for _ in range(reps):
+                t0 = time.perf_counter()
+                pl.ntt_host(h, dom, lg_k=lg_k, npoly=npoly, out=o)
+                ts.append((time.perf_counter() - t0) * 1e3)
+            best[lg_k] = pct(ts, 50)
+        k = min(best, key=best.get)
+        ms = best[k]
+        out["npoly_%d" % npoly] = {"ms": round(ms, 3), "ms_per_poly": round(ms / npoly, 3),
+                                   "value": round(npoly * n / (ms * 1e-3), 1), "unit": "Fr elems/s",
+                                   "pcie_gbps_each_way": round(npoly * n * 32 / (ms * 1e-3) / 1e9, 2),
+                                   "best_lg_k": k, "ms_by_lg_k": {str(a): round(v, 3) for a, v in best.items()}}
+        del h, o
+    out["workload"] = ("NTT 2^%d host-to-host (pinned in, pinned out), median of %d calls, best copy-group "
+                       "count" % (lg, reps))
+    return out
 
 
 def bench_lde(pl, synth, lg, lb, steps, warmup, stream, peak):
@@ -562,6 +600,10 @@ def main():
         "measured_mad_peak_tmad_s": round(peak / 1e12, 3),
         "pass_splits": {str(k): v for k, v in plans.items()} if plans else "default",
     }
+    if not args.no_h2h:
+        result["host_to_host"] = bench_host_to_host(pl, synth, lg, stream)
+        progress("host-to-host ntt: %s" % json.dumps(result["host_to_host"]["npoly_1"]))
+        torch.cuda.empty_cache()
     if not args.no_lde:
         result["lde"] = bench_lde(pl, synth, lg, 2, args.steps, args.warmup, stream, peak)
         progress("lde: %.3f ms" % result["lde"]["ms"])
