@@ -188,6 +188,14 @@ int pl_lde(const void* d_coeffs, void* d_out, const pl_lde_domain* d, void* d_wo
  * prod_{j<=i} p[j] (what accumulate_mul_poly_ leaves in its output). */
 size_t pl_scan_work_bytes(uint64_t n);
 int pl_grand_product(void* d_inout, uint64_t n, void* d_incl_out, void* d_work, pl_stream_t stream);
+/* The same grand product with no allocation and no workspace argument (no
+ * counterpart in the reference): for callers that cannot allocate, such as
+ * code under HIP graph capture or a request loop that must not call
+ * hipMalloc.  Its block partials live in a 4 MiB device array the library
+ * holds per GPU until the process ends; n <= 2^28 (PL_ERR_UNSUPPORTED
+ * beyond).  Calls on one GPU share that array: calls that overlap on the
+ * device (different streams, not ordered by events) race. */
+int pl_grand_product_noalloc(void* d_inout, uint64_t n, void* d_incl_out, pl_stream_t stream);
 
 /* Evaluation [evaluate, function.cu:170-181; mont_arithmetic.cu:104-163]:
  * *d_y = sum_{i<n} poly[i] * x^i, x a device element (the reference passes

@@ -102,6 +102,7 @@ uint64_t scan_scratch_elems(uint64_t n);
 // zero-fill by kernel (no memset nodes: callers may capture in hipGraphs)
 hipError_t fill_zero(fr* p, uint64_t n, hipStream_t s);
 hipError_t grand_product(fr* inout, fr* incl_out, uint64_t n, fr* scratch, hipStream_t s);
+uint64_t grand_product_scratch_elems(uint64_t n);  // the prefix product's partials (a few per 4096 elements)
 hipError_t div_linear(fr* inout, uint64_t n, const fr* c, fr* rem, fr* scratch, hipStream_t s);
 hipError_t evaluate(const fr* coeffs, uint64_t n, const fr* x, fr* y, fr* scratch, hipStream_t s);
 hipError_t eval_div(const fr* in, fr* q, uint64_t n, const fr* z, fr* y, fr* scratch, hipStream_t s);

@@ -487,6 +487,8 @@ uint64_t scan_scratch_elems(uint64_t n) {
   return a > b ? a : b;
 }
 
+uint64_t grand_product_scratch_elems(uint64_t n) { return pfx_scratch_elems(n); }
+
 hipError_t grand_product(fr* inout, fr* incl_out, uint64_t n, fr* scratch, hipStream_t s) {
   return prefix_product(inout, inout, incl_out, n, nullptr, scratch, s);
 }

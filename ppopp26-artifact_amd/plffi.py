@@ -65,6 +65,7 @@ SIGNATURES = {
     "pl_lde": (_i, [_vp, _vp, _vp, _vp, _vp]),
     "pl_scan_work_bytes": (ctypes.c_size_t, [_u64]),
     "pl_grand_product": (_i, [_vp, _u64, _vp, _vp, _vp]),
+    "pl_grand_product_noalloc": (_i, [_vp, _u64, _vp, _vp]),
     "pl_eval": (_i, [_vp, _u64, _vp, _vp, _vp, _vp]),
     "pl_div_linear": (_i, [_vp, _u64, _vp, _vp, _vp, _vp]),
     "pl_vec_work_bytes": (ctypes.c_size_t, [_u64]),
@@ -261,6 +262,13 @@ def grand_product(p, incl_out=None, work=None, stream=None):
     """in place: p[i] <- prod_{j<i} p[j]"""
     check("pl_grand_product", lib().pl_grand_product(_ptr(p), p.shape[0], _ptr(incl_out), _ptr(work),
                                                      _stream(stream)))
+    return p
+
+
+def grand_product_noalloc(p, incl_out=None, stream=None):
+    """in place: p[i] <- prod_{j<i} p[j], library-held scratch (no allocation)"""
+    check("pl_grand_product_noalloc", lib().pl_grand_product_noalloc(_ptr(p), p.shape[0], _ptr(incl_out),
+                                                                     _stream(stream)))
     return p
 
 
