@@ -177,6 +177,34 @@ size_t pl_lde_domain_bytes(const pl_lde_domain* d);
 size_t pl_lde_work_bytes(int lg_n, int lg_blowup);
 int pl_lde(const void* d_coeffs, void* d_out, const pl_lde_domain* d, void* d_work, pl_stream_t stream);
 
+/* ---- coset NTT / iNTT ----------------------------------------------------
+ * [Ntt_coset / Intt_coset, PLONK/utils/function.cuh:116-142,
+ *  function.cu:354-395; ntt_zkp_coset_*, zksnark_ntt.cu:521-908]
+ * Over the shifted domain g<w_N>, N = 2^lg, g = 7 (h_shift NULL) or a
+ * caller-given Montgomery element:
+ *   forward  X[k] = sum_i x_i (g w_N^k)^i
+ *   inverse  x_i  = g^-i (1/N) sum_k X_k w_N^-ik
+ * pl_coset_ntt runs the whole transform (d_in may equal d_out; d_work NULL
+ * or pl_coset_work_bytes(lg)).  pl_coset_scale is the coset-power step alone
+ * on chunk chunk_id of a natural-order vector (chunk[i] *= g^(+-(chunk_id *
+ * chunk_elems + i)), in place; the reference's init).  pl_coset_ntt_step
+ * splits the transform into the reference's three calls (forward1/2/3) over
+ * the whole vector: step 1 the first pass with the coset powers (d_in ->
+ * d_out, may alias), step 2 the middle passes (in place on d_in), step 3 the
+ * last pass into natural order (d_in -> d_out, distinct; inverse: times
+ * g^-i); steps 1..3 in order equal pl_coset_ntt. */
+typedef struct pl_coset_domain pl_coset_domain;
+int pl_coset_domain_create(int lg, const void* h_shift, pl_coset_domain** out);
+int pl_coset_domain_destroy(pl_coset_domain* d);
+size_t pl_coset_domain_bytes(const pl_coset_domain* d);
+size_t pl_coset_work_bytes(int lg);
+int pl_coset_ntt(const void* d_in, void* d_out, int inverse, const pl_coset_domain* d, void* d_work,
+                 pl_stream_t stream);
+int pl_coset_scale(void* d_chunk, uint64_t chunk_elems, uint64_t chunk_id, int inverse, const pl_coset_domain* d,
+                   pl_stream_t stream);
+int pl_coset_ntt_step(const void* d_in, void* d_out, int step, int inverse, const pl_coset_domain* d,
+                      pl_stream_t stream);
+
 /* ---- scans -------------------------------------------------------------
  * All three are work-efficient block scans (~3 Montgomery products per
  * element).  d_work: NULL or pl_scan_work_bytes(n) bytes of device scratch.

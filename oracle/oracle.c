@@ -420,6 +420,15 @@ static void periodic_range(void* c, uint64_t lo, uint64_t hi) {
   for (uint64_t k = lo; k < hi; k++) memcpy(x->out + 4 * k, x->in + 4 * (k & x->mask), 32);
 }
 
+/* inverse coset NTT: x_i = shift^-i (1/N) sum_k X_k w_N^-ik, in place */
+void or_coset_intt(uint64_t* data, int lg, const uint64_t* shift) {
+  uint64_t inv[4];
+  or_ntt(data, lg, 1);
+  or_inv(inv, shift);
+  struct powers_ctx pc = {data, inv, NULL, 0, 1};
+  par_for(1ull << lg, 1, powers_range, &pc);
+}
+
 /* out[j*B + t] = coset_NTT_N(c; shift*w_M^t)[j], B = 2^lg_blowup */
 void or_lde(const uint64_t* coeffs, uint64_t* out, int lg_n, int lg_blowup, const uint64_t* shift) {
   uint64_t n = 1ull << lg_n;

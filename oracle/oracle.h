@@ -79,6 +79,10 @@ int or_get_threads(void);
 void or_ntt(uint64_t* data, int lg, int inverse);
 /* coset NTT: out[k] = sum_i c_i (shift*w_N^k)^i, in place */
 void or_coset_ntt(uint64_t* data, int lg, const uint64_t* shift);
+/* inverse coset NTT: x_i = shift^-i iNTT(X)_i, in place (pinned by the round
+ * trip with or_coset_ntt, whose column-0 role in or_lde the brute-force LDE
+ * fixtures pin) */
+void or_coset_intt(uint64_t* data, int lg, const uint64_t* shift);
 /* our_lde (lde.cu:1267-1365): out[k] = sum_{i<N} c_i (shift*w_M^k)^i,
  * M = N << lg_blowup.  The reference's shift is 7^(2^lg_blowup). */
 void or_lde(const uint64_t* coeffs, uint64_t* out, int lg_n, int lg_blowup, const uint64_t* shift);

@@ -53,6 +53,7 @@ def lib():
         L.or_get_threads.restype = ctypes.c_int
         L.or_ntt.argtypes = [_u64p, ctypes.c_int, ctypes.c_int]
         L.or_coset_ntt.argtypes = [_u64p, ctypes.c_int, _u64p]
+        L.or_coset_intt.argtypes = [_u64p, ctypes.c_int, _u64p]
         L.or_lde.argtypes = [_u64p, _u64p, ctypes.c_int, ctypes.c_int, _u64p]
         L.or_naive_lde.argtypes = [_u64p, _u64p, ctypes.c_int, ctypes.c_int]
         L.or_grand_product.argtypes = [_u64p, ctypes.c_uint64]
@@ -187,6 +188,12 @@ def ntt(x, lg, inverse=False):
 def coset_ntt(x, lg, shift):
     y = np.array(x, dtype=np.uint64, copy=True, order="C")
     lib().or_coset_ntt(P(y), lg, P(np.ascontiguousarray(shift)))
+    return y
+
+
+def coset_intt(x, lg, shift):
+    y = np.array(x, dtype=np.uint64, copy=True, order="C")
+    lib().or_coset_intt(P(y), lg, P(np.ascontiguousarray(shift)))
     return y
 
 

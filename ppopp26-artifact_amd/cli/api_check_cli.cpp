@@ -4,7 +4,9 @@
 // Files (raw 32-byte elements): x.bin y.bin (inputs), ntt.bin intt.bin,
 // gp.bin gp_incl.bin, eval.bin, div.bin, add.bin mul.bin div_ew.bin,
 // add_cpu.bin (CPU-dispatched add), lde.bin (our_lde semantics, lambda=2),
-// stage0.bin stage6.bin (Ntt::stage_no_rotate on seeded chunks).
+// stage0.bin stage6.bin (Ntt::stage_no_rotate on seeded chunks), coset.bin
+// icoset.bin coset_steps.bin icoset_steps.bin coset_init0.bin coset_lde3.bin
+// (Ntt_coset / Intt_coset).
 #include <string>
 
 #include "common.hpp"
@@ -83,6 +85,29 @@ int main(int argc, char** argv) {
       threw = true;
     }
     if (!threw) throw std::runtime_error("Intt::stage_no_rotate did not refuse");
+    // Ntt_coset / Intt_coset: one call, three steps, chunked init, coset LDE
+    Ntt_coset nc(32, lg);
+    Intt_coset ic(32, lg);
+    SyncedMemory xs = seeded(n, 1001);
+    xs.mutable_gpu_data();
+    dump(dir + "/coset.bin", nc.forward(xs));
+    dump(dir + "/icoset.bin", ic.forward(xs));
+    SyncedMemory st(n * 32), sti(n * 32);
+    SyncedMemory a1 = seeded(n, 1001), a2 = seeded(n, 1001);
+    nc.forward1(a1, lg, 0, 0);
+    nc.forward2(a1, lg, 0, 0);
+    nc.forward3(a1, st, lg, 0, 0, 0);
+    ic.forward1(a2, lg, 0, 0);
+    ic.forward2(a2, lg, 0, 0);
+    ic.forward3(a2, sti, lg, 0, 0, 0);
+    dump(dir + "/coset_steps.bin", st);
+    dump(dir + "/icoset_steps.bin", sti);
+    SyncedMemory ch = seeded(n / 4, 1001);  // chunk 0 of x, then chunk 0 scaled back by Intt_coset::init
+    nc.init(ch, lg, 0);
+    dump(dir + "/coset_init0.bin", ch);
+    SyncedMemory lc(n * 8 * 32);
+    nc.init_and_forward(xs, lc, lg, 0, 3, 0, false);
+    dump(dir + "/coset_lde3.bin", lc);
     std::cout << "api_check ok" << std::endl;
     return 0;
   } catch (const std::exception& ex) {

@@ -218,6 +218,103 @@ Intt::Intt(int domain_size, hipStream_t stream) : Ntt(domain_size, stream) { inv
 
 void Intt::_forward_(SyncedMemory input, hipStream_t s) { forward_(input, input, s); }
 
+namespace pl {
+// coset domains of one Ntt_coset object, by size; LDE domains by (size, blowup)
+struct CosetDomains {
+  std::mutex mu;
+  std::map<int, pl_coset_domain*> coset;
+  std::map<std::pair<int, int>, pl_lde_domain*> lde;
+  ~CosetDomains() {
+    for (auto& kv : coset) pl_coset_domain_destroy(kv.second);
+    for (auto& kv : lde) pl_lde_domain_destroy(kv.second);
+  }
+  pl_coset_domain* get(int lg) {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = coset.find(lg);
+    if (it != coset.end()) return it->second;
+    pl_coset_domain* d = nullptr;
+    check(pl_coset_domain_create(lg, nullptr, &d), "pl_coset_domain_create");
+    coset[lg] = d;
+    return d;
+  }
+  pl_lde_domain* get_lde(int lg, int lb) {
+    std::lock_guard<std::mutex> g(mu);
+    auto it = lde.find({lg, lb});
+    if (it != lde.end()) return it->second;
+    const pl::host::fr64 seven = pl::host::from_u64(7);
+    pl_lde_domain* d = nullptr;
+    check(pl_lde_domain_create(lg, lb, seven.l, &d), "pl_lde_domain_create");
+    lde[{lg, lb}] = d;
+    return d;
+  }
+};
+}  // namespace pl
+
+Ntt_coset::Ntt_coset(int domain_size, int coset_size)
+    : lg_domain_size(coset_size), doms_(std::make_shared<pl::CosetDomains>()) {
+  (void)domain_size;
+  Params = SyncedMemory(4 * 16384 * kFr);
+  check(pl_params_fill(Params.mutable_gpu_data(), nullptr), "pl_params_fill");
+}
+
+Intt_coset::Intt_coset(int domain_size, int coset_size, hipStream_t stream) : Ntt_coset(domain_size, coset_size) {
+  (void)stream;
+  inverse_ = true;
+}
+
+void Ntt_coset::init(SyncedMemory input, int lg, int chunk_id, bool is_intt, hipStream_t s) {
+  void* p = input.mutable_gpu_data_async(s);
+  check(pl_coset_scale(p, elems(input), (uint64_t)chunk_id, is_intt ? 1 : 0, doms_->get(lg), (pl_stream_t)s),
+        "pl_coset_scale");
+}
+
+void Intt_coset::init(SyncedMemory input, int lg, int chunk_id, bool is_intt, hipStream_t s) {
+  Ntt_coset::init(input, lg, chunk_id, is_intt, s);
+}
+
+void Ntt_coset::init_and_forward(SyncedMemory input, SyncedMemory output, int lg, int chunk_id, int lambda, int stage,
+                                 bool is_intt, hipStream_t s) {
+  (void)chunk_id;
+  (void)stage;
+  if (is_intt) throw std::logic_error("init_and_forward: forward coset LDE only");
+  void* pi = input.mutable_gpu_data_async(s);
+  void* po = output.mutable_gpu_data_async(s);
+  check(pl_lde(pi, po, doms_->get_lde(lg, lambda), nullptr, (pl_stream_t)s), "pl_lde");
+}
+
+void Ntt_coset::forward1(SyncedMemory input, int lg, int stage, int chunk_id, hipStream_t s) {
+  (void)stage;
+  (void)chunk_id;
+  void* p = input.mutable_gpu_data_async(s);
+  check(pl_coset_ntt_step(p, p, 1, inverse_ ? 1 : 0, doms_->get(lg), (pl_stream_t)s), "pl_coset_ntt_step");
+}
+
+void Ntt_coset::forward2(SyncedMemory input, int lg, int stage, int chunk_id, hipStream_t s) {
+  (void)stage;
+  (void)chunk_id;
+  void* p = input.mutable_gpu_data_async(s);
+  check(pl_coset_ntt_step(p, nullptr, 2, inverse_ ? 1 : 0, doms_->get(lg), (pl_stream_t)s), "pl_coset_ntt_step");
+}
+
+void Ntt_coset::forward3(SyncedMemory input, SyncedMemory output, int lg, int stage, int chunk_id, int lambda,
+                         hipStream_t s) {
+  (void)stage;
+  (void)chunk_id;
+  (void)lambda;
+  void* pi = input.mutable_gpu_data_async(s);
+  void* po = output.mutable_gpu_data_async(s);
+  check(pl_coset_ntt_step(pi, po, 3, inverse_ ? 1 : 0, doms_->get(lg), (pl_stream_t)s), "pl_coset_ntt_step");
+}
+
+SyncedMemory Ntt_coset::forward(SyncedMemory input, hipStream_t s) {
+  SyncedMemory out(input.size());
+  const int lg = lg_of(elems(input));
+  void* pi = input.mutable_gpu_data_async(s);
+  void* po = out.mutable_gpu_data_async(s);
+  check(pl_coset_ntt(pi, po, inverse_ ? 1 : 0, doms_->get(lg), nullptr, (pl_stream_t)s), "pl_coset_ntt");
+  return out;
+}
+
 uint32_t bit_rev(uint32_t i, uint32_t nbits) {
   uint32_t r = 0;
   for (uint32_t b = 0; b < nbits; b++) r |= ((i >> b) & 1u) << (nbits - 1 - b);

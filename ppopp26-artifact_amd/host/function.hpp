@@ -102,6 +102,50 @@ class Intt : public Ntt {
   void _forward_(SyncedMemory input, hipStream_t stream = (hipStream_t)0);  // in place
 };
 
+// [Ntt_coset / Intt_coset, function.cuh:116-142] The coset transform over
+// g<w_N>, g = 7, N = 2^coset_size (pl_coset_*):
+//   Ntt_coset   X[k] = sum_i x_i (g w_N^k)^i
+//   Intt_coset  x_i = g^-i iNTT(X)_i
+// init scales chunk chunk_id of the natural-order vector by the coset powers
+// (g^(+-(chunk_id * C + i)), C = the chunk's length) in place; forward1 /
+// forward2 / forward3 run the transform in three calls over the whole vector
+// (the first pass with the coset powers fused, in place; the middle passes,
+// in place; the last pass into `output`, natural order) and compose to the
+// whole transform, which forward() runs in one call; init_and_forward writes
+// the coset LDE of the 2^lg_domain_size coefficients in `input`, blowup
+// 2^lambda, shift g, into `output` (natural order).  The stage / chunk_id
+// arguments of forward1/2/3 are the reference's chunked schedule, which the
+// whole-vector steps do not need (accepted and ignored).
+namespace pl {
+struct CosetDomains;
+}
+class Ntt_coset {
+ public:
+  SyncedMemory Params;
+  int lg_domain_size;
+  Ntt_coset(int domain_size, int coset_size);
+  void init(SyncedMemory input, int lg_domain_size, int chunk_id, bool is_intt = false,
+            hipStream_t stream = (hipStream_t)0);
+  void init_and_forward(SyncedMemory input, SyncedMemory output, int lg_domain_size, int chunk_id, int lambda,
+                        int stage, bool is_intt, hipStream_t stream = (hipStream_t)0);
+  void forward1(SyncedMemory input, int lg_domain_size, int stage, int chunk_id, hipStream_t stream = (hipStream_t)0);
+  void forward2(SyncedMemory input, int lg_domain_size, int stage, int chunk_id, hipStream_t stream = (hipStream_t)0);
+  void forward3(SyncedMemory input, SyncedMemory output, int lg_domain_size, int stage, int chunk_id, int lambda,
+                hipStream_t stream = (hipStream_t)0);
+  SyncedMemory forward(SyncedMemory input, hipStream_t stream = (hipStream_t)0);
+
+ protected:
+  bool inverse_ = false;
+  std::shared_ptr<pl::CosetDomains> doms_;
+};
+
+class Intt_coset : public Ntt_coset {
+ public:
+  Intt_coset(int domain_size, int coset_size, hipStream_t stream = (hipStream_t)0);
+  void init(SyncedMemory input, int lg_domain_size, int chunk_id, bool is_intt = true,
+            hipStream_t stream = (hipStream_t)0);
+};
+
 // host helpers (function.cuh:162-244)
 uint32_t bit_rev(uint32_t i, uint32_t nbits);
 void calculate_map1(uint32_t* map, int lg_N);

@@ -63,6 +63,13 @@ SIGNATURES = {
     "pl_lde_domain_bytes": (ctypes.c_size_t, [_vp]),
     "pl_lde_work_bytes": (ctypes.c_size_t, [_i, _i]),
     "pl_lde": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "pl_coset_domain_create": (_i, [_i, _vp, ctypes.POINTER(_vp)]),
+    "pl_coset_domain_destroy": (_i, [_vp]),
+    "pl_coset_domain_bytes": (ctypes.c_size_t, [_vp]),
+    "pl_coset_work_bytes": (ctypes.c_size_t, [_i]),
+    "pl_coset_ntt": (_i, [_vp, _vp, _i, _vp, _vp, _vp]),
+    "pl_coset_scale": (_i, [_vp, _u64, _u64, _i, _vp, _vp]),
+    "pl_coset_ntt_step": (_i, [_vp, _vp, _i, _i, _vp, _vp]),
     "pl_scan_work_bytes": (ctypes.c_size_t, [_u64]),
     "pl_grand_product": (_i, [_vp, _u64, _vp, _vp, _vp]),
     "pl_grand_product_noalloc": (_i, [_vp, _u64, _vp, _vp]),
@@ -255,6 +262,54 @@ def lde(c, dom, out=None, work=None, stream=None):
     if out is None:
         out = torch.empty((n << dom.lg_blowup, 4), dtype=torch.int64, device=c.device)
     check("pl_lde", lib().pl_lde(_ptr(c), _ptr(out), dom.h, _ptr(work), _stream(stream)))
+    return out
+
+
+class CosetDomain:
+    """Tables of the coset transform over g<w_N> (Ntt_coset / Intt_coset);
+    shift: None (g = 7) or a numpy (1, 4) uint64 Montgomery element."""
+
+    def __init__(self, lg, shift=None):
+        import numpy as np
+        self.lg = lg
+        sp = None
+        if shift is not None:
+            self._shift = np.ascontiguousarray(shift, dtype=np.uint64).reshape(4)
+            sp = ctypes.c_void_p(self._shift.ctypes.data)
+        h = ctypes.c_void_p()
+        check("pl_coset_domain_create", lib().pl_coset_domain_create(lg, sp, ctypes.byref(h)))
+        self.h = h
+
+    def close(self):
+        if self.h:
+            lib().pl_coset_domain_destroy(self.h)
+            self.h = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def coset_ntt(x, dom, inverse=False, out=None, work=None, stream=None):
+    import torch
+    if out is None:
+        out = torch.empty_like(x)
+    check("pl_coset_ntt", lib().pl_coset_ntt(_ptr(x), _ptr(out), 1 if inverse else 0, dom.h, _ptr(work),
+                                             _stream(stream)))
+    return out
+
+
+def coset_scale(chunk, chunk_id, dom, inverse=False, stream=None):
+    check("pl_coset_scale", lib().pl_coset_scale(_ptr(chunk), chunk.shape[0], chunk_id, 1 if inverse else 0, dom.h,
+                                                 _stream(stream)))
+    return chunk
+
+
+def coset_ntt_step(x, out, step, dom, inverse=False, stream=None):
+    check("pl_coset_ntt_step", lib().pl_coset_ntt_step(_ptr(x), _ptr(out), step, 1 if inverse else 0, dom.h,
+                                                       _stream(stream)))
     return out
 
 

@@ -47,6 +47,14 @@ def test_cpp_api(tmp_path):
     np.testing.assert_array_equal(f("lde.bin"), po.lde(x, lg, 2, po.ref_lde_shift(2)))
     np.testing.assert_array_equal(f("stage0.bin"), po.stage_no_rotate(x, lg, 6, 0, 0))
     np.testing.assert_array_equal(f("stage6.bin"), po.stage_no_rotate(x[:1024], lg, 6, 6, 1))
+    g = po.from_u64(7)
+    np.testing.assert_array_equal(f("coset.bin"), po.coset_ntt(x, lg, g))
+    np.testing.assert_array_equal(f("icoset.bin"), po.coset_intt(x, lg, g))
+    np.testing.assert_array_equal(f("coset_steps.bin"), po.coset_ntt(x, lg, g))
+    np.testing.assert_array_equal(f("icoset_steps.bin"), po.coset_intt(x, lg, g))
+    q = len(x) // 4
+    np.testing.assert_array_equal(f("coset_init0.bin"), po.mul(x[:q], np.concatenate([po.pow_(g, i) for i in range(q)])))
+    np.testing.assert_array_equal(f("coset_lde3.bin"), po.lde(x, lg, 3, g))
 
 
 @pytest.mark.parametrize("mode", ["--check-split", "--check-baseline"])

@@ -278,3 +278,27 @@ def test_reference_field_eval_and_perm_product_against_the_goldens(golden):
         z = po.ref_perm_grand_product(cols, k["perm_beta_%d" % lg], k["perm_gamma_%d" % lg], ks,
                                       po.root_of_unity(lg))
         np.testing.assert_array_equal(z, po.grand_product(k["perm_ratio_%d" % lg]))
+
+
+def test_coset_intt_inverts_coset_ntt(golden):
+    """or_coset_intt undoes or_coset_ntt; or_coset_ntt itself is the first
+    column of or_lde, which the brute-force fixtures pin (shift 7 and the
+    reference's 7^(2^lg_blowup))."""
+    k = golden("lde_small.npz")
+    for lg in (1, 3, 5):
+        c = k["in_%d" % lg]
+        B = 2
+        lde = k["lde_g_%d_1" % lg]
+        col0 = lde[0::B]
+        np.testing.assert_array_equal(po.coset_ntt(c, lg, po.from_u64(7)), col0)
+    for lg, shift in ((6, po.from_u64(7)), (10, po.uniform(1, 5)), (12, po.from_u64(5))):
+        x = po.uniform(1 << lg, 40 + lg)
+        np.testing.assert_array_equal(po.coset_intt(po.coset_ntt(x, lg, shift), lg, shift), x)
+        # and the coset powers by themselves: coset_ntt(x) = ntt(x_i shift^i)
+        pw = np.concatenate([po.pow_(shift, i) for i in range(8)])
+        y = x.copy()
+        y[:8] = po.mul(x[:8], pw)
+        rest = po.mul(x[8:], np.concatenate([po.pow_(shift, i) for i in range(8, 1 << lg)])) if lg <= 10 else None
+        if rest is not None:
+            y[8:] = rest
+            np.testing.assert_array_equal(po.coset_ntt(x, lg, shift), po.ntt(y, lg))
