@@ -448,6 +448,42 @@ void or_lde(const uint64_t* coeffs, uint64_t* out, int lg_n, int lg_blowup, cons
   free(tmp);
 }
 
+/* w_3N = 7^((r-1)/(3*2^32)) squared (32-lg) times (3 divides r - 1) */
+void or_root_of_unity_3n(uint64_t* o, int lg) {
+  static const uint64_t RM1[4] = {0xffffffff00000000ull, 0x53bda402fffe5bfeull, 0x3339d80809a1d805ull,
+                                  0x73eda753299d7d48ull};
+  uint64_t e[4];
+  for (int i = 0; i < 4; i++) e[i] = (RM1[i] >> 32) | (i < 3 ? RM1[i + 1] << 32 : 0);
+  u128 rem = 0;
+  for (int i = 3; i >= 0; i--) {
+    u128 cur = (rem << 64) | e[i];
+    e[i] = (uint64_t)(cur / 3);
+    rem = cur % 3;
+  }
+  uint64_t seven[4];
+  or_from_u64(seven, 7);
+  or_pow(o, seven, e);
+  for (int i = lg; i < 32; i++) or_mul(o, o, o);
+}
+
+/* blowup-3 LDE: out[3j + t] = coset_NTT_N(c; shift*w_3N^t)[j] */
+void or_lde3(const uint64_t* coeffs, uint64_t* out, int lg_n, const uint64_t* shift) {
+  uint64_t n = 1ull << lg_n;
+  uint64_t w3[4];
+  or_root_of_unity_3n(w3, lg_n);
+  uint64_t* tmp = (uint64_t*)malloc(32 * n);
+  uint64_t st[4];
+  memcpy(st, shift, 32);
+  for (uint64_t t = 0; t < 3; t++) {
+    memcpy(tmp, coeffs, 32 * n);
+    or_coset_ntt(tmp, lg_n, st);
+    struct strided_ctx ic = {out, tmp, 3, t, 0};
+    par_for(n, 1, interleave_range, &ic);
+    or_mul(st, st, w3);
+  }
+  free(tmp);
+}
+
 void or_naive_lde(const uint64_t* coeffs, uint64_t* out, int lg_n, int lg_blowup) {
   uint64_t n = 1ull << lg_n;
   uint64_t M = n << lg_blowup;

@@ -86,6 +86,12 @@ void or_coset_intt(uint64_t* data, int lg, const uint64_t* shift);
 /* our_lde (lde.cu:1267-1365): out[k] = sum_{i<N} c_i (shift*w_M^k)^i,
  * M = N << lg_blowup.  The reference's shift is 7^(2^lg_blowup). */
 void or_lde(const uint64_t* coeffs, uint64_t* out, int lg_n, int lg_blowup, const uint64_t* shift);
+/* primitive 3*2^lg-th root of unity 7^((r-1)/(3*2^lg)); its cube is w_{2^lg} */
+void or_root_of_unity_3n(uint64_t* o, int lg);
+/* blowup-3 LDE (no counterpart in the reference): out[k] = sum_i c_i
+ * (shift*w_3N^k)^i, k < 3N, i.e. out[3j + t] = coset NTT with shift
+ * shift*w_3N^t at j (pinned by brute force from Python integers) */
+void or_lde3(const uint64_t* coeffs, uint64_t* out, int lg_n, const uint64_t* shift);
 /* naive_lde (lde.cu:1367-1490): out[k] = NTT_N(c)[k mod N] */
 void or_naive_lde(const uint64_t* coeffs, uint64_t* out, int lg_n, int lg_blowup);
 

@@ -54,6 +54,8 @@ def lib():
         L.or_ntt.argtypes = [_u64p, ctypes.c_int, ctypes.c_int]
         L.or_coset_ntt.argtypes = [_u64p, ctypes.c_int, _u64p]
         L.or_coset_intt.argtypes = [_u64p, ctypes.c_int, _u64p]
+        L.or_root_of_unity_3n.argtypes = [_u64p, ctypes.c_int]
+        L.or_lde3.argtypes = [_u64p, _u64p, ctypes.c_int, _u64p]
         L.or_lde.argtypes = [_u64p, _u64p, ctypes.c_int, ctypes.c_int, _u64p]
         L.or_naive_lde.argtypes = [_u64p, _u64p, ctypes.c_int, ctypes.c_int]
         L.or_grand_product.argtypes = [_u64p, ctypes.c_uint64]
@@ -200,6 +202,18 @@ def coset_intt(x, lg, shift):
 def lde(c, lg_n, lg_blowup, shift):
     out = fr_array(len(c) << lg_blowup)
     lib().or_lde(P(np.ascontiguousarray(c)), P(out), lg_n, lg_blowup, P(np.ascontiguousarray(shift)))
+    return out
+
+
+def root_of_unity_3n(lg):
+    o = fr_array(1)
+    lib().or_root_of_unity_3n(P(o), lg)
+    return o
+
+
+def lde3(c, lg_n, shift):
+    out = fr_array(3 * len(c))
+    lib().or_lde3(P(np.ascontiguousarray(c)), P(out), lg_n, P(np.ascontiguousarray(shift)))
     return out
 
 

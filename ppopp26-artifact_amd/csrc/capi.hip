@@ -187,12 +187,47 @@ size_t pl_lde_work_bytes(int lg_n, int lg_blowup) {
   return (size_t)sizeof(pl::fr) << (lg_n + lg_blowup);
 }
 
+// blowup 3 (BASELINE's "3 coset NTTs of size N with shifts g w_3N^t"; no
+// counterpart in the reference, whose our_lde takes power-of-two blowups)
+int pl_lde3_domain_create(int lg_n, const void* h_shift, pl_lde_domain** out) {
+  if (!out || lg_n < 1) return PL_ERR_ARG;
+  if (lg_n > 26) return PL_ERR_UNSUPPORTED;
+  *out = nullptr;
+  int ndev = 0;
+  if (hipGetDeviceCount(&ndev) != hipSuccess || ndev == 0) return PL_ERR_NO_DEVICE;
+  pl::host::fr64 shift = pl::host::from_u64(7);
+  if (h_shift) {
+    memcpy(shift.l, h_shift, 32);
+    if (pl::host::geq_p(shift.l)) return PL_ERR_ARG;
+  }
+  pl_lde_domain* d = new (std::nothrow) pl_lde_domain();
+  if (!d) return PL_ERR_NOMEM;
+  hipError_t e = pl::lde_domain_build(d->d, lg_n, -1, shift, nullptr);
+  if (e != hipSuccess) {
+    pl::lde_domain_free(d->d);
+    delete d;
+    return hip_fail(e);
+  }
+  *out = d;
+  return PL_OK;
+}
+
+size_t pl_lde_domain_work_bytes(const pl_lde_domain* d) {
+  if (!d) return 0;
+  const size_t m = (size_t)d->d.blowup << d->d.lg_n;
+  // blowup 3: the pass scratch and the cosets' outputs before the interleave
+  return sizeof(pl::fr) * (d->d.blowup == 3 ? 2 * m : m);
+}
+
+int pl_lde_domain_blowup(const pl_lde_domain* d) { return d ? d->d.blowup : PL_ERR_ARG; }
+
 int pl_lde(const void* d_coeffs, void* d_out, const pl_lde_domain* d, void* d_work, pl_stream_t stream) {
   if (!d || !d_coeffs || !d_out) return PL_ERR_ARG;
   if (!aligned16(d_coeffs) || !aligned16(d_out) || !aligned16(d_work)) return PL_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
   const int lg_n = d->d.lg_n, lb = d->d.lg_blowup;
-  size_t wb = pl_lde_work_bytes(lg_n, lb);
+  const bool three = d->d.blowup == 3;
+  size_t wb = pl_lde_domain_work_bytes(d);
   pl::fr* tmp = (pl::fr*)d_work;
   bool own = false;
   if (!tmp) {
@@ -200,17 +235,27 @@ int pl_lde(const void* d_coeffs, void* d_out, const pl_lde_domain* d, void* d_wo
     if (e != hipSuccess) return hip_fail(e);
     own = true;
   }
+  const uint64_t n = 1ull << lg_n;
   pl::NttCall c;
   c.in = (const pl::fr*)d_coeffs;
-  c.out = (pl::fr*)d_out;
   c.tmp = tmp;
   c.in_pstride = 0;  // every coset reads the same coefficients
-  c.npoly = 1u << lb;
   c.premul = d->d.cp;
-  c.pm_mask = (1u << lb) - 1;
-  c.lgI = lb;
   c.inverse = false;
+  if (three) {
+    // three coset NTTs side by side (coset t at t N), then out[3 j + t]
+    c.out = tmp + 3 * n;
+    c.npoly = 3;
+    c.pm_mask = 3;
+    c.lgI = 0;
+  } else {
+    c.out = (pl::fr*)d_out;
+    c.npoly = 1u << lb;
+    c.pm_mask = (1u << lb) - 1;
+    c.lgI = lb;
+  }
   hipError_t e = pl::ntt_launch(d->d.ntt, c, s);
+  if (e == hipSuccess && three) e = pl::interleave3(c.out, (pl::fr*)d_out, n, s);
   if (own) (void)hipFreeAsync(tmp, s);
   return e == hipSuccess ? PL_OK : hip_fail(e);
 }

@@ -141,6 +141,23 @@ inline fr64 root_of_unity(int lg) {
   return w;
 }
 
+// primitive 3*2^lg-th root of unity: 7^((r-1)/(3*2^32)) squared (32-lg)
+// times, so its cube is root_of_unity(lg) (3 divides r - 1)
+inline fr64 root_of_unity_3n(int lg) {
+  uint64_t e[4];
+  uint64_t rm1[4] = {kP[0] - 1, kP[1], kP[2], kP[3]};
+  for (int i = 0; i < 4; i++) e[i] = (rm1[i] >> 32) | (i < 3 ? rm1[i + 1] << 32 : 0);
+  unsigned __int128 rem = 0;
+  for (int i = 3; i >= 0; i--) {  // e /= 3, exact
+    const unsigned __int128 cur = (rem << 64) | e[i];
+    e[i] = (uint64_t)(cur / 3);
+    rem = cur % 3;
+  }
+  fr64 w = pow(from_u64(7), e);
+  for (int i = lg; i < 32; i++) w = mul(w, w);
+  return w;
+}
+
 inline fr64 domain_size_inverse(int lg) {
   uint64_t t[4] = {0, 0, 0, 0};
   t[lg >> 6] = 1ull << (lg & 63);

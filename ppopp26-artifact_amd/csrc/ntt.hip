@@ -242,19 +242,22 @@ done:
 // of the value times R' (the factor form of the pass kernels' product)
 hipError_t coset_table_build(fr** cp, size_t* bytes, int lg_n, int lg_blowup, const host::fr64& shift,
                              hipStream_t s) {
+  return coset_table_build_n(cp, bytes, lg_n, 1u << lg_blowup, shift, host::root_of_unity(lg_n + lg_blowup), s);
+}
+
+hipError_t coset_table_build_n(fr** cp, size_t* bytes, int lg_n, uint32_t B, const host::fr64& shift,
+                               const host::fr64& step, hipStream_t s) {
   const uint64_t n = 1ull << lg_n;
-  const uint64_t B = 1ull << lg_blowup;
   *bytes = (n * B) * sizeof(fr);
   hipError_t e = hipMalloc((void**)cp, *bytes);
   if (e != hipSuccess) return e;
   fr* win = nullptr;
   if ((e = hipMalloc((void**)&win, 2 * 16384 * sizeof(fr))) != hipSuccess) return e;
-  const host::fr64 wM = host::root_of_unity(lg_n + lg_blowup);
   const host::fr64 c32 = host::from_u64(32);  // stored form -> c R' mod r
   host::fr64 st = shift;
   for (uint64_t t = 0; t < B; t++) {
     if ((e = fill_pow(*cp + t * n, n, 0, st, &c32, win, s)) != hipSuccess) break;
-    st = host::mul(st, wM);
+    st = host::mul(st, step);
   }
   if (e == hipSuccess) {
     hipLaunchKernelGGL(pbd_kernel, dim3(grid_for(n * B, 256)), dim3(256), 0, s, *cp, n * B);
@@ -265,13 +268,28 @@ hipError_t coset_table_build(fr** cp, size_t* bytes, int lg_n, int lg_blowup, co
   return e;
 }
 
+// lg_blowup < 0: blowup 3, cosets shift * w_3N^t (N = 2^lg_n)
 hipError_t lde_domain_build(LdeDomain& d, int lg_n, int lg_blowup, const host::fr64& shift, hipStream_t s) {
   d.lg_n = lg_n;
-  d.lg_blowup = lg_blowup;
+  d.lg_blowup = lg_blowup < 0 ? 0 : lg_blowup;
+  d.blowup = lg_blowup < 0 ? 3 : 1 << lg_blowup;
   d.shift = shift;
   hipError_t e = domain_build(d.ntt, lg_n, s);
   if (e != hipSuccess) return e;
+  if (lg_blowup < 0) return coset_table_build_n(&d.cp, &d.bytes, lg_n, 3, shift, host::root_of_unity_3n(lg_n), s);
   return coset_table_build(&d.cp, &d.bytes, lg_n, lg_blowup, shift, s);
+}
+
+__global__ void interleave3_kernel(const fr* __restrict__ in, fr* __restrict__ out, uint64_t n) {
+  const uint64_t k = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (k >= 3 * n) return;
+  const uint64_t j = k / 3, t = k - 3 * j;
+  fr_store(out + k, fr_load(in + t * n + j));
+}
+
+hipError_t interleave3(const fr* in, fr* out, uint64_t n, hipStream_t s) {
+  hipLaunchKernelGGL(interleave3_kernel, dim3(grid_for(3 * n, 256)), dim3(256), 0, s, in, out, n);
+  return hipGetLastError();
 }
 
 // The reference's windowed power tables (parameters.cuh:7-33; the first

@@ -55,6 +55,7 @@ struct Domain {
 // Coset tables for the LDE: cp[t << lg_n | i] = (shift * w_M^t)^i.
 struct LdeDomain {
   int lg_n = 0, lg_blowup = 0;
+  int blowup = 0;           // cosets: 2^lg_blowup, or 3 (shifts shift * w_3N^t, output interleave 3)
   Domain ntt;               // size-2^lg_n transform
   fr* cp = nullptr;         // 2^(lg_n + lg_blowup) entries
   size_t bytes = 0;
@@ -65,6 +66,11 @@ hipError_t lde_domain_build(LdeDomain& d, int lg_n, int lg_blowup, const host::f
 // the coset factor table alone (*bytes = its size)
 hipError_t coset_table_build(fr** cp, size_t* bytes, int lg_n, int lg_blowup, const host::fr64& shift,
                              hipStream_t s);
+// B cosets: cp[t << lg_n | i] = (shift * step^t)^i as pbd words of the value times R'
+hipError_t coset_table_build_n(fr** cp, size_t* bytes, int lg_n, uint32_t B, const host::fr64& shift,
+                               const host::fr64& step, hipStream_t s);
+// out[3 j + t] = in[t N + j], t < 3, j < N (the blowup-3 LDE's interleave)
+hipError_t interleave3(const fr* in, fr* out, uint64_t n, hipStream_t s);
 void lde_domain_free(LdeDomain& d);
 
 // Launch geometry for one call: a batch of `npoly` polynomials of size 2^lg.

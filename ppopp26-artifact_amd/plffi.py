@@ -63,6 +63,9 @@ SIGNATURES = {
     "pl_lde_domain_bytes": (ctypes.c_size_t, [_vp]),
     "pl_lde_work_bytes": (ctypes.c_size_t, [_i, _i]),
     "pl_lde": (_i, [_vp, _vp, _vp, _vp, _vp]),
+    "pl_lde3_domain_create": (_i, [_i, _vp, ctypes.POINTER(_vp)]),
+    "pl_lde_domain_work_bytes": (ctypes.c_size_t, [_vp]),
+    "pl_lde_domain_blowup": (_i, [_vp]),
     "pl_coset_domain_create": (_i, [_i, _vp, ctypes.POINTER(_vp)]),
     "pl_coset_domain_destroy": (_i, [_vp]),
     "pl_coset_domain_bytes": (ctypes.c_size_t, [_vp]),
@@ -228,7 +231,8 @@ class LdeDomain:
     """Coset tables for our_lde: 2^lg_n coefficients -> 2^(lg_n+lg_blowup) values.
     shift: None (the reference's 7^(2^lg_blowup)) or a numpy (1, 4) uint64 Montgomery element."""
 
-    def __init__(self, lg_n, lg_blowup, shift=None):
+    def __init__(self, lg_n, lg_blowup, shift=None, blowup3=False):
+        """blowup3: the blowup-3 domain (cosets shift * w_3N^t; lg_blowup ignored)"""
         import numpy as np
         self.lg_n, self.lg_blowup = lg_n, lg_blowup
         h = ctypes.c_void_p()
@@ -236,8 +240,13 @@ class LdeDomain:
         if shift is not None:
             self._shift = np.ascontiguousarray(shift, dtype=np.uint64).reshape(4)
             sp = ctypes.c_void_p(self._shift.ctypes.data)
-        check("pl_lde_domain_create", lib().pl_lde_domain_create(lg_n, lg_blowup, sp, ctypes.byref(h)))
+        if blowup3:
+            check("pl_lde3_domain_create", lib().pl_lde3_domain_create(lg_n, sp, ctypes.byref(h)))
+        else:
+            check("pl_lde_domain_create", lib().pl_lde_domain_create(lg_n, lg_blowup, sp, ctypes.byref(h)))
         self.h = h
+        self.blowup = lib().pl_lde_domain_blowup(h)
+        self.work_bytes = lib().pl_lde_domain_work_bytes(h)
 
     @property
     def nbytes(self):
@@ -260,7 +269,7 @@ def lde(c, dom, out=None, work=None, stream=None):
     n = 1 << dom.lg_n
     assert c.shape == (n, 4) and c.dtype == torch.int64 and c.is_cuda
     if out is None:
-        out = torch.empty((n << dom.lg_blowup, 4), dtype=torch.int64, device=c.device)
+        out = torch.empty((n * dom.blowup, 4), dtype=torch.int64, device=c.device)
     check("pl_lde", lib().pl_lde(_ptr(c), _ptr(out), dom.h, _ptr(work), _stream(stream)))
     return out
 

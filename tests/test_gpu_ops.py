@@ -47,6 +47,28 @@ def test_lde_vs_oracle(pl, lg_n, lb):
     assert_fr_equal(out, po.lde(c, lg_n, lb, po.ref_lde_shift(lb)), "lde %d %d" % (lg_n, lb))
 
 
+@pytest.mark.parametrize("lg_n", [1, 3, 6, 7, 12, 16, 20, 22])
+def test_lde_blowup3_vs_oracle(pl, lg_n):
+    """The blowup-3 LDE (3 coset NTTs of size N with shifts 7 w_3N^t, output
+    interleaved to the natural order of the 3N-point domain) against the
+    oracle, whose closed form is pinned by brute force; at 2^20 and 2^22 the
+    BASELINE pipeline on evaluations (iNTT first) as well."""
+    from conftest import assert_fr_equal
+    import torch
+    c = po.uniform(1 << lg_n, po.seed(po.OP_LDE, lg_n, 3))
+    print("lde blowup 3 lg=%d" % lg_n, flush=True)
+    dom = pl.LdeDomain(lg_n, 0, blowup3=True)
+    assert dom.blowup == 3
+    out = pl.to_host(pl.lde(pl.to_dev(c), dom))
+    assert_fr_equal(out, po.lde3(c, lg_n, po.from_u64(7)), "lde3 %d" % lg_n)
+    if lg_n >= 20:
+        coeffs = pl.ntt(pl.to_dev(c), pl.Domain(lg_n), inverse=True)
+        out = pl.to_host(pl.lde(coeffs, dom))
+        assert_fr_equal(out, po.lde3(po.ntt(c, lg_n, inverse=True), lg_n, po.from_u64(7)), "evals->lde3")
+    dom.close()
+    torch.cuda.empty_cache()
+
+
 @pytest.mark.parametrize("lg_n,lb", [(10, 1), (12, 3), (16, 2), (20, 2)])
 def test_lde_naive_vs_oracle(pl, lg_n, lb):
     """naive_lde (lde.cu:1367-1490): out[k] = NTT_N(c)[k mod N]"""

@@ -302,3 +302,30 @@ def test_coset_intt_inverts_coset_ntt(golden):
         if rest is not None:
             y[8:] = rest
             np.testing.assert_array_equal(po.coset_ntt(x, lg, shift), po.ntt(y, lg))
+
+
+def test_blowup3_lde_by_brute_force():
+    """or_lde3 against the closed form out[k] = sum_i c_i (g w^k)^i over the
+    3N-point domain, w = 7^((r-1)/(3N)) computed here in Python integers
+    (its order is exactly 3N and its cube is the N-th root of the tables)."""
+    Rinv = pow(2 ** 256, -1, R_MOD)
+
+    def val(a):  # Montgomery limbs -> the field value
+        return [x * Rinv % R_MOD for x in to_int(a)]
+
+    for lg in (0, 1, 2, 4, 5):
+        n = 1 << lg
+        w = pow(7, (R_MOD - 1) // (3 * n), R_MOD)
+        assert pow(w, 3 * n, R_MOD) == 1 and pow(w, n, R_MOD) != 1 and (n == 1 or pow(w, 3 * n // 2, R_MOD) != 1)
+        assert val(po.root_of_unity_3n(lg)) == [w]
+        if lg:
+            assert val(po.root_of_unity(lg)) == [pow(w, 3, R_MOD)]
+        c = po.uniform(n, 70 + lg)
+        ci = val(c)
+        for g in (7, 5):
+            got = val(po.lde3(c, lg, po.from_u64(g)))
+            want = []
+            for k in range(3 * n):
+                x = g * pow(w, k, R_MOD) % R_MOD
+                want.append(sum(ci[i] * pow(x, i, R_MOD) for i in range(n)) % R_MOD)
+            assert got == want, (lg, g)
