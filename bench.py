@@ -140,7 +140,7 @@ def bench_ntt(pl, synth, lg, steps, warmup, stream, peak):
     dom = pl.Domain(lg)
     x = dev_uniform(pl, n, synth.seed(synth.OP_NTT, lg))
     y = torch.empty_like(x)
-    work = pl.byte_buffer(pl.ntt_work_bytes(lg))
+    work = torch.empty_like(x)
     # correctness gate on the measured buffers: iNTT(NTT(x)) == x
     pl.ntt(x, dom, out=y, work=work)
     z = pl.ntt(y, dom, inverse=True, work=work)
@@ -233,7 +233,7 @@ def bench_lde(pl, synth, lg, lb, steps, warmup, stream, peak):
     dom = pl.LdeDomain(lg, lb)
     c = dev_uniform(pl, n, synth.seed(synth.OP_LDE, lg, lb))
     out = torch.empty((m, 4), dtype=torch.int64, device="cuda")
-    work = pl.byte_buffer(dom.work_bytes)
+    work = torch.empty((m, 4), dtype=torch.int64, device="cuda")
     step = lambda: pl.lde(c, dom, out=out, work=work)  # noqa: E731
     for _ in range(warmup):
         step()

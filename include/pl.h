@@ -69,18 +69,14 @@ int pl_ntt(const void* d_in, void* d_out, int lg, int inverse, const pl_domain* 
 /* npoly contiguous polynomials of size 2^lg in one launch sequence */
 int pl_ntt_batch(const void* d_in, void* d_out, int lg, uint32_t npoly, int inverse,
                  const pl_domain* d, void* d_work, pl_stream_t stream);
-/* workspace for pl_ntt_batch: the passes' intermediate split words (36 bytes
- * per element, see pl_ntt_pass) for lg >= 6 (any pass split of such a size
- * may need it), 0 below */
+/* workspace for pl_ntt_batch: npoly * 2^lg elements for lg >= 6 (any pass
+ * split of such a size may need it), 0 below */
 size_t pl_ntt_work_bytes(int lg, uint32_t npoly);
 
 /* Pass-level access to the same transform (the intermediate layout is this
- * library's own, and intermediate values are "split words" congruent to the
- * field elements, not canonical Montgomery values: nine signed 32-bit limbs,
- * limbs 0..7 as 32 bytes per element followed, after the npoly 2^lg
- * elements, by a plane of limb 8, 4 bytes per element; a buffer that holds
- * a pass's intermediate output needs pl_ntt_pass_buffer_bytes(lg, npoly)
- * bytes; the reference's per-chunk step with its own layout is
+ * library's own, and intermediate values are 32-byte packed balanced-digit
+ * words congruent to the field elements, not canonical Montgomery values;
+ * the reference's per-chunk step with its own layout is
  * pl_ntt_stage_no_rotate below).  Running passes
  * 0 .. pl_domain_num_passes(d)-1 in order, each reading the previous pass's
  * output, equals pl_ntt_batch.  Intermediate passes may run in place; the
@@ -90,7 +86,6 @@ size_t pl_ntt_work_bytes(int lg, uint32_t npoly);
  * with. */
 int pl_ntt_num_passes(int lg);
 int pl_domain_num_passes(const pl_domain* d);
-size_t pl_ntt_pass_buffer_bytes(int lg, uint32_t npoly);
 int pl_ntt_pass(const void* d_in, void* d_out, int lg, uint32_t npoly, int pass, int inverse,
                 const pl_domain* d, pl_stream_t stream);
 
@@ -208,10 +203,7 @@ int pl_lde_domain_blowup(const pl_lde_domain* d);
  * the whole vector: step 1 the first pass with the coset powers (d_in ->
  * d_out, may alias), step 2 the middle passes (in place on d_in), step 3 the
  * last pass into natural order (d_in -> d_out, distinct; inverse: times
- * g^-i); steps 1..3 in order equal pl_coset_ntt.  Between the three calls
- * the caller's buffer holds limbs 0..7 of the intermediate split words and
- * the domain holds limb 8, so one three-call transform per domain is in
- * flight at a time. */
+ * g^-i); steps 1..3 in order equal pl_coset_ntt. */
 typedef struct pl_coset_domain pl_coset_domain;
 int pl_coset_domain_create(int lg, const void* h_shift, pl_coset_domain** out);
 int pl_coset_domain_destroy(pl_coset_domain* d);

@@ -85,7 +85,7 @@ size_t pl_domain_bytes(const pl_domain* d) { return d ? d->d.bytes : 0; }
 // built under any split.
 size_t pl_ntt_work_bytes(int lg, uint32_t npoly) {
   if (lg < 1 || lg > 28) return 0;
-  return lg >= 2 * pl::kLgPerT ? pl::split_bytes((uint64_t)npoly << lg) : 0;
+  return lg >= 2 * pl::kLgPerT ? ((size_t)npoly << lg) * sizeof(pl::fr) : 0;
 }
 
 int pl_ntt_batch(const void* d_in, void* d_out, int lg, uint32_t npoly, int inverse, const pl_domain* d,
@@ -95,7 +95,7 @@ int pl_ntt_batch(const void* d_in, void* d_out, int lg, uint32_t npoly, int inve
   if (!aligned16(d_in) || !aligned16(d_out) || !aligned16(d_work)) return PL_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
   // the domain's own split decides whether the passes need the buffer
-  size_t wb = d->d.plan.P >= 2 ? pl::split_bytes((uint64_t)npoly << lg) : 0;
+  size_t wb = d->d.plan.P >= 2 ? ((size_t)npoly << lg) * sizeof(pl::fr) : 0;
   pl::fr* tmp = (pl::fr*)d_work;
   bool own = false;
   if (wb && !tmp) {
@@ -121,11 +121,6 @@ int pl_ntt_num_passes(int lg) {
   return p.P == 0 ? 1 : p.P;
 }
 
-size_t pl_ntt_pass_buffer_bytes(int lg, uint32_t npoly) {
-  if (lg < 1 || lg > 28 || npoly == 0) return 0;
-  return pl::split_bytes((uint64_t)npoly << lg);
-}
-
 int pl_domain_num_passes(const pl_domain* d) {
   if (!d) return PL_ERR_ARG;
   return d->d.plan.P == 0 ? 1 : d->d.plan.P;
@@ -142,12 +137,8 @@ int pl_ntt_pass(const void* d_in, void* d_out, int lg, uint32_t npoly, int pass,
   pl::NttCall c;
   c.npoly = npoly;
   c.inverse = inverse != 0;
-  // intermediate buffers hold split words: the limb-8 plane after the npoly N elements
-  const uint64_t total = (uint64_t)npoly << lg;
-  const uint32_t* in8 = pass > 0 ? pl::plane_of((const pl::fr*)d_in, total) : nullptr;
-  uint32_t* out8 = pass < np - 1 ? pl::plane_of((pl::fr*)d_out, total) : nullptr;
-  hipError_t e = pl::ntt_launch_one(d->d, c, pass, (const pl::fr*)d_in, in8, 1ull << lg, nullptr, nullptr,
-                                    (pl::fr*)d_out, out8, (hipStream_t)stream);
+  hipError_t e = pl::ntt_launch_one(d->d, c, pass, (const pl::fr*)d_in, 1ull << lg, nullptr, (pl::fr*)d_out,
+                                    (hipStream_t)stream);
   return e == hipSuccess ? PL_OK : hip_fail(e);
 }
 
@@ -193,7 +184,7 @@ size_t pl_lde_domain_bytes(const pl_lde_domain* d) { return d ? d->d.bytes + d->
 
 size_t pl_lde_work_bytes(int lg_n, int lg_blowup) {
   if (lg_n < 1 || lg_blowup < 1 || lg_n + lg_blowup > 28) return 0;
-  return pl::split_bytes(1ull << (lg_n + lg_blowup));
+  return (size_t)sizeof(pl::fr) << (lg_n + lg_blowup);
 }
 
 // blowup 3 (BASELINE's "3 coset NTTs of size N with shifts g w_3N^t"; no
@@ -224,8 +215,8 @@ int pl_lde3_domain_create(int lg_n, const void* h_shift, pl_lde_domain** out) {
 size_t pl_lde_domain_work_bytes(const pl_lde_domain* d) {
   if (!d) return 0;
   const size_t m = (size_t)d->d.blowup << d->d.lg_n;
-  // the passes' split words; blowup 3: then the cosets' outputs before the interleave
-  return pl::split_bytes(m) + (d->d.blowup == 3 ? m * sizeof(pl::fr) : 0);
+  // blowup 3: the pass scratch and the cosets' outputs before the interleave
+  return sizeof(pl::fr) * (d->d.blowup == 3 ? 2 * m : m);
 }
 
 int pl_lde_domain_blowup(const pl_lde_domain* d) { return d ? d->d.blowup : PL_ERR_ARG; }
@@ -250,11 +241,10 @@ int pl_lde(const void* d_coeffs, void* d_out, const pl_lde_domain* d, void* d_wo
   c.tmp = tmp;
   c.in_pstride = 0;  // every coset reads the same coefficients
   c.premul = d->d.cp;
-  c.premul8 = d->d.cp8();
   c.inverse = false;
   if (three) {
     // three coset NTTs side by side (coset t at t N), then out[3 j + t]
-    c.out = (pl::fr*)((char*)tmp + pl::split_bytes(3 * n));
+    c.out = tmp + 3 * n;
     c.npoly = 3;
     c.pm_mask = 3;
     c.lgI = 0;

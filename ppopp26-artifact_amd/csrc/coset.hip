@@ -26,7 +26,6 @@ struct pl_coset_domain {
   pl::fr* cp = nullptr;   // pbd words of g^i R' (the pass kernels' factor form), i < N
   pl::fr* pw = nullptr;   // g^i, stored form
   pl::fr* ipw = nullptr;  // g^-i, stored form
-  uint32_t* step8 = nullptr;  // limb 8 of the three-call form's intermediate words (one transform in flight)
   size_t bytes = 0;
 };
 
@@ -38,9 +37,7 @@ void coset_free(pl_coset_domain& d) {
   pl::domain_free(d.ntt);
   for (pl::fr* p : {d.cp, d.pw, d.ipw})
     if (p) (void)hipFree(p);
-  if (d.step8) (void)hipFree(d.step8);
   d.cp = d.pw = d.ipw = nullptr;
-  d.step8 = nullptr;
 }
 
 hipError_t coset_build(pl_coset_domain& d, int lg, const pl::host::fr64& shift) {
@@ -51,11 +48,10 @@ hipError_t coset_build(pl_coset_domain& d, int lg, const pl::host::fr64& shift) 
   const uint64_t n = 1ull << lg;
   if ((e = hipMalloc((void**)&d.pw, n * sizeof(pl::fr))) != hipSuccess) return e;
   if ((e = hipMalloc((void**)&d.ipw, n * sizeof(pl::fr))) != hipSuccess) return e;
-  if ((e = hipMalloc((void**)&d.step8, n * sizeof(uint32_t))) != hipSuccess) return e;
   const pl::host::fr64 one = pl::host::one();
   if ((e = pl::fill_pow_table(d.pw, n, shift, one, nullptr)) != hipSuccess) return e;
   if ((e = pl::fill_pow_table(d.ipw, n, pl::host::inv(shift), one, nullptr)) != hipSuccess) return e;
-  d.bytes = cpb + 2 * n * sizeof(pl::fr) + n * sizeof(uint32_t) + d.ntt.bytes;
+  d.bytes = cpb + 2 * n * sizeof(pl::fr) + d.ntt.bytes;
   return hipSuccess;
 }
 
@@ -103,7 +99,7 @@ size_t pl_coset_domain_bytes(const pl_coset_domain* d) { return d ? d->bytes : 0
 
 size_t pl_coset_work_bytes(int lg) {
   if (lg < 1 || lg > 28) return 0;
-  return pl::split_bytes(1ull << lg);
+  return sizeof(pl::fr) << lg;
 }
 
 // [Ntt_coset / Intt_coset: init + forward1 + forward2 + forward3 as one call]
@@ -128,7 +124,6 @@ int pl_coset_ntt(const void* d_in, void* d_out, int inverse, const pl_coset_doma
   c.inverse = inverse != 0;
   if (!inverse) {
     c.premul = d->cp;  // g^i on the first pass's loads
-    c.premul8 = pl::plane_of((const pl::fr*)d->cp, n);
     c.pm_mask = 0;
   }
   hipError_t e = pl::ntt_launch(d->ntt, c, s);
@@ -180,20 +175,15 @@ int pl_coset_ntt_step(const void* d_in, void* d_out, int step, int inverse, cons
       if (!inverse) e = pl::vec_op(kMul, (const pl::fr*)d_in, d->pw, (pl::fr*)d_out, n, nullptr, s);
       else if (d_in != d_out) e = hipMemcpyAsync(d_out, d_in, n * sizeof(pl::fr), hipMemcpyDeviceToDevice, s);
     } else if (step == 3) {
-      e = pl::ntt_launch_one(d->ntt, c, 0, (const pl::fr*)d_in, nullptr, n, nullptr, nullptr, (pl::fr*)d_out,
-                             nullptr, s);
+      e = pl::ntt_launch_one(d->ntt, c, 0, (const pl::fr*)d_in, n, nullptr, (pl::fr*)d_out, s);
     }
   } else if (step == 1) {
-    const uint32_t* pm8 = inverse ? nullptr : pl::plane_of((const pl::fr*)d->cp, n);
-    e = pl::ntt_launch_one(d->ntt, c, 0, (const pl::fr*)d_in, nullptr, n, inverse ? nullptr : d->cp, pm8,
-                           (pl::fr*)d_out, d->step8, s);
+    e = pl::ntt_launch_one(d->ntt, c, 0, (const pl::fr*)d_in, n, inverse ? nullptr : d->cp, (pl::fr*)d_out, s);
   } else if (step == 2) {
     for (int p = 1; p + 1 < np && e == hipSuccess; p++)
-      e = pl::ntt_launch_one(d->ntt, c, p, (const pl::fr*)d_in, d->step8, n, nullptr, nullptr, (pl::fr*)d_in,
-                             d->step8, s);
+      e = pl::ntt_launch_one(d->ntt, c, p, (const pl::fr*)d_in, n, nullptr, (pl::fr*)d_in, s);
   } else {
-    e = pl::ntt_launch_one(d->ntt, c, np - 1, (const pl::fr*)d_in, d->step8, n, nullptr, nullptr, (pl::fr*)d_out,
-                           nullptr, s);
+    e = pl::ntt_launch_one(d->ntt, c, np - 1, (const pl::fr*)d_in, n, nullptr, (pl::fr*)d_out, s);
   }
   if (e == hipSuccess && step == 3 && inverse)
     e = pl::vec_op(kMul, (const pl::fr*)d_out, d->ipw, (pl::fr*)d_out, n, nullptr, s);

@@ -391,35 +391,6 @@ __device__ __forceinline__ fr v9_canonical(const v9& v) {
   return fr_reduce_once(o);
 }
 
-// ---- split words: nine signed limbs in HBM without packing ------------------
-// An array of T split words is T x 32 bytes holding limbs 0..7 as 32-bit
-// integers, followed by a plane of T x 4 bytes holding limb 8 (the plane of
-// a buffer starting at t is (uint32_t*)(t + T)).  Loads are two aligned
-// 16-byte loads and one 4-byte load with no unpacking; the pass kernels hand
-// intermediate results from pass to pass this way, and their factor tables
-// (pass and inner twiddles, coset powers) hold balanced digits this way.
-__host__ __device__ inline const uint32_t* split_plane(const fr* t, uint64_t count) {
-  return reinterpret_cast<const uint32_t*>(t + count);
-}
-__host__ __device__ inline uint32_t* split_plane(fr* t, uint64_t count) {
-  return reinterpret_cast<uint32_t*>(t + count);
-}
-
-__device__ __forceinline__ void load_split(const fr* t, const uint32_t* t8, uint64_t i, uint32_t (&w)[9]) {
-  const fr v = fr_load(t + i);
-#pragma unroll
-  for (int k = 0; k < 8; k++) w[k] = v.l[k];
-  w[8] = t8[i];
-}
-
-__device__ __forceinline__ void store_split(fr* t, uint32_t* t8, uint64_t i, const uint32_t* w) {
-  fr v;
-#pragma unroll
-  for (int k = 0; k < 8; k++) v.l[k] = w[k];
-  fr_store(t + i, v);
-  t8[i] = w[8];
-}
-
 // an ordering point for the nine limbs of v (no instruction)
 __device__ __forceinline__ void pin9(v9& v) {
   asm volatile("" : "+v"(v.l[0]), "+v"(v.l[1]), "+v"(v.l[2]), "+v"(v.l[3]), "+v"(v.l[4]), "+v"(v.l[5]),

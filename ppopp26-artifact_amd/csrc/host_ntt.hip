@@ -42,9 +42,6 @@ struct Streams {
   }
 };
 
-// one device slot: the polynomial (natural order) and the passes' split words
-size_t slot_bytes(uint64_t n) { return n * sizeof(fr) + split_bytes(n); }
-
 uint64_t pow2_groups(int lg_k, uint64_t lines, uint64_t lines_per_tile) {
   uint64_t g = 1ull << std::min(std::max(lg_k, 0), 20);
   const uint64_t tiles = std::max<uint64_t>(1, lines / std::max<uint64_t>(1, lines_per_tile));
@@ -87,9 +84,8 @@ static hipError_t ntt_host(const Domain& d, const fr* h_in, fr* h_out, uint32_t 
   c.inverse = inverse;
   for (uint32_t i = 0; i < npoly && e == hipSuccess; i++) {
     const int k = (int)(i % nslots);
-    fr* data = (fr*)((char*)work + (uint64_t)k * slot_bytes(N));
-    fr* tmp = data + N;  // split words: limbs 0..7, then the limb-8 plane
-    uint32_t* tmp8 = plane_of(tmp, N);
+    fr* data = work + (uint64_t)k * 2 * N;
+    fr* tmp = data + N;
     const fr* src = h_in + (uint64_t)i * N;
     fr* dst = h_out + (uint64_t)i * N;
     if (i >= (uint32_t)nslots && (e = hipStreamWaitEvent(st.cin, slot_free[k], 0)) != hipSuccess) break;
@@ -113,15 +109,14 @@ static hipError_t ntt_host(const Domain& d, const fr* h_in, fr* h_out, uint32_t 
                              (size_t)1 << m0, hipMemcpyHostToDevice, st.cin);
         if (e == hipSuccess) e = hipEventRecord(ev_in[g], st.cin);
         if (e == hipSuccess) e = hipStreamWaitEvent(s, ev_in[g], 0);
-        if (e == hipSuccess) e = ntt_launch_part(d, c, 0, data, nullptr, tmp, tmp8, j0, j0, W0, s);
+        if (e == hipSuccess) e = ntt_launch_part(d, c, 0, data, tmp, j0, j0, W0, s);
       }
       // middle passes in place on the whole vector
-      for (int p = 1; p + 1 < P.P && e == hipSuccess; p++)
-        e = ntt_launch_one(d, c, p, tmp, tmp8, N, nullptr, nullptr, tmp, tmp8, s);
+      for (int p = 1; p + 1 < P.P && e == hipSuccess; p++) e = ntt_launch_one(d, c, p, tmp, N, nullptr, tmp, s);
       // last pass by line groups; each group's outputs leave at once
       for (uint64_t g = 0; g < G1 && e == hipSuccess; g++) {
         const uint64_t l0 = g * W1;
-        e = ntt_launch_part(d, c, P.P - 1, tmp, tmp8, data, nullptr, last_pass_row(P, l0) << mL, l0, W1, s);
+        e = ntt_launch_part(d, c, P.P - 1, tmp, data, last_pass_row(P, l0) << mL, l0, W1, s);
         if (e == hipSuccess) e = hipEventRecord(ev_out[g], s);
         if (e == hipSuccess) e = hipStreamWaitEvent(st.cout, ev_out[g], 0);
         if (e == hipSuccess)
@@ -149,7 +144,7 @@ extern "C" {
 
 size_t pl_ntt_host_work_bytes(int lg, uint32_t npoly) {
   if (lg < 1 || lg > 28 || npoly == 0) return 0;
-  return (size_t)(npoly > 1 ? 2 : 1) * pl::slot_bytes(1ull << lg);
+  return (size_t)(npoly > 1 ? 2 : 1) * 2 * (sizeof(pl::fr) << lg);
 }
 
 // [our_ntt, ntt.cu:979-1037]

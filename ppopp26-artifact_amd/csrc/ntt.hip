@@ -6,12 +6,12 @@
 
 namespace pl {
 
-// x times a premultiplication factor (split word i of c R'), canonical: x c
-__device__ __forceinline__ fr premul_one(const fr& x, const fr* f, const uint32_t* f8, uint64_t i) {
+// x times a premultiplication factor (pbd word of c R'), canonical: x c
+__device__ __forceinline__ fr premul_one(const fr& x, const fr& f) {
   v9 a;
   unpack_ext(x.l, a.l);
   uint32_t w[9];
-  load_split(f, f8, i, w);
+  pbd_unpack(f.l, w);
   return v9_canonical(v9_mul<false, false, false>(a, w));
 }
 
@@ -28,10 +28,10 @@ __global__ void __launch_bounds__(256) ntt_tiny_kernel(PassArgs a, uint32_t npol
   fr acc = fr_zero();
   if (valid) {
     const fr* src = a.in + poly * a.in_pstride;
-    const uint64_t pmo = (uint64_t)(poly & a.pm_mask) << lg;
+    const fr* pm = a.premul ? a.premul + ((uint64_t)(poly & a.pm_mask) << lg) : nullptr;
     for (uint32_t j = 0; j < n; j++) {
       fr v = fr_load(src + j);
-      if (a.premul) v = premul_one(v, a.premul, a.premul8, pmo + j);
+      if (pm) v = premul_one(v, fr_load(pm + j));
       const fr w = fr_load(a.inner + (((j * k) & (n - 1)) << (8 - lg)));
       acc = fr_add(acc, fr_mul(v, w));
     }
@@ -47,13 +47,12 @@ __global__ void __launch_bounds__(256) ntt_tiny_kernel(PassArgs a, uint32_t npol
 
 // Single-pass sizes with a per-element factor table (small coset NTTs):
 // out[poly][i] = in[poly * in_pstride + i] * premul[(poly & pm_mask) << lg | i]
-__global__ void premul_kernel(const fr* in, fr* out, uint64_t in_pstride, const fr* premul, const uint32_t* premul8,
-                              uint32_t pm_mask, uint32_t lg, uint64_t total) {
+__global__ void premul_kernel(const fr* in, fr* out, uint64_t in_pstride, const fr* premul, uint32_t pm_mask,
+                              uint32_t lg, uint64_t total) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= total) return;
   uint64_t poly = i >> lg, pos = i & ((1ull << lg) - 1);
-  fr v = premul_one(fr_load(in + poly * in_pstride + pos), premul, premul8,
-                    ((uint64_t)(poly & pm_mask) << lg) + pos);
+  fr v = premul_one(fr_load(in + poly * in_pstride + pos), fr_load(premul + ((uint64_t)(poly & pm_mask) << lg) + pos));
   fr_store(out + i, v);
 }
 
@@ -78,15 +77,14 @@ __global__ void pow_window_kernel(fr* out, uint64_t n, const fr* w0, const fr* w
   fr_store(out + i, v);
 }
 
-// pass-kernel tables: w R' mod r -> split words of its balanced digits, in
-// place, limb 8 into the plane t8
-__global__ void split_table_kernel(fr* t, uint32_t* t8, uint64_t n) {
+// pass-kernel tables: w R' mod r -> pbd words (balanced digits), in place
+__global__ void pbd_kernel(fr* t, uint64_t n) {
   uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if (i >= n) return;
   const fr v = fr_load(t + i);
-  uint32_t b[9];
-  unpack_ext(v.l, b);
-  store_split(t, t8, i, b);
+  fr o;
+  to_pbd(v.l, o.l);
+  fr_store(t + i, o);
 }
 
 static inline fr to_dev(const host::fr64& h) {
@@ -186,10 +184,7 @@ hipError_t domain_build(Domain& d, int lg, hipStream_t s) {
   const NttPlan& P = d.plan;
   size_t count = 4 * 256 + 2 * 16384;  // inner tables (both forms), w_N power windows
   for (int p = 0; p + 1 < P.P; p++) count += 2 * (1ull << P.lgM[p]);
-  // the pass kernels' tables (from inner29_fwd to the end) are split words:
-  // their limb-8 plane follows the block's elements
-  const uint64_t n29 = count - (2 * 256 + 2 * 16384);
-  d.bytes = count * sizeof(fr) + ((n29 * 4 + 255) & ~(uint64_t)255);
+  d.bytes = count * sizeof(fr);
   hipError_t e = hipMalloc((void**)&d.block, d.bytes);
   if (e != hipSuccess) return e;
   fr* cur = d.block;
@@ -202,16 +197,6 @@ hipError_t domain_build(Domain& d, int lg, hipStream_t s) {
   for (int p = 0; p + 1 < P.P; p++) {
     d.tw_fwd[p] = cur; cur += 1ull << P.lgM[p];
     d.tw_inv[p] = cur; cur += 1ull << P.lgM[p];
-  }
-  {
-    // plane entry k belongs to the table entry inner29_fwd + k
-    uint32_t* plane = split_plane(d.block, count);
-    d.inner29_fwd8 = plane;
-    d.inner29_inv8 = plane + (d.inner29_inv - d.inner29_fwd);
-    for (int p = 0; p + 1 < P.P; p++) {
-      d.tw_fwd8[p] = plane + (d.tw_fwd[p] - d.inner29_fwd);
-      d.tw_inv8[p] = plane + (d.tw_inv[p] - d.inner29_fwd);
-    }
   }
   fr* win = nullptr;
   e = hipMalloc((void**)&win, 2 * 16384 * sizeof(fr));
@@ -243,8 +228,8 @@ hipError_t domain_build(Domain& d, int lg, hipStream_t s) {
   }
   {
     // the pass kernels' tables (inner29_*, tw_*) are contiguous to the end
-    hipLaunchKernelGGL(split_table_kernel, dim3(grid_for(n29, 256)), dim3(256), 0, s, d.inner29_fwd, d.inner29_fwd8,
-                       n29);
+    const uint64_t n29 = (uint64_t)(cur - d.inner29_fwd);
+    hipLaunchKernelGGL(pbd_kernel, dim3(grid_for(n29, 256)), dim3(256), 0, s, d.inner29_fwd, n29);
     if ((e = hipGetLastError()) != hipSuccess) goto done;
   }
   e = hipStreamSynchronize(s);
@@ -263,7 +248,7 @@ hipError_t coset_table_build(fr** cp, size_t* bytes, int lg_n, int lg_blowup, co
 hipError_t coset_table_build_n(fr** cp, size_t* bytes, int lg_n, uint32_t B, const host::fr64& shift,
                                const host::fr64& step, hipStream_t s) {
   const uint64_t n = 1ull << lg_n;
-  *bytes = split_bytes(n * B);  // split words: the limb-8 plane follows the entries
+  *bytes = (n * B) * sizeof(fr);
   hipError_t e = hipMalloc((void**)cp, *bytes);
   if (e != hipSuccess) return e;
   fr* win = nullptr;
@@ -275,8 +260,7 @@ hipError_t coset_table_build_n(fr** cp, size_t* bytes, int lg_n, uint32_t B, con
     st = host::mul(st, step);
   }
   if (e == hipSuccess) {
-    hipLaunchKernelGGL(split_table_kernel, dim3(grid_for(n * B, 256)), dim3(256), 0, s, *cp, split_plane(*cp, n * B),
-                       n * B);
+    hipLaunchKernelGGL(pbd_kernel, dim3(grid_for(n * B, 256)), dim3(256), 0, s, *cp, n * B);
     e = hipGetLastError();
   }
   if (e == hipSuccess) e = hipStreamSynchronize(s);
@@ -345,15 +329,13 @@ static void launch_pass_la(int la, bool inv, bool last, const PassArgs& a, hipSt
 
 // Kernel arguments of pass p of the plan, reading `in` (poly stride
 // in_pstride, optional premul table) and writing `out`.
-static void fill_pass_args(const Domain& d, const NttCall& c, int p, const fr* in, const uint32_t* in8,
-                           uint64_t in_pstride, const fr* premul, const uint32_t* premul8, fr* out, uint32_t* out8,
-                           PassArgs& a) {
+static void fill_pass_args(const Domain& d, const NttCall& c, int p, const fr* in, uint64_t in_pstride,
+                           const fr* premul, fr* out, PassArgs& a) {
   const NttPlan& P = d.plan;
   const int lg = d.lg;
   memset(&a, 0, sizeof(a));
   a.inner = c.inverse ? d.inner_inv : d.inner_fwd;
   a.inner29 = c.inverse ? d.inner29_inv : d.inner29_fwd;
-  a.inner29_8 = c.inverse ? d.inner29_inv8 : d.inner29_fwd8;
   a.lgN = lg;
   a.lgI = c.lgI;
   a.pm_mask = c.pm_mask;
@@ -364,19 +346,15 @@ static void fill_pass_args(const Domain& d, const NttCall& c, int p, const fr* i
   }
   a.in_ext = (P.P == 0 || p == 0) ? 1 : 0;
   a.in = in;
-  a.in8 = in8;
   a.out = out;
-  a.out8 = out8;
   a.in_pstride = in_pstride;
   a.premul = premul;
-  a.premul8 = premul8;
   if (P.P == 0) {
     a.do_scale = c.inverse ? 1 : 0;
     return;
   }
   const bool last = p == P.P - 1;
   a.tw = last ? nullptr : (c.inverse ? d.tw_inv[p] : d.tw_fwd[p]);
-  a.tw8 = last ? nullptr : (c.inverse ? d.tw_inv8[p] : d.tw_fwd8[p]);
   a.lgMp = P.lgM[p];
   a.lgS = P.lgS[p];
   a.nlines = (uint64_t)c.npoly << (lg - P.m[p]);
@@ -389,27 +367,24 @@ static void fill_pass_args(const Domain& d, const NttCall& c, int p, const fr* i
 
 // Pass p of the plan.  Intermediate passes keep the in-place layout, so in
 // may equal out for them; the last pass writes natural order.
-hipError_t ntt_launch_one(const Domain& d, const NttCall& c, int p, const fr* in, const uint32_t* in8,
-                          uint64_t in_pstride, const fr* premul, const uint32_t* premul8, fr* out, uint32_t* out8,
-                          hipStream_t s) {
+hipError_t ntt_launch_one(const Domain& d, const NttCall& c, int p, const fr* in, uint64_t in_pstride,
+                          const fr* premul, fr* out, hipStream_t s) {
   const NttPlan& P = d.plan;
   const int lg = d.lg;
   if (premul && (P.P == 0 || p == P.P - 1)) {
     // the last-pass and tiny kernels carry no factor table (register budget):
-    // scale the input into the scratch buffer first (canonical values, read
-    // as caller data).  The scratch copy also makes a shared-input call
-    // (in_pstride 0) safe when out aliases in.
+    // scale the input into the scratch buffer first.  The scratch copy also
+    // makes a shared-input call (in_pstride 0) safe when out aliases in.
     if (!c.tmp) return hipErrorInvalidValue;
     uint64_t total = (uint64_t)c.npoly << lg;
     hipLaunchKernelGGL(premul_kernel, dim3(grid_for(total, 256)), dim3(256), 0, s, in, c.tmp, in_pstride, premul,
-                       premul8, c.pm_mask, (uint32_t)lg, total);
+                       c.pm_mask, (uint32_t)lg, total);
     in = c.tmp;
     in_pstride = 1ull << lg;
     premul = nullptr;
-    premul8 = nullptr;
   }
   PassArgs a;
-  fill_pass_args(d, c, p, in, in8, in_pstride, premul, premul8, out, out8, a);
+  fill_pass_args(d, c, p, in, in_pstride, premul, out, a);
   if (P.P == 0) {
     uint64_t total = (uint64_t)c.npoly << lg;
     hipLaunchKernelGGL(ntt_tiny_kernel, dim3(grid_for(total, 256)), dim3(256), 0, s, a, c.npoly);
@@ -420,23 +395,19 @@ hipError_t ntt_launch_one(const Domain& d, const NttCall& c, int p, const fr* in
 }
 
 // Pass p of a single-polynomial transform on `nlines` of its lines: the
-// in/out pointers and planes advanced by in_off/out_off elements (the pass
-// twiddles follow the output column).  Pass 0's lines are columns j at
-// stride N >> m[0], so a group of consecutive columns is in_off = out_off =
-// j0; the last pass's line lp writes lp + (N >> m[P-1]) k and reads sub-row
+// in/out pointers advanced by in_off/out_off elements (the pass twiddles
+// follow the output column).  Pass 0's lines are columns j at stride
+// N >> m[0], so a group of consecutive columns is in_off = out_off = j0; the
+// last pass's line lp writes lp + (N >> m[P-1]) k and reads sub-row
 // digit-reversed(lp), a bit permutation, so a group of lines g*W .. g*W+W-1
 // (W a power of two) is out_off = g*W, in_off = pi(g*W) << m[P-1].
-hipError_t ntt_launch_part(const Domain& d, const NttCall& c, int p, const fr* in, const uint32_t* in8, fr* out,
-                           uint32_t* out8, uint64_t in_off, uint64_t out_off, uint64_t nlines, hipStream_t s) {
+hipError_t ntt_launch_part(const Domain& d, const NttCall& c, int p, const fr* in, fr* out, uint64_t in_off,
+                           uint64_t out_off, uint64_t nlines, hipStream_t s) {
   const NttPlan& P = d.plan;
   if (P.P < 2 || c.premul || c.npoly != 1 || p < 0 || p >= P.P) return hipErrorInvalidValue;
   PassArgs a;
-  fill_pass_args(d, c, p, in + in_off, in8 ? in8 + in_off : nullptr, 1ull << d.lg, nullptr, nullptr, out + out_off,
-                 out8 ? out8 + out_off : nullptr, a);
-  if (a.tw) {
-    a.tw += out_off;
-    a.tw8 += out_off;
-  }
+  fill_pass_args(d, c, p, in + in_off, 1ull << d.lg, nullptr, out + out_off, a);
+  if (a.tw) a.tw += out_off;
   a.nlines = nlines;
   launch_pass_la(P.m[p] - kLgPerT, c.inverse, p == P.P - 1, a, s);
   return hipGetLastError();
@@ -458,23 +429,18 @@ bool pass_args_regular(const Domain& d, const NttCall& c, int p, PassArgs& a) {
   const NttPlan& P = d.plan;
   if (P.P == 0 || (c.premul && P.P == 1) || p < 0 || p >= P.P) return false;
   const bool last = p == P.P - 1;
-  const uint64_t total = (uint64_t)c.npoly << d.lg;
-  uint32_t* tmp8 = c.tmp ? plane_of(c.tmp, total) : nullptr;
-  fill_pass_args(d, c, p, p == 0 ? c.in : c.tmp, p == 0 ? nullptr : tmp8, p == 0 ? c.in_pstride : (1ull << d.lg),
-                 p == 0 ? c.premul : nullptr, p == 0 ? c.premul8 : nullptr, last ? c.out : c.tmp,
-                 last ? nullptr : tmp8, a);
+  fill_pass_args(d, c, p, p == 0 ? c.in : c.tmp, p == 0 ? c.in_pstride : (1ull << d.lg), p == 0 ? c.premul : nullptr,
+                 last ? c.out : c.tmp, a);
   return true;
 }
 
 hipError_t ntt_launch(const Domain& d, const NttCall& c, hipStream_t s) {
   const int np = d.plan.P == 0 ? 1 : d.plan.P;
   const uint64_t N = 1ull << d.lg;
-  uint32_t* tmp8 = c.tmp ? plane_of(c.tmp, (uint64_t)c.npoly << d.lg) : nullptr;
   for (int p = 0; p < np; p++) {
     const bool last = p == np - 1;
-    hipError_t e = ntt_launch_one(d, c, p, p == 0 ? c.in : c.tmp, p == 0 ? nullptr : tmp8, p == 0 ? c.in_pstride : N,
-                                  p == 0 ? c.premul : nullptr, p == 0 ? c.premul8 : nullptr, last ? c.out : c.tmp,
-                                  last ? nullptr : tmp8, s);
+    hipError_t e = ntt_launch_one(d, c, p, p == 0 ? c.in : c.tmp, p == 0 ? c.in_pstride : N,
+                                  p == 0 ? c.premul : nullptr, last ? c.out : c.tmp, s);
     if (e != hipSuccess) return e;
   }
   return hipSuccess;

@@ -117,15 +117,10 @@ __device__ __forceinline__ void dft_dif(v9 (&x)[NX]) {
 struct PassArgs {
   const fr* in;
   fr* out;
-  const uint32_t* in8;  // !in_ext: limb 8 of the input's split words (same index as in)
-  uint32_t* out8;       // !LAST: limb 8 of the output's split words (same index as out)
-  const fr* premul;     // !LAST: optional factor per input element [(poly & pm_mask) << lgN | pos], split words of c R'
-  const uint32_t* premul8;
-  const fr* tw;         // !LAST: pass twiddles (split words of w R' mod r), index j + S*k within the sub-problem
-  const uint32_t* tw8;
+  const fr* premul;     // !LAST: optional factor per input element [(poly & pm_mask) << lgN | pos], pbd words of c R''
+  const fr* tw;         // !LAST: pass twiddles (pbd words of w R' mod r), index j + S*k within the sub-problem
   const fr* inner;      // w_256^e (or w_256^-e), e < 256, stored form (small-size kernel)
-  const fr* inner29;    // the same as split words of w R' mod r (pass kernels)
-  const uint32_t* inner29_8;
+  const fr* inner29;    // the same as pbd words of w R' mod r (pass kernels)
   uint64_t in_pstride;  // element stride between input polys (0: shared input)
   uint64_t nlines;      // lines (columns) in this launch
   uint32_t pm_mask;
@@ -135,7 +130,7 @@ struct PassArgs {
   uint32_t lgI;         // LAST: output interleave
   uint32_t ndig;        // LAST: number of earlier passes
   uint32_t do_scale;    // multiply every output by `scale` (single-pass transforms)
-  uint32_t in_ext;      // the input is caller data (any 256-bit values), not a previous pass's split words
+  uint32_t in_ext;      // the input is caller data (any 256-bit values), not a previous pass's pbd words
   uint32_t lgrad[PL_MAX_PASSES];  // LAST: log2 radices of the earlier passes, pass 1 first
   fr scale;             // stored form (small-size kernel)
   fr scale29;           // pbd word of scale R' mod r (pass kernels)
@@ -269,9 +264,7 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, const uint64_t 
     // branch loads and unpacks on its own, so no loaded word lives across it.
     if (a.in_ext) {
       if (!LAST && a.premul) {
-        const uint64_t pmo = ((uint64_t)(poly & a.pm_mask) << a.lgN) + pos0 + (uint64_t)rho1 * estride;
-        const fr* pm = a.premul + pmo;
-        const uint32_t* pm8 = a.premul8 + pmo;
+        const fr* pm = a.premul + ((uint64_t)(poly & a.pm_mask) << a.lgN) + pos0 + (uint64_t)rho1 * estride;
         const fr* sp = src + pos0 + (uint64_t)rho1 * estride;
         // factors are pbd words of c R' (the coset powers); the product with
         // balanced output digits lies in (-r, 0.03 r) and leaves balanced
@@ -282,16 +275,13 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, const uint64_t 
         static_for<0, kPerT / kPmG>([&](auto g_) {
           constexpr int g = decltype(g_)::value;
           const fr* q = pm;
-          const uint32_t* q8 = pm8;
           const fr* d = sp;
-          asm volatile("" : "+v"(q), "+v"(q8), "+v"(d));
+          asm volatile("" : "+v"(q), "+v"(d));
           fr f[kPmG], v[kPmG];
-          uint32_t f8[kPmG];
           static_for<0, kPmG>([&](auto r_) {
             constexpr int r = decltype(r_)::value;
             v[r] = fr_load(d + (uint64_t)(A * (kPmG * g + r)) * estride);
             f[r] = fr_load(q + (uint64_t)(A * (kPmG * g + r)) * estride);
-            f8[r] = q8[(uint64_t)(A * (kPmG * g + r)) * estride];
           });
           static_for<0, kPmG>([&](auto r_) {
             constexpr int r = decltype(r_)::value;
@@ -299,9 +289,7 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, const uint64_t 
             v9 xx;
             unpack_ext(v[r].l, xx.l);
             uint32_t w[9];
-#pragma unroll
-            for (int i = 0; i < 8; i++) w[i] = f[r].l[i];
-            w[8] = f8[r];
+            pbd_unpack(f[r].l, w);
             x[k] = v9_mul<false, false, true>(xx, w);
             pin9(x[k]);
           });
@@ -318,10 +306,10 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, const uint64_t 
         });
       }
     } else {
-      const uint32_t* src8 = a.in8 + poly * a.in_pstride;
       static_for<0, kPerT>([&](auto r2_) {
         constexpr int r2 = decltype(r2_)::value;
-        load_split(src, src8, pos0 + (uint64_t)(rho1 + A * r2) * estride, x[r2].l);
+        const fr raw = fr_load(src + pos0 + (uint64_t)(rho1 + A * r2) * estride);
+        pbd_unpack(raw.l, x[r2].l);
       });
     }
   } else {
@@ -345,7 +333,7 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, const uint64_t 
       const int e = rho1 * k2;  // < M'
       pin9(x[brev(k2, kLgPerT)]);
       uint32_t w[9];
-      load_split(a.inner29, a.inner29_8, (uint64_t)(e << (8 - m)), w);
+      pbd_unpack(fr_load(a.inner29 + (e << (8 - m))).l, w);
       x[brev(k2, kLgPerT)] = v9_mul<false, false, true>(x[brev(k2, kLgPerT)], w);
     });
   }
@@ -452,11 +440,8 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, const uint64_t 
     const uint64_t lp = line2 & ((1ull << lgL) - 1);
     const uint64_t j = lp & ((1ull << a.lgS) - 1);
     const uint64_t rr = lp >> a.lgS;
-    const uint64_t o0 = (poly << a.lgN) + (rr << a.lgMp) + j;
-    fr* dst = a.out + o0;
-    uint32_t* dst8 = a.out8 + o0;
+    fr* dst = a.out + (poly << a.lgN) + (rr << a.lgMp) + j;
     const fr* tw = a.tw + j;
-    const uint32_t* tw8 = a.tw8 + j;
     static_for<0, F2>([&](auto f_) {
       constexpr int f = decltype(f_)::value;
       const int k2 = g * F2 + f;
@@ -466,9 +451,11 @@ __device__ __forceinline__ void ntt_pass_body(const PassArgs& a, const uint64_t 
         v9 v = y[f * A + brev(k1, LA)];
         pin9(v);
         uint32_t w[9];
-        load_split(tw, tw8, k << a.lgS, w);
+        pbd_unpack(fr_load(tw + (k << a.lgS)).l, w);
         v = v9_mul<false, true, true>(v, w);
-        store_split(dst, dst8, k << a.lgS, v.l);
+        fr o;
+        pbd_pack(v.l, o.l);
+        fr_store(dst + (k << a.lgS), o);
       });
     });
   }

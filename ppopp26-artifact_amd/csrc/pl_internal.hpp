@@ -38,18 +38,14 @@ struct Domain {
   NttPlan plan;
   // The pass kernels' factors are held as w R' mod r (R' = 2^261, the radix-2^29
   // product's Montgomery constant; fr29.hpp), i.e. the stored form times 32.
-  fr* tw_fwd[PL_MAX_PASSES] = {};  // pass twiddles, 2^lgM[p] entries, p < P-1 (split words)
+  fr* tw_fwd[PL_MAX_PASSES] = {};  // pass twiddles, 2^lgM[p] entries, p < P-1
   fr* tw_inv[PL_MAX_PASSES] = {};  // inverse; pass 0 also carries 2^-lg when P >= 2
-  uint32_t* tw_fwd8[PL_MAX_PASSES] = {};  // their limb-8 planes
-  uint32_t* tw_inv8[PL_MAX_PASSES] = {};
   fr* pw0 = nullptr;               // w_N^i = pw0[i & 16383] * pw1[i >> 14]
   fr* pw1 = nullptr;
   fr* inner_fwd = nullptr;         // omega_256^e, e < 256 (stored form; small-size kernel)
   fr* inner_inv = nullptr;
-  fr* inner29_fwd = nullptr;       // the same times 32 (pass kernels; split words)
+  fr* inner29_fwd = nullptr;       // the same times 32 (pass kernels)
   fr* inner29_inv = nullptr;
-  uint32_t* inner29_fwd8 = nullptr;
-  uint32_t* inner29_inv8 = nullptr;
   fr* block = nullptr;             // single allocation holding all tables
   size_t bytes = 0;
   host::fr64 n_inv;
@@ -61,8 +57,7 @@ struct LdeDomain {
   int lg_n = 0, lg_blowup = 0;
   int blowup = 0;           // cosets: 2^lg_blowup, or 3 (shifts shift * w_3N^t, output interleave 3)
   Domain ntt;               // size-2^lg_n transform
-  fr* cp = nullptr;         // blowup << lg_n entries, split words (limb-8 plane after them)
-  const uint32_t* cp8() const { return reinterpret_cast<const uint32_t*>(cp + ((uint64_t)blowup << lg_n)); }
+  fr* cp = nullptr;         // 2^(lg_n + lg_blowup) entries
   size_t bytes = 0;
   host::fr64 shift;
 };
@@ -78,36 +73,27 @@ hipError_t coset_table_build_n(fr** cp, size_t* bytes, int lg_n, uint32_t B, con
 hipError_t interleave3(const fr* in, fr* out, uint64_t n, hipStream_t s);
 void lde_domain_free(LdeDomain& d);
 
-// bytes of `count` split words (fr29.hpp): 32-byte limbs, then the limb-8 plane
-inline size_t split_bytes(uint64_t count) { return count * 32 + ((count * 4 + 255) & ~(uint64_t)255); }
-inline uint32_t* plane_of(fr* t, uint64_t count) { return reinterpret_cast<uint32_t*>(t + count); }
-inline const uint32_t* plane_of(const fr* t, uint64_t count) { return reinterpret_cast<const uint32_t*>(t + count); }
-
 // Launch geometry for one call: a batch of `npoly` polynomials of size 2^lg.
 struct NttCall {
   const fr* in = nullptr;      // first pass input
   fr* out = nullptr;           // final output
-  fr* tmp = nullptr;           // npoly * 2^lg split words (intermediate passes; split_bytes)
+  fr* tmp = nullptr;           // npoly * 2^lg scratch (intermediate passes)
   uint64_t in_pstride = 0;     // element stride between input polys (0: shared)
   uint32_t npoly = 1;
-  const fr* premul = nullptr;  // per-input-element factor table [(poly & pm_mask) << lg | pos], split words
-  const uint32_t* premul8 = nullptr;
+  const fr* premul = nullptr;  // per-input-element factor table [(poly & pm_mask) << lg | pos]
   uint32_t pm_mask = 0;
   uint32_t lgI = 0;            // output interleave: out[(pos << lgI) + (poly & (2^lgI-1))]
   bool inverse = false;
 };
 
 hipError_t ntt_launch(const Domain& d, const NttCall& c, hipStream_t s);
-// pass p: in (with its limb-8 plane in8 when p > 0: split words), premul
-// (pass 0; split words), out (with out8 when p is not the last pass)
-hipError_t ntt_launch_one(const Domain& d, const NttCall& c, int p, const fr* in, const uint32_t* in8,
-                          uint64_t in_pstride, const fr* premul, const uint32_t* premul8, fr* out, uint32_t* out8,
-                          hipStream_t s);
-// pass p of a one-polynomial call over nlines of its lines, pointers (and
-// planes) advanced by in_off / out_off elements (column groups of pass 0,
-// line groups of the last pass; see ntt.hip)
-hipError_t ntt_launch_part(const Domain& d, const NttCall& c, int p, const fr* in, const uint32_t* in8, fr* out,
-                           uint32_t* out8, uint64_t in_off, uint64_t out_off, uint64_t nlines, hipStream_t s);
+hipError_t ntt_launch_one(const Domain& d, const NttCall& c, int p, const fr* in, uint64_t in_pstride,
+                          const fr* premul, fr* out, hipStream_t s);
+// pass p of a one-polynomial call over nlines of its lines, pointers advanced
+// by in_off / out_off elements (column groups of pass 0, line groups of the
+// last pass; see ntt.hip)
+hipError_t ntt_launch_part(const Domain& d, const NttCall& c, int p, const fr* in, fr* out, uint64_t in_off,
+                           uint64_t out_off, uint64_t nlines, hipStream_t s);
 uint64_t last_pass_row(const NttPlan& P, uint64_t lp);
 hipError_t domain_build(Domain& d, int lg, hipStream_t s);
 // maps a HIP error to a pl_status and records its text for pl_last_hip_error

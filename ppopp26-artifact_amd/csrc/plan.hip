@@ -107,7 +107,6 @@ hipError_t phase_plan(PhasePlan& ph, const std::vector<Item>& items, Bump& bump,
       c.in_pstride = 0;  // every coset reads the same coefficients
       c.npoly = 1u << it.lg_blowup;
       c.premul = cp;
-      c.premul8 = plane_of(cp, (1ull << it.lg_blowup) << it.lg);
       c.pm_mask = c.npoly - 1;
       c.lgI = it.lg_blowup;
     } else {
@@ -119,7 +118,7 @@ hipError_t phase_plan(PhasePlan& ph, const std::vector<Item>& items, Bump& bump,
     }
     const bool need_tmp = d->plan.P >= 2 || c.premul;
     if (need_tmp)
-      c.tmp = it.use_shared_tmp ? it.shared_tmp : (fr*)bump.get(split_bytes((uint64_t)c.npoly << it.lg));
+      c.tmp = it.use_shared_tmp ? it.shared_tmp : (fr*)bump.get(((size_t)c.npoly << it.lg) * sizeof(fr));
     ph.plain_calls.push_back({d, c});
     PassArgs a;
     if (!pass_args_regular(*d, c, 0, a)) {

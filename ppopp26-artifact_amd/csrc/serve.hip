@@ -228,7 +228,7 @@ static hipError_t group_plan(const pl_server& S, const std::vector<Job*>& jobs, 
       size_t mx = 0;
       for (auto& it : its)
         if (it.op == IT_LDE || it.op == IT_INTT || it.op == IT_NTT)
-          mx = std::max(mx, split_bytes(1ull << (it.lg + (it.op == IT_LDE ? it.lg_blowup : 0))));
+          mx = std::max(mx, (size_t)32 << (it.lg + (it.op == IT_LDE ? it.lg_blowup : 0)));
       if (mx) {
         fr* t = (fr*)bb.get(mx);
         for (auto& it : its)

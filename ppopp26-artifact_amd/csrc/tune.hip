@@ -170,7 +170,7 @@ int pl_autotune(int lg_lo, int lg_hi, const char* cache_path, int reps, int flag
   hipEvent_t e0 = nullptr, e1 = nullptr;
   hipError_t e = hipMalloc(&in, cap * sizeof(pl::fr));
   if (e == hipSuccess) e = hipMalloc(&out, cap * sizeof(pl::fr));
-  if (e == hipSuccess) e = hipMalloc(&tmp, pl::split_bytes(cap));
+  if (e == hipSuccess) e = hipMalloc(&tmp, cap * sizeof(pl::fr));
   if (e == hipSuccess) e = hipMalloc(&w, pl::uniform_work_bytes(cap));
   if (e == hipSuccess) e = hipEventCreate(&e0);
   if (e == hipSuccess) e = hipEventCreate(&e1);

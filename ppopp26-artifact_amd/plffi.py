@@ -46,7 +46,6 @@ SIGNATURES = {
     "pl_ntt_work_bytes": (ctypes.c_size_t, [_i, _u32]),
     "pl_ntt_num_passes": (_i, [_i]),
     "pl_domain_num_passes": (_i, [_vp]),
-    "pl_ntt_pass_buffer_bytes": (ctypes.c_size_t, [_i, _u32]),
     "pl_ntt_pass": (_i, [_vp, _vp, _i, _u32, _i, _i, _vp, _vp]),
     "pl_ntt_stage_no_rotate": (_i, [_vp, _u64, _i, _i, _i, _u64, _vp, _vp]),
     "pl_ct_stage_indices": (None, [_u64, _i, _i, ctypes.POINTER(_u64), ctypes.POINTER(_u64)]),
@@ -625,17 +624,6 @@ def ntt_host(h, dom, lg_k=4, npoly=1, inverse=False, out=None, stream=None):
     check("pl_ntt_host", lib().pl_ntt_host(ctypes.c_void_p(h.data_ptr()), ctypes.c_void_p(out.data_ptr()), dom.lg,
                                            npoly, 1 if inverse else 0, dom.h, lg_k, None, _stream(stream)))
     return out
-
-
-def byte_buffer(nbytes, device="cuda"):
-    """A device tensor of at least nbytes bytes (rows of 32 bytes)."""
-    import torch
-    return torch.empty(((nbytes + 31) // 32, 4), dtype=torch.int64, device=device)
-
-
-def pass_buffer(lg, npoly=1):
-    """A buffer for a pass's intermediate output (split words)."""
-    return byte_buffer(lib().pl_ntt_pass_buffer_bytes(lg, npoly))
 
 
 def ntt_num_passes(lg):

@@ -49,14 +49,10 @@ def test_library_loads_and_host_functions():
         [1, 1, 1, 1, 2, 2, 3, 3, 4, 4, 5, 5]
     assert L.pl_ntt_num_passes(29) == -3 and L.pl_ntt_num_passes(0) == -3
     assert L.pl_ntt_work_bytes(5, 1) == 0          # every split of 2^5 is one pass, in place
-    # intermediate split words: 32 bytes of limbs 0..7 per element, then a
-    # plane of limb 8 (4 bytes per element, rounded up to 256 bytes)
-    split = lambda n: 32 * n + ((4 * n + 255) & ~255)  # noqa: E731
-    assert L.pl_ntt_work_bytes(6, 1) == split(1 << 6)    # 2^6 may be split 3+3 (size fixed by lg alone)
-    assert L.pl_ntt_work_bytes(24, 1) == split(1 << 24)  # one intermediate buffer
-    assert L.pl_ntt_work_bytes(12, 3) == split(3 << 12)
-    assert L.pl_ntt_pass_buffer_bytes(12, 3) == split(3 << 12)
-    assert L.pl_lde_work_bytes(24, 2) == split(1 << 26)
+    assert L.pl_ntt_work_bytes(6, 1) == 32 << 6    # 2^6 may be split 3+3 (size fixed by lg alone)
+    assert L.pl_ntt_work_bytes(24, 1) == 32 << 24  # one intermediate buffer
+    assert L.pl_ntt_work_bytes(12, 3) == 3 * (32 << 12)
+    assert L.pl_lde_work_bytes(24, 2) == 32 << 26
     assert L.pl_scan_work_bytes(1 << 20) >= 32 * 2 * 256
 
 
