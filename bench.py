@@ -65,7 +65,7 @@ def parse():
     ap.add_argument("--no-batch", action="store_true")
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-miss", action="store_true", help="skip grand product / evaluation / division")
-    ap.add_argument("--serve-jobs", type=int, default=40, help="jobs per served run")
+    ap.add_argument("--serve-jobs", type=int, default=200, help="jobs in the served mix (every run uses the same mix)")
     ap.add_argument("--serve-slots", type=int, default=3, help="job groups running at once (batched form)")
     ap.add_argument("--serve-large", type=int, default=25, help="log2 of the work that makes a job large")
     ap.add_argument("--plans", default=os.path.join(ROOT, "ppopp26-artifact_amd", "tune", "mi355x_plans.txt"),
@@ -401,39 +401,46 @@ def summarize(jobs, res):
 
 
 def bench_served(pl, synth, n_jobs, slots, large_lg):
+    """One seeded job mix (n_jobs proof jobs, unit-rate Poisson arrival
+    times) run through the request loop in both forms: first with every job
+    queued at once (each form's capacity on this mix, outputs compared by
+    digest), then with the mix's arrival times scaled to 0.5, 0.8 and 1.1
+    times the plain form's capacity on the same mix."""
     import torch
     # resident inputs: 8 columns per size (jobs of one size share columns,
     # each job has its own beta, gamma, zeta)
     pool = {lg: [dev_uniform(pl, 1 << lg, synth.seed(synth.OP_SERVE, lg, c)) for c in range(8)]
             for lg in range(14, 25)}
     rates_rel = [0.5, 0.8, 1.1]
-    wl = [synth.served_jobs(ri, n_jobs) for ri in range(3)]
-    scal = [[tuple(dev_uniform(pl, 1, synth.seed(synth.OP_SERVE, 0, 10000 * (ri + 1) + 3 * i + k))
-                   for k in range(3)) for i in range(n_jobs)] for ri in range(3)]
+    wl = synth.served_jobs(0, n_jobs)
+    jobs = [(lg, lb) for lg, lb, _ in wl]
+    scal = [tuple(dev_uniform(pl, 1, synth.seed(synth.OP_SERVE, 0, 10000 + 3 * i + k)) for k in range(3))
+            for i in range(n_jobs)]
     # warm-up: one job of every size and blowup (domains and coset tables are
     # built on first use and stay resident)
-    warm = sorted(set((lg, lb) for w in wl for lg, lb, _ in w))
-    run_server(pl, pl.SERVE_BATCHED, warm, None, pool, [scal[0][0]] * len(warm), False)
+    warm = sorted(set(jobs))
+    run_server(pl, pl.SERVE_BATCHED, warm, None, pool, [scal[0]] * len(warm), False, slots, large_lg)
     torch.cuda.synchronize()
     progress("served: warm-up done (%d sizes)" % len(warm))
-    jobs0 = [(lg, lb) for lg, lb, _ in wl[0]]
-    # capacity: every job queued at once; outputs verified between the forms
-    sat_p, _ = run_server(pl, pl.SERVE_PLAIN, jobs0, None, pool, scal[0], True)
-    sat_b, st_b = run_server(pl, pl.SERVE_BATCHED, jobs0, None, pool, scal[0], True, slots, large_lg)
+    # capacity of each form on this mix: every job queued at once; outputs
+    # verified between the forms
+    sat_p, _ = run_server(pl, pl.SERVE_PLAIN, jobs, None, pool, scal, True)
+    sat_b, st_b = run_server(pl, pl.SERVE_BATCHED, jobs, None, pool, scal, True, slots, large_lg)
     exact = all(a[3] == b[3] for a, b in zip(sat_p, sat_b))
-    progress("served: saturation runs done, bit-exact %s" % exact)
-    cap_p = summarize(jobs0, sat_p)
-    cap_b = summarize(jobs0, sat_b)
-    plain_jobs_per_s = len(jobs0) / cap_p["span_s"]
+    cap_p = summarize(jobs, sat_p)
+    cap_b = summarize(jobs, sat_b)
+    progress("served: capacity %.3g vs %.3g elems/s (batched / plain), bit-exact %s" % (
+        cap_b["elems_per_s"], cap_p["elems_per_s"], exact))
+    plain_jobs_per_s = len(jobs) / cap_p["span_s"]
     rates = []
-    for ri, rel in enumerate(rates_rel):
-        jobs = [(lg, lb) for lg, lb, _ in wl[ri]]
+    for rel in rates_rel:
         rate = rel * plain_jobs_per_s
-        times = [t / rate for _, _, t in wl[ri]]
-        rp, _ = run_server(pl, pl.SERVE_PLAIN, jobs, times, pool, scal[ri], False)
-        rb, _ = run_server(pl, pl.SERVE_BATCHED, jobs, times, pool, scal[ri], False, slots, large_lg)
+        times = [t / rate for _, _, t in wl]
+        rp, _ = run_server(pl, pl.SERVE_PLAIN, jobs, times, pool, scal, False)
+        rb, _ = run_server(pl, pl.SERVE_BATCHED, jobs, times, pool, scal, False, slots, large_lg)
         sp, sb = summarize(jobs, rp), summarize(jobs, rb)
-        progress("served: rate %.1f jobs/s: p99 batched %.1f ms, plain %.1f ms" % (rate, sb["p99_ms"], sp["p99_ms"]))
+        progress("served: %.1f x plain capacity (%.1f jobs/s): p99 batched %.1f ms, plain %.1f ms" % (
+            rel, rate, sb["p99_ms"], sp["p99_ms"]))
         rates.append({"rate_jobs_per_s": round(rate, 2), "load_vs_plain_capacity": rel,
                       "batched_p50_ms": sb["p50_ms"], "batched_p99_ms": sb["p99_ms"],
                       "plain_p50_ms": sp["p50_ms"], "plain_p99_ms": sp["p99_ms"],
@@ -442,15 +449,17 @@ def bench_served(pl, synth, n_jobs, slots, large_lg):
     torch.cuda.synchronize()
     return {
         "workload": "proof jobs: 8 cols x 2^lg (lg 14..24), iNTT -> LDE (lambda 2|4) -> perm GP -> eval -> div; "
-                    "%d jobs per run" % n_jobs,
+                    "one seeded mix of %d jobs for every run" % n_jobs,
         "value": cap_b["elems_per_s"], "unit": "Fr elems/s (job inputs, all jobs queued at once)",
         "plain_value": cap_p["elems_per_s"],
         "speedup_vs_plain": round(cap_b["elems_per_s"] / cap_p["elems_per_s"], 3),
         "saturated_p50_ms": cap_b["p50_ms"], "saturated_p99_ms": cap_b["p99_ms"],
         "plain_saturated_p50_ms": cap_p["p50_ms"], "plain_saturated_p99_ms": cap_p["p99_ms"],
+        "plain_capacity_jobs_per_s": round(plain_jobs_per_s, 2),
         "poisson": rates,
         "bit_exact_vs_plain": exact,
-        "slots": slots, "large_work_lg": large_lg, "groups": st_b["groups"], "arena_peak_gib": round(st_b["arena_peak"] / 2**30, 2),
+        "slots": slots, "large_work_lg": large_lg, "groups": st_b["groups"],
+        "arena_peak_gib": round(st_b["arena_peak"] / 2**30, 2),
         "arena_gib": round(st_b["arena_bytes"] / 2**30, 1), "domain_cache_gib": round(st_b["domain_bytes"] / 2**30, 2),
     }
 
