@@ -295,6 +295,21 @@ size_t pl_vec_work_bytes(uint64_t n);
 int pl_vec_op(int op, const void* d_a, const void* d_b, void* d_out, uint64_t n, void* d_work,
               pl_stream_t stream);
 
+/* [exp_mod: PLONK/utils/function.cuh:32, function.cu:113-120,
+ *  mont/cuda/mont_arithmetic.cu:258-273, 450-463]
+ * out[i] = in[i]^exp; exp = 0 writes one, exp = 1 copies.  The reference
+ * traps on a negative exponent; here that is PL_ERR_ARG.  out may alias in. */
+int pl_exp_mod(const void* d_in, void* d_out, uint64_t n, int exp, pl_stream_t stream);
+/* [gen_sequence / gen_sequence_: function.cuh:46-47, function.cu:154-160,
+ *  poly_eval_kernel mont_arithmetic.cu:104-114]
+ * out[i] = x^i for i < n (out[0] = one, out[1] = x); n <= 2^28; out must not
+ * overlap x. */
+int pl_gen_sequence(uint64_t n, const void* d_x, void* d_out, pl_stream_t stream);
+/* [repeat_to_poly / repeat_to_poly_: function.cuh:49-50, function.cu:162-168,
+ *  repeat_kernel mont_arithmetic.cu:214-220]
+ * out[i] = *x for i < n; out must not overlap x. */
+int pl_repeat_to_poly(const void* d_x, void* d_out, uint64_t n, pl_stream_t stream);
+
 /* ---- PLONK permutation ratio --------------------------------------------
  * (the quotient term shape of zksnark_compute_permutation_constraint.cu:13-14,
  *  built from element-wise calls in the reference)

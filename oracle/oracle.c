@@ -158,6 +158,27 @@ void or_vec_inv(uint64_t* o, const uint64_t* a, uint64_t n) {
   for (uint64_t i = 0; i < n; i++) or_inv(o + 4 * i, a + 4 * i);
 }
 
+/* exp_mod: o[i] = a[i]^e; e = 0 gives one, e = 1 a copy
+ * (PLONK/utils/mont/cpu/mont_arithmetic.cpp:66-81) */
+void or_vec_exp(uint64_t* o, const uint64_t* a, uint64_t n, uint32_t e) {
+  const uint64_t ee[4] = {e, 0, 0, 0};
+  for (uint64_t i = 0; i < n; i++) {
+    if (e == 1)
+      memcpy(o + 4 * i, a + 4 * i, 32);
+    else
+      or_pow(o + 4 * i, a + 4 * i, ee);
+  }
+}
+
+/* gen_sequence: o[i] = x^i as a running product, o[0] = one, o[1] = x
+ * (poly_eval_kernel, PLONK/utils/mont/cuda/mont_arithmetic.cu:104-114) */
+void or_sequence(uint64_t* o, const uint64_t* x, uint64_t n) {
+  if (n == 0) return;
+  memcpy(o, ONE, 32);
+  if (n > 1) memcpy(o + 4, x, 32);
+  for (uint64_t i = 2; i < n; i++) or_mul(o + 4 * i, o + 4 * (i - 1), x);
+}
+
 void or_group_gen(uint64_t* o) { or_from_u64(o, 7); }
 
 /* omega_{2^32} = 7^((r-1)/2^32); omega_{2^lg} = omega_{2^32}^(2^(32-lg)).

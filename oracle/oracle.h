@@ -54,6 +54,10 @@ void or_vec_add(uint64_t* o, const uint64_t* a, const uint64_t* b, uint64_t n);
 void or_vec_sub(uint64_t* o, const uint64_t* a, const uint64_t* b, uint64_t n);
 void or_vec_mul(uint64_t* o, const uint64_t* a, const uint64_t* b, uint64_t n);
 void or_vec_inv(uint64_t* o, const uint64_t* a, uint64_t n);
+/* exp_mod, e = 0 -> one, e = 1 -> copy (cpu/mont_arithmetic.cpp:66-81) */
+void or_vec_exp(uint64_t* o, const uint64_t* a, uint64_t n, uint32_t e);
+/* gen_sequence o[i] = x^i by a running product (cuda/mont_arithmetic.cu:104-114) */
+void or_sequence(uint64_t* o, const uint64_t* x, uint64_t n);
 
 /* domain constants (parameters/bls12_381.h) */
 void or_root_of_unity(uint64_t* o, int lg);       /* omega_{2^lg}, 0 <= lg <= 32 */

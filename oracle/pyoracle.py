@@ -41,6 +41,8 @@ def lib():
         for name in ["or_vec_add", "or_vec_sub", "or_vec_mul"]:
             getattr(L, name).argtypes = [_u64p, _u64p, _u64p, ctypes.c_uint64]
         L.or_vec_inv.argtypes = [_u64p, _u64p, ctypes.c_uint64]
+        L.or_vec_exp.argtypes = [_u64p, _u64p, ctypes.c_uint64, ctypes.c_uint32]
+        L.or_sequence.argtypes = [_u64p, _u64p, ctypes.c_uint64]
         L.or_root_of_unity.argtypes = [_u64p, ctypes.c_int]
         L.or_domain_size_inverse.argtypes = [_u64p, ctypes.c_int]
         L.or_group_gen.argtypes = [_u64p]
@@ -144,6 +146,18 @@ def sub(a, b):
 def inv(a):
     o = fr_array(len(a))
     lib().or_vec_inv(P(o), P(np.ascontiguousarray(a)), len(a))
+    return o
+
+
+def vec_exp(a, e):
+    o = fr_array(len(a))
+    lib().or_vec_exp(P(o), P(np.ascontiguousarray(a)), len(a), e)
+    return o
+
+
+def sequence(x, n):
+    o = fr_array(n)
+    lib().or_sequence(P(o), P(np.ascontiguousarray(x)), n)
     return o
 
 

@@ -1,12 +1,14 @@
 // Exercises the C++ operator API (SyncedMemory + function.hpp) on seeded
 // inputs and dumps raw results, so tests can check them against the oracle:
 //   api_check <out_dir> <lg>
+//   api_check --host-helpers <out_dir> <lg>   (no GPU call)
 // Files (raw 32-byte elements): x.bin y.bin (inputs), ntt.bin intt.bin,
 // gp.bin gp_incl.bin, eval.bin, div.bin, add.bin mul.bin div_ew.bin,
 // add_cpu.bin (CPU-dispatched add), lde.bin (our_lde semantics, lambda=2),
 // stage0.bin stage6.bin (Ntt::stage_no_rotate on seeded chunks), coset.bin
 // icoset.bin coset_steps.bin icoset_steps.bin coset_init0.bin coset_lde3.bin
-// (Ntt_coset / Intt_coset).
+// (Ntt_coset / Intt_coset), exp5.bin exp5_cpu.bin (exp_mod(x, 5) on the GPU
+// and the CPU), seq.bin (gen_sequence(n + 5, pt)), rep.bin (repeat_to_poly(pt, 33)).
 #include <string>
 
 #include "common.hpp"
@@ -24,7 +26,41 @@ static SyncedMemory seeded(uint64_t n, uint64_t seed) {
   return m;
 }
 
+// calculate_map1 / bit_rev_step1_parallel (function.cuh:157-244) on the host
+// only: in.bin (2^lg seeded elements), map.bin (the 2^lg map, u32), brev_t<T>.bin for T = 1, 3, 8 threads,
+// and brev_idx.bin (the 2^lg inputs through the upper half of the 2^(lg+1)
+// map, idx = 2^lg, into 2^(lg+1) zeroed outputs)
+struct el32 {
+  uint64_t w[4];
+};
+static int host_helpers(const std::string& dir, int lg) {
+  const uint64_t n = 1ull << lg;
+  std::vector<el32> in(n), out(n), out2(2 * n);
+  cli::fill_seeded(in.data(), n, 1001);
+  std::vector<uint32_t> map(n), map2(2 * n);
+  calculate_map1(map.data(), lg);
+  calculate_map1(map2.data(), lg + 1);
+  auto write = [&](const std::string& name, const void* p, size_t bytes) {
+    FILE* f = std::fopen((dir + "/" + name).c_str(), "wb");
+    if (!f) throw std::runtime_error("cannot write " + name);
+    std::fwrite(p, 1, bytes, f);
+    std::fclose(f);
+  };
+  write("in.bin", in.data(), n * 32);
+  write("map.bin", map.data(), n * 4);
+  for (int t : {1, 3, 8}) {
+    std::fill(out.begin(), out.end(), el32{});
+    bit_rev_step1_parallel(out.data(), in.data(), map.data(), lg, t, 0);
+    write("brev_t" + std::to_string(t) + ".bin", out.data(), n * 32);
+  }
+  bit_rev_step1_parallel(out2.data(), in.data(), map2.data(), lg, 4, n);
+  write("brev_idx.bin", out2.data(), 2 * n * 32);
+  std::cout << "host helpers ok" << std::endl;
+  return 0;
+}
+
 int main(int argc, char** argv) {
+  if (argc == 4 && std::string(argv[1]) == "--host-helpers") return host_helpers(argv[2], std::atoi(argv[3]));
   if (argc != 3) {
     std::cerr << "Usage: " << argv[0] << " <out_dir> <lg>\n";
     return 1;
@@ -64,6 +100,10 @@ int main(int argc, char** argv) {
     SyncedMemory xc = seeded(n, 1001), yc = seeded(n, 1002);  // heads at CPU -> CPU path
     dump(dir + "/add_cpu.bin", add_mod(xc, yc));
     dump(dir + "/mul_cpu.bin", mul_mod(xc, yc));
+    dump(dir + "/exp5.bin", exp_mod(x, 5));
+    dump(dir + "/exp5_cpu.bin", exp_mod(seeded(n, 1001), 5));
+    dump(dir + "/seq.bin", gen_sequence(n + 5, pt));
+    dump(dir + "/rep.bin", repeat_to_poly(pt, 33));
     // our_lde with lambda = 2 through the C-ABI
     pl_lde_domain* ld = nullptr;
     cli::check(pl_lde_domain_create(lg, 2, nullptr, &ld), "pl_lde_domain_create");

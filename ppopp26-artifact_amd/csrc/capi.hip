@@ -20,6 +20,12 @@ int hip_fail(hipError_t e) {
 
 bool aligned16(const void* p) { return ((uintptr_t)p & 15) == 0; }
 
+// [a, a + na) and [b, b + nb) share a byte
+bool ranges_overlap(const void* a, uint64_t na, const void* b, uint64_t nb) {
+  const uintptr_t x = (uintptr_t)a, y = (uintptr_t)b;
+  return x < y + nb && y < x + na;
+}
+
 }  // namespace
 
 namespace pl {
@@ -337,6 +343,26 @@ int pl_vec_op(int op, const void* d_a, const void* d_b, void* d_out, uint64_t n,
     if (e != hipSuccess) return hip_fail(e);
   }
   hipError_t e = pl::vec_op(op, (const pl::fr*)d_a, (const pl::fr*)d_b, (pl::fr*)d_out, n, sc.p, (hipStream_t)stream);
+  return e == hipSuccess ? PL_OK : hip_fail(e);
+}
+
+int pl_exp_mod(const void* d_in, void* d_out, uint64_t n, int exp, pl_stream_t stream) {
+  if (exp < 0 || (n && (!d_in || !d_out)) || !aligned16(d_in) || !aligned16(d_out)) return PL_ERR_ARG;
+  hipError_t e = pl::exp_mod((const pl::fr*)d_in, (pl::fr*)d_out, n, (uint32_t)exp, (hipStream_t)stream);
+  return e == hipSuccess ? PL_OK : hip_fail(e);
+}
+
+int pl_gen_sequence(uint64_t n, const void* d_x, void* d_out, pl_stream_t stream) {
+  if (n > (1ull << 28) || (n && (!d_x || !d_out)) || !aligned16(d_x) || !aligned16(d_out)) return PL_ERR_ARG;
+  if (n && ranges_overlap(d_x, 32, d_out, n * 32)) return PL_ERR_ARG;
+  hipError_t e = pl::gen_sequence((const pl::fr*)d_x, (pl::fr*)d_out, n, (hipStream_t)stream);
+  return e == hipSuccess ? PL_OK : hip_fail(e);
+}
+
+int pl_repeat_to_poly(const void* d_x, void* d_out, uint64_t n, pl_stream_t stream) {
+  if ((n && (!d_x || !d_out)) || !aligned16(d_x) || !aligned16(d_out)) return PL_ERR_ARG;
+  if (n && ranges_overlap(d_x, 32, d_out, n * 32)) return PL_ERR_ARG;
+  hipError_t e = pl::repeat_to_poly((const pl::fr*)d_x, (pl::fr*)d_out, n, (hipStream_t)stream);
   return e == hipSuccess ? PL_OK : hip_fail(e);
 }
 

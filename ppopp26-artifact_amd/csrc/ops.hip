@@ -879,6 +879,76 @@ hipError_t vec_op(int op, const fr* a, const fr* b, fr* out, uint64_t n, fr* scr
   return hipGetLastError();
 }
 
+// ---- powers and broadcasts ------------------------------------------------------
+// exp_mod: out[i] = a[i]^e, left-to-right square and multiply with the
+// exponent uniform over the launch; e = 0 gives one and e = 1 a copy
+// (mont_arithmetic.cu:258-273, cpu/mont_arithmetic.cpp:66-81).
+__global__ void __launch_bounds__(256) exp_kernel(const fr* a, fr* out, uint64_t n, uint32_t e) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  fr_store(out + i, e == 0 ? fr_one() : fr_pow_u64(fr_load(a + i), e));
+}
+
+// gen_sequence: out[i] = x^i (poly_eval_kernel, mont_arithmetic.cu:104-114,
+// raises x to each i alone).  Here the head out[0..min(n, 2^14)) is raised
+// directly (at most 14 squarings), and every later tile of 2^14 elements is
+// the head times one factor: out[(q << 14) + j] = out[j] * x^(q << 14), with
+// x^(q << 14) = out[q]^(2^14) (q < 2^14 for n <= 2^28), one product per element.
+constexpr uint32_t kSeqLgTile = 14;
+__global__ void __launch_bounds__(256) seq_head_kernel(const fr* x, fr* out, uint64_t n0) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n0) return;
+  fr_store(out + i, fr_pow_u64(fr_load(x), i));
+}
+// block b: tile q = b + 1; reads out[q] and out[0..2^14), writes tile q only
+__global__ void __launch_bounds__(256) seq_tail_kernel(fr* out, uint64_t n) {
+  __shared__ uint4 hq[2];
+  const uint64_t q = (uint64_t)blockIdx.x + 1;
+  if (threadIdx.x == 0) {
+    fr h = fr_load(out + q);
+    for (uint32_t k = 0; k < kSeqLgTile; k++) h = fr_sqr(h);
+    hq[0] = make_uint4(h.l[0], h.l[1], h.l[2], h.l[3]);
+    hq[1] = make_uint4(h.l[4], h.l[5], h.l[6], h.l[7]);
+  }
+  __syncthreads();
+  fr h;
+  h.l[0] = hq[0].x; h.l[1] = hq[0].y; h.l[2] = hq[0].z; h.l[3] = hq[0].w;
+  h.l[4] = hq[1].x; h.l[5] = hq[1].y; h.l[6] = hq[1].z; h.l[7] = hq[1].w;
+  const uint64_t base = q << kSeqLgTile;
+  for (uint32_t j = threadIdx.x; j < (1u << kSeqLgTile) && base + j < n; j += blockDim.x)
+    fr_store(out + base + j, fr_mul(fr_load(out + j), h));
+}
+
+// repeat_to_poly: out[i] = *x (repeat_kernel, mont_arithmetic.cu:214-220)
+__global__ void __launch_bounds__(256) repeat_kernel(const fr* x, fr* out, uint64_t n) {
+  const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
+  if (i >= n) return;
+  fr_store(out + i, fr_load(x));
+}
+
+hipError_t exp_mod(const fr* a, fr* out, uint64_t n, uint32_t e, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(exp_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, out, n, e);
+  return hipGetLastError();
+}
+
+hipError_t gen_sequence(const fr* x, fr* out, uint64_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  const uint64_t n0 = std::min<uint64_t>(n, 1ull << kSeqLgTile);
+  hipLaunchKernelGGL(seq_head_kernel, dim3((unsigned)((n0 + 255) / 256)), dim3(256), 0, s, x, out, n0);
+  if (n > n0) {
+    const uint64_t tiles = (n + (1ull << kSeqLgTile) - 1) >> kSeqLgTile;
+    hipLaunchKernelGGL(seq_tail_kernel, dim3((unsigned)(tiles - 1)), dim3(256), 0, s, out, n);
+  }
+  return hipGetLastError();
+}
+
+hipError_t repeat_to_poly(const fr* x, fr* out, uint64_t n, hipStream_t s) {
+  if (n == 0) return hipSuccess;
+  hipLaunchKernelGGL(repeat_kernel, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, x, out, n);
+  return hipGetLastError();
+}
+
 static RatioArgs ratio_args(const fr* const w[3], const fr* const sg[3], const fr* beta, const fr* gamma,
                             const fr* w0, const fr* w1, const host::fr64 k[3], uint64_t n) {
   RatioArgs a;

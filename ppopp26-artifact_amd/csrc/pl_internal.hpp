@@ -116,6 +116,9 @@ hipError_t eval_div(const fr* in, fr* q, uint64_t n, const fr* z, fr* y, fr* scr
 // element-wise ops, batch inversion, permutation ratio (ops.hip)
 uint64_t ops_scratch_elems(uint64_t n);
 hipError_t vec_op(int op, const fr* a, const fr* b, fr* out, uint64_t n, fr* scratch, hipStream_t s);
+hipError_t exp_mod(const fr* a, fr* out, uint64_t n, uint32_t e, hipStream_t s);
+hipError_t gen_sequence(const fr* x, fr* out, uint64_t n, hipStream_t s);  // n <= 2^28
+hipError_t repeat_to_poly(const fr* x, fr* out, uint64_t n, hipStream_t s);
 hipError_t perm_ratio(const fr* const w[3], const fr* const sg[3], const fr* beta, const fr* gamma,
                       const fr* w0, const fr* w1, const host::fr64 k[3], fr* ratio, uint64_t n, fr* scratch,
                       hipStream_t s);

@@ -79,6 +79,59 @@ SyncedMemory to_base(SyncedMemory input, hipStream_t s) { return unary(PL_VEC_FR
 SyncedMemory neg_mod(SyncedMemory input) { return unary(PL_VEC_NEG, input, nullptr); }
 SyncedMemory inv_mod(SyncedMemory input) { return unary(PL_VEC_INV, input, nullptr); }
 
+// exp = 0 -> one, exp = 1 -> copy; CPU when the head is at the CPU
+// (function.cu:113-120, cpu/mont_arithmetic.cpp:66-81)
+SyncedMemory exp_mod(SyncedMemory input, int exp) {
+  if (exp < 0) throw std::invalid_argument("exp_mod: negative exponent");
+  SyncedMemory out(input.size());
+  if (input.head() == SyncedMemory::HEAD_AT_CPU) {
+    const uint64_t n = elems(input);
+    const void* pi = input.cpu_data();
+    void* po = out.mutable_cpu_data();
+    for (uint64_t i = 0; i < n; i++) {
+      pl::host::fr64 x = ld(pi, i), r = pl::host::one();
+      if (exp == 1) {
+        r = x;
+      } else {
+        for (int b = 31; b >= 0; b--) {
+          r = pl::host::mul(r, r);
+          if ((exp >> b) & 1) r = pl::host::mul(r, x);
+        }
+      }
+      st(po, i, r);
+    }
+    return out;
+  }
+  void* pi = input.mutable_gpu_data();
+  void* po = out.mutable_gpu_data();
+  check(pl_exp_mod(pi, po, elems(input), exp, nullptr), "pl_exp_mod");
+  return out;
+}
+
+// out[i] = x^i (function.cu:154-160)
+void gen_sequence_(uint64_t N, SyncedMemory x, SyncedMemory y, hipStream_t s) {
+  void* px = x.mutable_gpu_data_async(s);
+  void* py = y.mutable_gpu_data_async(s);
+  check(pl_gen_sequence(N, px, py, (pl_stream_t)s), "pl_gen_sequence");
+}
+SyncedMemory gen_sequence(uint64_t N, SyncedMemory x, hipStream_t s) {
+  SyncedMemory y(N * kFr);
+  gen_sequence_(N, x, y, s);
+  return y;
+}
+
+// out[i] = x (function.cu:162-168)
+void repeat_to_poly_(SyncedMemory x, SyncedMemory y, uint64_t N, hipStream_t s) {
+  void* px = x.mutable_gpu_data_async(s);
+  void* py = y.mutable_gpu_data_async(s);
+  check(pl_repeat_to_poly(px, py, N, (pl_stream_t)s), "pl_repeat_to_poly");
+}
+SyncedMemory repeat_to_poly(SyncedMemory x, uint64_t N, hipStream_t s) {
+  SyncedMemory y(N * kFr);
+  repeat_to_poly_(x, y, N, s);
+  return y;
+}
+
 #define PL_BINARY_OPS(NAME, OP, SCALAR)                                                       \
   SyncedMemory NAME(SyncedMemory a, SyncedMemory b, hipStream_t s) {                          \
     SyncedMemory out(a.size());                                                               \

@@ -31,6 +31,7 @@ SyncedMemory to_mont(SyncedMemory input);
 SyncedMemory to_base(SyncedMemory input, hipStream_t stream = (hipStream_t)0);
 SyncedMemory neg_mod(SyncedMemory input);
 SyncedMemory inv_mod(SyncedMemory input);
+SyncedMemory exp_mod(SyncedMemory input, int exp);
 
 SyncedMemory add_mod(SyncedMemory input1, SyncedMemory input2, hipStream_t stream = (hipStream_t)0);
 void add_mod_(SyncedMemory input1, SyncedMemory input2, hipStream_t stream = (hipStream_t)0);
@@ -54,6 +55,11 @@ SyncedMemory evaluate(SyncedMemory poly, SyncedMemory x);
 SyncedMemory poly_div_poly(SyncedMemory divid, SyncedMemory c);
 SyncedMemory accumulate_mul_poly(SyncedMemory product, hipStream_t stream = (hipStream_t)0);
 void accumulate_mul_poly_(SyncedMemory product, SyncedMemory output, hipStream_t stream = (hipStream_t)0);
+
+SyncedMemory gen_sequence(uint64_t N, SyncedMemory x, hipStream_t stream = (hipStream_t)0);
+void gen_sequence_(uint64_t N, SyncedMemory x, SyncedMemory y, hipStream_t stream = (hipStream_t)0);
+SyncedMemory repeat_to_poly(SyncedMemory x, uint64_t N, hipStream_t stream = (hipStream_t)0);
+void repeat_to_poly_(SyncedMemory x, SyncedMemory y, uint64_t N, hipStream_t stream = (hipStream_t)0);
 
 SyncedMemory pad_poly(SyncedMemory x, uint64_t N, hipStream_t stream = (hipStream_t)0);
 SyncedMemory repeat_zero(uint64_t N);

@@ -82,6 +82,9 @@ SIGNATURES = {
     "pl_ntt_naive": (_i, [_vp, _vp, _i, _vp, _vp]),
     "pl_lde_naive": (_i, [_vp, _vp, _i, _i, _vp, _vp]),
     "pl_vec_op": (_i, [_i, _vp, _vp, _vp, _u64, _vp, _vp]),
+    "pl_exp_mod": (_i, [_vp, _vp, _u64, _i, _vp]),
+    "pl_gen_sequence": (_i, [_u64, _vp, _vp, _vp]),
+    "pl_repeat_to_poly": (_i, [_vp, _vp, _u64, _vp]),
     "pl_perm_ratio": (_i, [ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp, _vp, _vp, _i, _vp, _vp, _vp, _vp]),
     "pl_perm_grand_product": (_i, [ctypes.POINTER(_vp), ctypes.POINTER(_vp), _vp, _vp, _vp, _i, _vp, _vp, _vp,
                                    _vp]),
@@ -357,6 +360,33 @@ def vec_op(op, a, b=None, out=None, work=None, stream=None):
         out = torch.empty_like(a)
     check("pl_vec_op", lib().pl_vec_op(VEC_OPS[op], _ptr(a), _ptr(b), _ptr(out), a.shape[0], _ptr(work),
                                        _stream(stream)))
+    return out
+
+
+def exp_mod(a, e, out=None, stream=None):
+    """out[i] = a[i]^e (e = 0: one, e = 1: copy)"""
+    import torch
+    if out is None:
+        out = torch.empty_like(a)
+    check("pl_exp_mod", lib().pl_exp_mod(_ptr(a), _ptr(out), a.shape[0], e, _stream(stream)))
+    return out
+
+
+def gen_sequence(n, x, out=None, stream=None):
+    """out[i] = x^i, i < n"""
+    import torch
+    if out is None:
+        out = torch.empty((n, 4), dtype=torch.int64, device=x.device)
+    check("pl_gen_sequence", lib().pl_gen_sequence(n, _ptr(x), _ptr(out), _stream(stream)))
+    return out
+
+
+def repeat_to_poly(x, n, out=None, stream=None):
+    """out[i] = x, i < n"""
+    import torch
+    if out is None:
+        out = torch.empty((n, 4), dtype=torch.int64, device=x.device)
+    check("pl_repeat_to_poly", lib().pl_repeat_to_poly(_ptr(x), _ptr(out), n, _stream(stream)))
     return out
 
 

@@ -44,6 +44,10 @@ def test_cpp_api(tmp_path):
     np.testing.assert_array_equal(f("div_ew.bin"), po.mul(x, po.inv(y)))
     np.testing.assert_array_equal(f("add_cpu.bin"), po.add(x, y))
     np.testing.assert_array_equal(f("mul_cpu.bin"), po.mul(x, y))
+    np.testing.assert_array_equal(f("exp5.bin"), po.vec_exp(x, 5))
+    np.testing.assert_array_equal(f("exp5_cpu.bin"), po.vec_exp(x, 5))
+    np.testing.assert_array_equal(f("seq.bin"), po.sequence(pt, len(x) + 5))
+    np.testing.assert_array_equal(f("rep.bin"), np.repeat(pt, 33, axis=0))
     np.testing.assert_array_equal(f("lde.bin"), po.lde(x, lg, 2, po.ref_lde_shift(2)))
     np.testing.assert_array_equal(f("stage0.bin"), po.stage_no_rotate(x, lg, 6, 0, 0))
     np.testing.assert_array_equal(f("stage6.bin"), po.stage_no_rotate(x[:1024], lg, 6, 6, 1))

@@ -191,3 +191,59 @@ def test_perm_grand_product_deep(pl):
     z = pl.to_host(pl.perm_ratio(dev[:3], dev[3:], beta, gamma, ks, dom, grand_product=True))
     del dev
     np.testing.assert_array_equal(z, po.grand_product(ratio))
+
+
+# ---- exp_mod / gen_sequence / repeat_to_poly ---------------------------------
+
+@pytest.mark.parametrize("e", [0, 1, 2, 3, 5, 65537, 2 ** 31 - 1])
+@pytest.mark.parametrize("n", [1, 4097])
+def test_exp_mod(pl, e, n):
+    a = po.uniform(n, 300 + n + e % 997)
+    a[0] = 0
+    got = pl.to_host(pl.exp_mod(pl.to_dev(a), e))
+    np.testing.assert_array_equal(got, po.vec_exp(a, e))
+
+
+def test_exp_mod_in_place_and_errors(pl):
+    a = po.uniform(70000, 77)
+    d = pl.to_dev(a)
+    pl.exp_mod(d, 3, out=d)
+    np.testing.assert_array_equal(pl.to_host(d), po.vec_exp(a, 3))
+    with pytest.raises(pl.PlError):
+        pl.exp_mod(d, -1)
+
+
+@pytest.mark.parametrize("n", [1, 2, 300, 16384, 16385, 3 * 16384 + 77, 1 << 20])
+def test_gen_sequence(pl, n):
+    x = po.uniform(1, 400 + n % 1000)
+    got = pl.to_host(pl.gen_sequence(n, pl.to_dev(x)))
+    np.testing.assert_array_equal(got, po.sequence(x, n))
+
+
+def test_gen_sequence_edge_values(pl):
+    for x in (po.from_u64(0), po.one(), po.root_of_unity(10)):
+        got = pl.to_host(pl.gen_sequence(20000, pl.to_dev(x)))
+        np.testing.assert_array_equal(got, po.sequence(x, 20000))
+
+
+def test_gen_sequence_2p24(pl):
+    x = po.uniform(1, 24)
+    got = pl.to_host(pl.gen_sequence(1 << 24, pl.to_dev(x)))
+    np.testing.assert_array_equal(got, po.sequence(x, 1 << 24))
+
+
+def test_gen_sequence_errors(pl):
+    import torch
+    x = pl.to_dev(po.uniform(1, 1))
+    with pytest.raises(pl.PlError):
+        pl.gen_sequence((1 << 28) + 1, x, out=torch.empty((1, 4), dtype=torch.int64, device=x.device))
+    buf = pl.to_dev(po.uniform(8, 2))
+    with pytest.raises(pl.PlError):  # out overlaps x
+        pl.gen_sequence(4, buf[2:3], out=buf)
+
+
+@pytest.mark.parametrize("n", [1, 1000, (1 << 20) + 3])
+def test_repeat_to_poly(pl, n):
+    x = po.uniform(1, 500 + n % 1000)
+    got = pl.to_host(pl.repeat_to_poly(pl.to_dev(x), n))
+    np.testing.assert_array_equal(got, np.repeat(x, n, axis=0))

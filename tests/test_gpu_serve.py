@@ -121,6 +121,53 @@ def test_same_alone_and_in_mix(pl):
     torch.cuda.synchronize()
 
 
+def per_call_digests(pl, job):
+    """the job's five outputs from the per-call entry points (pl_ntt inverse,
+    pl_lde, pl_perm_grand_product, pl_eval, pl_div_linear), as digests"""
+    import torch
+    lg, lb = job["lg"], job["lb"]
+    n, m = 1 << lg, 1 << (lg + lb)
+    dom, ldom = pl.Domain(lg), pl.LdeDomain(lg, lb)
+    coeffs = torch.empty((8 * n, 4), dtype=torch.int64, device="cuda")
+    for j, d in enumerate(job["dev"]):
+        pl.ntt(d, dom, inverse=True, out=coeffs[j * n:(j + 1) * n])
+    out = {"coeffs": pl.digest(coeffs)}
+    lde = torch.empty((8 * m, 4), dtype=torch.int64, device="cuda")
+    for j in range(8):
+        pl.lde(coeffs[j * n:(j + 1) * n], ldom, out=lde[j * m:(j + 1) * m])
+    out["lde"] = pl.digest(lde)
+    del lde
+    ks = np.concatenate([po.from_u64(1), po.from_u64(7), po.from_u64(13)])
+    z = pl.perm_ratio(job["dev"][0:3], job["dev"][3:6], job["beta"], job["gamma"], ks, dom, grand_product=True)
+    out["z"] = pl.digest(z)
+    y = torch.cat([pl.evaluate(coeffs[j * n:(j + 1) * n], job["zeta"]) for j in range(8)])
+    out["y"] = pl.digest(y)
+    negz = pl.vec_op("neg", job["zeta"])
+    for j in range(8):
+        pl.div_linear(coeffs[j * n:(j + 1) * n], negz)
+    out["q"] = pl.digest(coeffs)
+    return out
+
+
+@pytest.mark.parametrize("lg,lb", [(20, 4), (22, 2), (22, 4), (24, 2)])
+def test_large_jobs_vs_per_call(pl, lg, lb):
+    # the sizes of the served workload (lg 14..24, lambda 2 and 4), in both
+    # server modes, against the per-call entry points (themselves pinned to
+    # the oracle) by digest
+    import torch
+    job = make_job(pl, lg, lb, 7)
+    want = per_call_digests(pl, job)
+    torch.cuda.empty_cache()
+    for mode in (pl.SERVE_BATCHED, pl.SERVE_PLAIN):
+        print("large served job", lg, lb, mode, flush=True)
+        srv = pl.Server(hbm_bytes=pl.job_bytes(lg, lb) * 2 + (8 << 30), mode=mode)
+        jid = srv.submit(lg, lb, job["dev"], job["beta"], job["gamma"], job["zeta"])
+        got = digests(pl, srv.wait(jid), lg, lb)
+        srv.release(jid)
+        srv.close()
+        assert got == want, (lg, lb, mode)
+
+
 def test_admission_by_footprint(pl):
     # an arena that holds one lg-12 job at a time: jobs still all complete,
     # one after another; a job larger than the arena fails cleanly

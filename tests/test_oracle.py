@@ -329,3 +329,30 @@ def test_blowup3_lde_by_brute_force():
                 x = g * pow(w, k, R_MOD) % R_MOD
                 want.append(sum(ci[i] * pow(x, i, R_MOD) for i in range(n)) % R_MOD)
             assert got == want, (lg, g)
+
+
+R_MONT = 1 << 256
+R_INV = pow(R_MONT, -1, R_MOD)
+
+
+def mont_pow_int(s, e):
+    """Montgomery word of x^e for the word s = x R mod r (Python big ints)"""
+    return pow(s * R_INV % R_MOD, e, R_MOD) * R_MONT % R_MOD
+
+
+@pytest.mark.parametrize("e", [0, 1, 2, 3, 4, 5, 17, 65537, 2 ** 31 - 1])
+def test_vec_exp_by_python_pow(e):
+    a = po.uniform(64, 11 + e % 1000)
+    a[3] = 0
+    a[4] = po.one()[0]
+    got = to_int(po.vec_exp(a, e))
+    want = [mont_pow_int(s, e) for s in to_int(a)]
+    assert got == want
+
+
+def test_sequence_by_python_pow():
+    for x in (po.uniform(1, 5), po.from_u64(0), po.one(), po.from_u64(2)):
+        s = to_int(x)[0]
+        got = to_int(po.sequence(x, 300))
+        assert got == [mont_pow_int(s, i) for i in range(300)]
+    assert po.sequence(po.uniform(1, 6), 0).shape == (0, 4)
