@@ -479,11 +479,194 @@ static hipError_t suffix_scan(const fr* a, fr* out, uint64_t n, const fr* pw, fr
 }
 
 // ---------------------------------------------------------------------------
+// evaluation, latency form (kLatMin <= n <= kLatMax)
+// ---------------------------------------------------------------------------
+// A Fr product costs a wave about the same issue time whether or not other
+// waves share its SIMD, so at these sizes the scan above is bound by the
+// products its busiest SIMD issues in sequence: the powers chain, then the
+// reduce (which waits for the powers), then the top level.  Here the powers
+// are raised beside the data instead of before it:
+//
+//   launch 1  data blocks 1..G: thread t of block b owns the run of K
+//             consecutive coefficients at (256 b + t) K and stores its
+//             Horner value L = sum_k a_k x^k, which needs x alone;
+//             table block 0 (on a CU of its own: G + 1 <= 256 blocks) raises
+//             x: mu = x^K, A_j = mu^j (j < 256) and B_b = mu^(256 b)
+//             (b < G), by doubling steps that follow the squaring chain;
+//   launch 2  block b: P_b = B_b sum_t L_(256 b + t) A_t;
+//   launch 3  y = sum_b P_b.
+//
+// The launch boundaries carry the tables and values between blocks; a
+// release/acquire hand-off inside one launch costs more than a launch here
+// (it writes back and invalidates the L2 of every XCD involved).
+// Workspace (Fr elements): A (256), B (256), P (256), L (256 G).
+constexpr uint64_t kLatMin = 1ull << 12;
+constexpr uint64_t kLatMax = 255ull * 256 * 28;  // runs of at most 28 coefficients (in registers)
+constexpr uint32_t kLatMaxG = 255;
+constexpr uint64_t kLatWsElems = 3 * 256 + 256ull * kLatMaxG;
+
+__device__ __forceinline__ fr rp_one_fr() {
+  fr o;
+#pragma unroll
+  for (int i = 0; i < 8; i++) o.l[i] = rp_one_limb(i);
+  return o;
+}
+__device__ __forceinline__ fr to_rp(fr v) {  // x R (stored) -> x R' (five doublings)
+  for (int d = 0; d < 5; d++) v = fr_add(v, v);
+  return v;
+}
+
+// Horner over a run held in registers, each element's digit split ordered
+// after the previous step (an empty asm that reads the running value), so
+// the loads issued before it are waited for one by one as the chain reaches
+// them instead of all at once
+template <int K>
+__device__ __forceinline__ fr horner29_inorder(fr (&v)[K], const uint32_t (&w9)[9]) {
+  v9 h;
+  unpack_ext(v[K - 1].l, h.l);
+  static_for<0, K - 1>([&](auto k_) {
+    constexpr int k = K - 2 - decltype(k_)::value;
+    fr e = v[k];
+    asm volatile("" : "+v"(e.l[0]), "+v"(e.l[1]), "+v"(e.l[2]), "+v"(e.l[3]), "+v"(e.l[4]), "+v"(e.l[5]),
+                 "+v"(e.l[6]), "+v"(e.l[7]) : "v"(h.l[0]));
+    v9 xv;
+    unpack_ext(e.l, xv.l);
+    h = v9_add(v9_mul<false, false, false>(h, w9), xv);
+  });
+  return v9_canonical(h);
+}
+
+template <int K>
+__global__ void __launch_bounds__(kScanThreads) eval_lat_kernel(const fr* a, uint64_t n, const fr* x, fr* ws,
+                                                                 uint32_t G) {
+  const int t = threadIdx.x;
+  if (blockIdx.x == 0) {
+    // ---- table block: wave 0 squares mu, waves 1..3 extend A, then B ----
+    __shared__ fr pw_s[16];
+    __shared__ fr tab[2][256];
+    const int w = t >> 6;
+    if (w == 0) {
+      const fr xp = to_rp(fr_load(x));
+      fr mu = xp;  // x^K R', left to right
+      for (int bit = 30 - __builtin_clz(K); bit >= 0; bit--) {
+        mu = mulp(mu, mu);
+        if ((K >> bit) & 1) mu = mulp(mu, xp);
+      }
+      if (t == 0) pw_s[0] = mu;
+    }
+    tab[0][t] = rp_one_fr();
+    tab[1][t] = rp_one_fr();
+    __syncthreads();
+    // step s: entries [2^s, 2^(s+1)) of A (s < 8) or of B (s >= 8), at most
+    // 128 per step, over the 192 lanes of waves 1..3 (lane j - 64 + 2^s')
+    // B needs steps 8 .. 8 + ceil(log2 G) - 1
+    const int steps = 8 + (G > 1 ? 32 - __builtin_clz(G - 1) : 0);
+    for (int s = 0; s < steps; s++) {
+      const fr p = pw_s[s];  // mu^(2^s) R'
+      if (w == 0) {
+        if (s + 1 < steps) {
+          const fr q = mulp(p, p);
+          if (t == 0) pw_s[s + 1] = q;
+        }
+      } else {
+        const int tb = s < 8 ? 0 : 1, ss = s & 7;
+        const int j = (1 << ss) + t - 64;
+        if (t - 64 < (1 << ss) && (tb == 0 || j < (int)G)) tab[tb][j] = mulp(tab[tb][j - (1 << ss)], p);
+      }
+      __syncthreads();
+    }
+    fr_store(ws + t, tab[0][t]);
+    if (t < (int)G) fr_store(ws + 256 + t, tab[1][t]);
+    return;
+  }
+  // ---- data block b: the Horner value of each thread's run ----
+  const uint32_t b = blockIdx.x - 1;
+  uint32_t w9[9];
+  unpack_ext(to_rp(fr_load(x)).l, w9);  // balanced digits of x R'
+  const uint64_t base = ((uint64_t)b * kScanThreads + t) * K;
+  fr v[K];
+  fr L;
+  if ((uint64_t)(b + 1) * kScanThreads * K <= n) {
+    static_for<0, K>([&](auto j_) {
+      constexpr int k = K - 1 - decltype(j_)::value;
+      v[k] = fr_load(a + base + k);
+    });
+    // the Horner chain starts after every load is issued (w9 passes through)
+    asm volatile("" : "+v"(w9[0]), "+v"(w9[1]), "+v"(w9[2]), "+v"(w9[3]), "+v"(w9[4]), "+v"(w9[5]), "+v"(w9[6]),
+                 "+v"(w9[7]), "+v"(w9[8])::"memory");
+    L = horner29_inorder<K>(v, w9);
+  } else {  // the last data block: runs past n read zeros
+    static_for<0, K>([&](auto j_) {
+      constexpr int k = K - 1 - decltype(j_)::value;
+      v[k] = base + k < n ? fr_load(a + base + k) : fr_zero();
+    });
+    L = horner29<K>(v, w9);
+  }
+  fr_store(ws + 768 + (uint64_t)b * kScanThreads + t, L);
+}
+
+// P_b = B_b sum_t L_(256 b + t) A_t
+__global__ void __launch_bounds__(kScanThreads) eval_lat_combine_kernel(fr* ws) {
+  __shared__ uint4 l[2][kScanThreads];
+  const int t = threadIdx.x;
+  const uint32_t b = blockIdx.x;
+  fr c = mulp(fr_load(ws + 768 + (uint64_t)b * kScanThreads + t), fr_load(ws + t));
+  lds_put(l, t, c);
+  __syncthreads();
+  for (int s = kScanThreads / 2; s > 0; s >>= 1) {
+    if (t < s) {
+      c = fr_add(c, lds_get(l, t + s));
+      lds_put(l, t, c);
+    }
+    __syncthreads();
+  }
+  if (t == 0) fr_store(ws + 512 + b, mulp(c, fr_load(ws + 256 + b)));
+}
+
+// y = sum of the G block values
+__global__ void __launch_bounds__(kScanThreads) eval_lat_sum_kernel(const fr* ws, uint32_t G, fr* y) {
+  __shared__ uint4 l[2][kScanThreads];
+  const int t = threadIdx.x;
+  fr v = t < (int)G ? fr_load(ws + 512 + t) : fr_zero();
+  lds_put(l, t, v);
+  __syncthreads();
+  for (int s = kScanThreads / 2; s > 0; s >>= 1) {
+    if (t < s) {
+      v = fr_add(v, lds_get(l, t + s));
+      lds_put(l, t, v);
+    }
+    __syncthreads();
+  }
+  if (t == 0) fr_store(y, v);
+}
+
+// run length K: the smallest multiple of 4 that keeps G <= 255 data blocks
+static hipError_t evaluate_lat(const fr* a, uint64_t n, const fr* x, fr* y, fr* ws, hipStream_t s) {
+  const uint64_t per4 = (uint64_t)kLatMaxG * kScanThreads * 4;
+  const uint32_t K = (uint32_t)(4 * ((n + per4 - 1) / per4));
+  const uint32_t G = (uint32_t)((n + (uint64_t)kScanThreads * K - 1) / ((uint64_t)kScanThreads * K));
+  const dim3 g(G + 1), b(kScanThreads);
+  switch (K) {
+    case 4: hipLaunchKernelGGL(eval_lat_kernel<4>, g, b, 0, s, a, n, x, ws, G); break;
+    case 8: hipLaunchKernelGGL(eval_lat_kernel<8>, g, b, 0, s, a, n, x, ws, G); break;
+    case 12: hipLaunchKernelGGL(eval_lat_kernel<12>, g, b, 0, s, a, n, x, ws, G); break;
+    case 16: hipLaunchKernelGGL(eval_lat_kernel<16>, g, b, 0, s, a, n, x, ws, G); break;
+    case 20: hipLaunchKernelGGL(eval_lat_kernel<20>, g, b, 0, s, a, n, x, ws, G); break;
+    case 24: hipLaunchKernelGGL(eval_lat_kernel<24>, g, b, 0, s, a, n, x, ws, G); break;
+    default: hipLaunchKernelGGL(eval_lat_kernel<28>, g, b, 0, s, a, n, x, ws, G); break;
+  }
+  hipLaunchKernelGGL(eval_lat_combine_kernel, dim3(G), dim3(kScanThreads), 0, s, ws);
+  hipLaunchKernelGGL(eval_lat_sum_kernel, dim3(1), dim3(kScanThreads), 0, s, (const fr*)ws, G, y);
+  return hipGetLastError();
+}
+
+// ---------------------------------------------------------------------------
 // entry points used by the C-ABI
 // ---------------------------------------------------------------------------
 
 uint64_t scan_scratch_elems(uint64_t n) {
   uint64_t a = pfx_scratch_elems(n), b = sfx_scratch_elems(n);
+  if (n >= kLatMin && n <= kLatMax && b < kLatWsElems) b = kLatWsElems;
   return a > b ? a : b;
 }
 
@@ -501,6 +684,7 @@ hipError_t div_linear(fr* inout, uint64_t n, const fr* c, fr* rem, fr* scratch, 
 }
 
 hipError_t evaluate(const fr* coeffs, uint64_t n, const fr* x, fr* y, fr* scratch, hipStream_t s) {
+  if (n >= kLatMin && n <= kLatMax) return evaluate_lat(coeffs, n, x, y, scratch, s);
   fr* pw = scratch;
   const int np = sfx_pw_count(n);
   hipLaunchKernelGGL(sfx_powers_kernel, dim3(1), dim3(64), 0, s, x, 0, pw, std::min(np, kPw));

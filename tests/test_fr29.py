@@ -144,3 +144,4 @@ def test_generated_columns_current_and_complete():
                 if i < k:
                     want.add(("m[%d]" % i, "nr29(%d)" % (k - i)))
         assert sorted(pairs) == sorted(want), k
+

@@ -186,13 +186,57 @@ def test_eval_golden(pl, golden):
             np.testing.assert_array_equal(pl.to_host(y), k["eval_y_%s_%d" % (tag, n)])
 
 
-@pytest.mark.parametrize("n", [4096, 4097, 123457, 1 << 20, 1 << 24])
+@pytest.mark.parametrize("n", [4095, 4096, 4097, 65281, 123457, 255 * 256 * 17 + 1, 1 << 20, (1 << 20) + 5,
+                               255 * 256 * 28, 255 * 256 * 28 + 1, 1 << 22, 1 << 24])
 def test_eval_vs_oracle(pl, n):
     print("eval", n, flush=True)
     c = po.uniform(n, po.seed(po.OP_EVAL, 0, n & 0xff))
     x = po.uniform(1, 7)
     y = pl.evaluate(pl.to_dev(c), pl.to_dev(x))
     np.testing.assert_array_equal(pl.to_host(y), po.eval_(c, x))
+
+
+def test_eval_reused_work_and_zero_point(pl):
+    # one work buffer across calls with different points (the mid-size form
+    # keeps a flag there that each call must clear), x = 0 and x = 1 included
+    import torch
+    n = (1 << 20) + 3
+    c = po.uniform(n, 31)
+    cd = pl.to_dev(c)
+    work = torch.empty((pl.lib().pl_scan_work_bytes(n) // 32 + 1, 4), dtype=torch.int64, device="cuda")
+    for x in (po.uniform(1, 1), po.from_u64(0), po.one(), po.uniform(1, 2)):
+        y = pl.evaluate(cd, pl.to_dev(x), work=work)
+        np.testing.assert_array_equal(pl.to_host(y), po.eval_(c, x))
+
+
+def test_eval_graph_replays(pl):
+    # captured once, replayed with a new point each time and eager work in
+    # between; every replay equals the oracle
+    import torch
+    n = 1 << 18
+    c = po.uniform(n, 32)
+    cd = pl.to_dev(c)
+    xd = pl.to_dev(po.uniform(1, 3))
+    y = torch.zeros((1, 4), dtype=torch.int64, device="cuda")
+    work = torch.empty((pl.lib().pl_scan_work_bytes(n) // 32 + 1, 4), dtype=torch.int64, device="cuda")
+    s = torch.cuda.Stream()
+    s.wait_stream(torch.cuda.current_stream())
+    with torch.cuda.stream(s):
+        pl.evaluate(cd, xd, y=y, work=work, stream=s)
+    torch.cuda.current_stream().wait_stream(s)
+    torch.cuda.synchronize()
+    g = torch.cuda.CUDAGraph()
+    with torch.cuda.graph(g, stream=s):
+        pl.evaluate(cd, xd, y=y, work=work, stream=s)
+    for rep in range(4):
+        x = po.uniform(1, 100 + rep)
+        xd.copy_(pl.to_dev(x))
+        y.zero_()
+        torch.cuda.synchronize()
+        g.replay()
+        torch.cuda.synchronize()
+        np.testing.assert_array_equal(pl.to_host(y), po.eval_(c, x))
+        pl.evaluate(cd, pl.to_dev(po.uniform(1, 7)), work=torch.empty_like(work))  # eager work between replays
 
 
 def test_eval_empty(pl):
