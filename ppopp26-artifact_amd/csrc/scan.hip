@@ -479,7 +479,7 @@ static hipError_t suffix_scan(const fr* a, fr* out, uint64_t n, const fr* pw, fr
 }
 
 // ---------------------------------------------------------------------------
-// evaluation, latency form (kLatMin <= n <= kLatMax)
+// evaluation and division, latency form (kLatMin <= n <= kLatMax)
 // ---------------------------------------------------------------------------
 // A Fr product costs a wave about the same issue time whether or not other
 // waves share its SIMD, so at these sizes the scan above is bound by the
@@ -489,21 +489,28 @@ static hipError_t suffix_scan(const fr* a, fr* out, uint64_t n, const fr* pw, fr
 //
 //   launch 1  data blocks 1..G: thread t of block b owns the run of K
 //             consecutive coefficients at (256 b + t) K and stores its
-//             Horner value L = sum_k a_k x^k, which needs x alone;
+//             Horner value L = sum_k a_k m^k, which needs m alone;
 //             table block 0 (on a CU of its own: G + 1 <= 256 blocks) raises
-//             x: mu = x^K, A_j = mu^j (j < 256) and B_b = mu^(256 b)
-//             (b < G), by doubling steps that follow the squaring chain;
+//             m: mu = m^K, its squares mu^(2^s), A_j = mu^j (j < 256) and
+//             B_b = mu^(256 b) (b < G), by doubling steps that follow the
+//             squaring chain;
+//   evaluation (m = x)
 //   launch 2  block b: P_b = B_b sum_t L_(256 b + t) A_t;
-//   launch 3  y = sum_b P_b.
+//   launch 3  y = sum_b P_b;
+//   division by X + c (m = -c; s_i = a_i + m s_(i+1), q_i = s_(i+1))
+//   launch 2  block b: P_b = sum_t L_(256 b + t) A_t (its suffix value);
+//   launch 3  block b forms its carry C_b = sum_j P_(b+1+j) B_j, folds it
+//             into its last thread, scans its 256 run values with mu (the
+//             carry at the end of each run) and rescans each run in place.
 //
 // The launch boundaries carry the tables and values between blocks; a
 // release/acquire hand-off inside one launch costs more than a launch here
 // (it writes back and invalidates the L2 of every XCD involved).
-// Workspace (Fr elements): A (256), B (256), P (256), L (256 G).
 constexpr uint64_t kLatMin = 1ull << 12;
 constexpr uint64_t kLatMax = 255ull * 256 * 28;  // runs of at most 28 coefficients (in registers)
 constexpr uint32_t kLatMaxG = 255;
-constexpr uint64_t kLatWsElems = 3 * 256 + 256ull * kLatMaxG;
+// workspace (Fr elements): A, B, P (256 each), mu^(2^s) (16), L (256 G)
+constexpr uint64_t kWsA = 0, kWsB = 256, kWsP = 512, kWsPw = 768, kWsL = 784;
 
 __device__ __forceinline__ fr rp_one_fr() {
   fr o;
@@ -514,6 +521,12 @@ __device__ __forceinline__ fr rp_one_fr() {
 __device__ __forceinline__ fr to_rp(fr v) {  // x R (stored) -> x R' (five doublings)
   for (int d = 0; d < 5; d++) v = fr_add(v, v);
   return v;
+}
+// the multiplier m R' of a scan: x, or -c for a division by X + c
+__device__ __forceinline__ fr load_m(const fr* x, int neg) {
+  fr m = fr_load(x);
+  if (neg) m = fr_neg(m);
+  return to_rp(m);
 }
 
 // Horner over a run held in registers, each element's digit split ordered
@@ -536,9 +549,27 @@ __device__ __forceinline__ fr horner29_inorder(fr (&v)[K], const uint32_t (&w9)[
   return v9_canonical(h);
 }
 
+// a thread's run of K coefficients in registers, loaded top first; zeros
+// past n (only the last data block checks, a uniform branch)
 template <int K>
-__global__ void __launch_bounds__(kScanThreads) eval_lat_kernel(const fr* a, uint64_t n, const fr* x, fr* ws,
-                                                                 uint32_t G) {
+__device__ __forceinline__ void load_run(const fr* a, uint64_t n, uint64_t base, bool full, fr (&v)[K]) {
+  if (full) {
+    static_for<0, K>([&](auto j_) {
+      constexpr int k = K - 1 - decltype(j_)::value;
+      v[k] = fr_load(a + base + k);
+    });
+  } else {
+    static_for<0, K>([&](auto j_) {
+      constexpr int k = K - 1 - decltype(j_)::value;
+      v[k] = base + k < n ? fr_load(a + base + k) : fr_zero();
+    });
+  }
+}
+
+// launch 1; steps: doubling steps of the table block (8 for A, then B's)
+template <int K>
+__global__ void __launch_bounds__(kScanThreads) lat_runs_kernel(const fr* a, uint64_t n, const fr* x, int neg,
+                                                                fr* ws, uint32_t G, int steps) {
   const int t = threadIdx.x;
   if (blockIdx.x == 0) {
     // ---- table block: wave 0 squares mu, waves 1..3 extend A, then B ----
@@ -546,8 +577,8 @@ __global__ void __launch_bounds__(kScanThreads) eval_lat_kernel(const fr* a, uin
     __shared__ fr tab[2][256];
     const int w = t >> 6;
     if (w == 0) {
-      const fr xp = to_rp(fr_load(x));
-      fr mu = xp;  // x^K R', left to right
+      const fr xp = load_m(x, neg);
+      fr mu = xp;  // m^K R', left to right
       for (int bit = 30 - __builtin_clz(K); bit >= 0; bit--) {
         mu = mulp(mu, mu);
         if ((K >> bit) & 1) mu = mulp(mu, xp);
@@ -557,10 +588,8 @@ __global__ void __launch_bounds__(kScanThreads) eval_lat_kernel(const fr* a, uin
     tab[0][t] = rp_one_fr();
     tab[1][t] = rp_one_fr();
     __syncthreads();
-    // step s: entries [2^s, 2^(s+1)) of A (s < 8) or of B (s >= 8), at most
-    // 128 per step, over the 192 lanes of waves 1..3 (lane j - 64 + 2^s')
-    // B needs steps 8 .. 8 + ceil(log2 G) - 1
-    const int steps = 8 + (G > 1 ? 32 - __builtin_clz(G - 1) : 0);
+    // step s: entries [2^s', 2^(s'+1)) of A (s < 8) or of B (s >= 8),
+    // s' = s mod 8, at most 128 per step, over the 192 lanes of waves 1..3
     for (int s = 0; s < steps; s++) {
       const fr p = pw_s[s];  // mu^(2^s) R'
       if (w == 0) {
@@ -575,42 +604,37 @@ __global__ void __launch_bounds__(kScanThreads) eval_lat_kernel(const fr* a, uin
       }
       __syncthreads();
     }
-    fr_store(ws + t, tab[0][t]);
-    if (t < (int)G) fr_store(ws + 256 + t, tab[1][t]);
+    fr_store(ws + kWsA + t, tab[0][t]);
+    if (t < (int)G) fr_store(ws + kWsB + t, tab[1][t]);
+    if (t < steps) fr_store(ws + kWsPw + t, pw_s[t]);
     return;
   }
   // ---- data block b: the Horner value of each thread's run ----
   const uint32_t b = blockIdx.x - 1;
   uint32_t w9[9];
-  unpack_ext(to_rp(fr_load(x)).l, w9);  // balanced digits of x R'
+  unpack_ext(load_m(x, neg).l, w9);  // balanced digits of m R'
   const uint64_t base = ((uint64_t)b * kScanThreads + t) * K;
+  const bool full = (uint64_t)(b + 1) * kScanThreads * K <= n;
   fr v[K];
+  load_run<K>(a, n, base, full, v);
   fr L;
-  if ((uint64_t)(b + 1) * kScanThreads * K <= n) {
-    static_for<0, K>([&](auto j_) {
-      constexpr int k = K - 1 - decltype(j_)::value;
-      v[k] = fr_load(a + base + k);
-    });
+  if (full) {
     // the Horner chain starts after every load is issued (w9 passes through)
     asm volatile("" : "+v"(w9[0]), "+v"(w9[1]), "+v"(w9[2]), "+v"(w9[3]), "+v"(w9[4]), "+v"(w9[5]), "+v"(w9[6]),
                  "+v"(w9[7]), "+v"(w9[8])::"memory");
     L = horner29_inorder<K>(v, w9);
-  } else {  // the last data block: runs past n read zeros
-    static_for<0, K>([&](auto j_) {
-      constexpr int k = K - 1 - decltype(j_)::value;
-      v[k] = base + k < n ? fr_load(a + base + k) : fr_zero();
-    });
+  } else {
     L = horner29<K>(v, w9);
   }
-  fr_store(ws + 768 + (uint64_t)b * kScanThreads + t, L);
+  fr_store(ws + kWsL + (uint64_t)b * kScanThreads + t, L);
 }
 
-// P_b = B_b sum_t L_(256 b + t) A_t
-__global__ void __launch_bounds__(kScanThreads) eval_lat_combine_kernel(fr* ws) {
+// launch 2: P_b = sum_t L_(256 b + t) A_t, times B_b for the evaluation
+__global__ void __launch_bounds__(kScanThreads) lat_combine_kernel(fr* ws, int with_b) {
   __shared__ uint4 l[2][kScanThreads];
   const int t = threadIdx.x;
   const uint32_t b = blockIdx.x;
-  fr c = mulp(fr_load(ws + 768 + (uint64_t)b * kScanThreads + t), fr_load(ws + t));
+  fr c = mulp(fr_load(ws + kWsL + (uint64_t)b * kScanThreads + t), fr_load(ws + kWsA + t));
   lds_put(l, t, c);
   __syncthreads();
   for (int s = kScanThreads / 2; s > 0; s >>= 1) {
@@ -620,14 +644,14 @@ __global__ void __launch_bounds__(kScanThreads) eval_lat_combine_kernel(fr* ws) 
     }
     __syncthreads();
   }
-  if (t == 0) fr_store(ws + 512 + b, mulp(c, fr_load(ws + 256 + b)));
+  if (t == 0) fr_store(ws + kWsP + b, with_b ? mulp(c, fr_load(ws + kWsB + b)) : c);
 }
 
-// y = sum of the G block values
+// evaluation, launch 3: y = sum of the G block values
 __global__ void __launch_bounds__(kScanThreads) eval_lat_sum_kernel(const fr* ws, uint32_t G, fr* y) {
   __shared__ uint4 l[2][kScanThreads];
   const int t = threadIdx.x;
-  fr v = t < (int)G ? fr_load(ws + 512 + t) : fr_zero();
+  fr v = t < (int)G ? fr_load(ws + kWsP + t) : fr_zero();
   lds_put(l, t, v);
   __syncthreads();
   for (int s = kScanThreads / 2; s > 0; s >>= 1) {
@@ -640,23 +664,112 @@ __global__ void __launch_bounds__(kScanThreads) eval_lat_sum_kernel(const fr* ws
   if (t == 0) fr_store(y, v);
 }
 
-// run length K: the smallest multiple of 4 that keeps G <= 255 data blocks
-static hipError_t evaluate_lat(const fr* a, uint64_t n, const fr* x, fr* y, fr* ws, hipStream_t s) {
-  const uint64_t per4 = (uint64_t)kLatMaxG * kScanThreads * 4;
-  const uint32_t K = (uint32_t)(4 * ((n + per4 - 1) / per4));
-  const uint32_t G = (uint32_t)((n + (uint64_t)kScanThreads * K - 1) / ((uint64_t)kScanThreads * K));
-  const dim3 g(G + 1), b(kScanThreads);
-  switch (K) {
-    case 4: hipLaunchKernelGGL(eval_lat_kernel<4>, g, b, 0, s, a, n, x, ws, G); break;
-    case 8: hipLaunchKernelGGL(eval_lat_kernel<8>, g, b, 0, s, a, n, x, ws, G); break;
-    case 12: hipLaunchKernelGGL(eval_lat_kernel<12>, g, b, 0, s, a, n, x, ws, G); break;
-    case 16: hipLaunchKernelGGL(eval_lat_kernel<16>, g, b, 0, s, a, n, x, ws, G); break;
-    case 20: hipLaunchKernelGGL(eval_lat_kernel<20>, g, b, 0, s, a, n, x, ws, G); break;
-    case 24: hipLaunchKernelGGL(eval_lat_kernel<24>, g, b, 0, s, a, n, x, ws, G); break;
-    default: hipLaunchKernelGGL(eval_lat_kernel<28>, g, b, 0, s, a, n, x, ws, G); break;
+// division, launch 3: block b forms its carry C_b = s at the end of block b
+// = sum_(j >= 0) P_(b+1+j) B_j (B_j = M^j, M = mu^256: one product per thread
+// and a sum), folds it into thread 255 (s at the start of that run =
+// L + mu C_b), scans the run values with mu (s at the start of each run; the
+// next thread's is the carry at its end) and rescans its run:
+// q_i = s_(i+1), s_i = a_i + m s_(i+1).  Block 0 also writes the remainder
+// s_0 = P_0 + M C_0.  Every thread reads its whole run before writing it, so
+// q may be a.
+template <int K>
+__global__ void __launch_bounds__(kScanThreads) div_lat_down_kernel(const fr* a, fr* q, uint64_t n, const fr* c,
+                                                                    fr* ws, uint32_t G, fr* rem) {
+  __shared__ uint4 l[2][kScanThreads];
+  const int t = threadIdx.x;
+  const uint32_t b = blockIdx.x;
+  const uint64_t base = ((uint64_t)b * kScanThreads + t) * K;
+  const bool full = (uint64_t)(b + 1) * kScanThreads * K <= n;
+  fr v[K];
+  load_run<K>(a, n, base, full, v);
+  fr cb = b + 1 + t < G ? mulp(fr_load(ws + kWsP + b + 1 + t), fr_load(ws + kWsB + t)) : fr_zero();
+  lds_put(l, t, cb);
+  __syncthreads();
+  for (int s = kScanThreads / 2; s > 0; s >>= 1) {
+    if (t < s) {
+      cb = fr_add(cb, lds_get(l, t + s));
+      lds_put(l, t, cb);
+    }
+    __syncthreads();
   }
-  hipLaunchKernelGGL(eval_lat_combine_kernel, dim3(G), dim3(kScanThreads), 0, s, ws);
+  cb = lds_get(l, 0);
+  __syncthreads();
+  if (b == 0 && t == 0 && rem) {
+    const fr p0 = fr_load(ws + kWsP);
+    fr_store(rem, G > 1 ? fr_add(p0, mulp(cb, fr_load(ws + kWsB + 1))) : p0);
+  }
+  fr h = fr_load(ws + kWsL + (uint64_t)b * kScanThreads + t);
+  if (t == kScanThreads - 1) h = fr_add(h, mulp(cb, fr_load(ws + kWsPw)));
+  block_suffix<0>(h, l, ws + kWsPw);
+  const fr cin = t + 1 < kScanThreads ? lds_get(l, t + 1) : cb;
+  uint32_t w9[9];
+  unpack_ext(load_m(c, 1).l, w9);
+  v9 sn;
+  unpack_ext(cin.l, sn.l);
+  static_for<0, K>([&](auto j_) {
+    constexpr int k = K - 1 - decltype(j_)::value;
+    if (base + k < n) fr_store(q + base + k, k == K - 1 ? cin : v9_canonical(sn));
+    if constexpr (k > 0) {
+      v9 xv;
+      unpack_ext(v[k].l, xv.l);
+      sn = v9_add(v9_mul<false, false, false>(sn, w9), xv);
+    }
+  });
+}
+
+// run length K: the smallest multiple of 4 that keeps G <= 255 data blocks
+static uint32_t lat_run(uint64_t n) {
+  const uint64_t per4 = (uint64_t)kLatMaxG * kScanThreads * 4;
+  return (uint32_t)(4 * ((n + per4 - 1) / per4));
+}
+static uint32_t lat_blocks(uint64_t n, uint32_t K) {
+  return (uint32_t)((n + (uint64_t)kScanThreads * K - 1) / ((uint64_t)kScanThreads * K));
+}
+static uint64_t lat_ws_elems(uint64_t n) { return kWsL + (uint64_t)kScanThreads * lat_blocks(n, lat_run(n)); }
+
+template <template <int> class F, typename... Args>
+static void lat_dispatch(uint32_t K, Args... args) {
+  switch (K) {
+    case 4: F<4>::run(args...); break;
+    case 8: F<8>::run(args...); break;
+    case 12: F<12>::run(args...); break;
+    case 16: F<16>::run(args...); break;
+    case 20: F<20>::run(args...); break;
+    case 24: F<24>::run(args...); break;
+    default: F<28>::run(args...); break;
+  }
+}
+template <int K>
+struct LatRuns {
+  static void run(dim3 g, hipStream_t s, const fr* a, uint64_t n, const fr* x, int neg, fr* ws, uint32_t G,
+                  int steps) {
+    hipLaunchKernelGGL(lat_runs_kernel<K>, g, dim3(kScanThreads), 0, s, a, n, x, neg, ws, G, steps);
+  }
+};
+template <int K>
+struct LatDown {
+  static void run(dim3 g, hipStream_t s, const fr* a, fr* q, uint64_t n, const fr* c, fr* ws, uint32_t G, fr* rem) {
+    hipLaunchKernelGGL(div_lat_down_kernel<K>, g, dim3(kScanThreads), 0, s, a, q, n, c, ws, G, rem);
+  }
+};
+
+// B and A need 8 + ceil(log2 G) doubling steps
+static int lat_steps(uint32_t G) { return 8 + (G > 1 ? 32 - __builtin_clz(G - 1) : 0); }
+
+static hipError_t evaluate_lat(const fr* a, uint64_t n, const fr* x, fr* y, fr* ws, hipStream_t s) {
+  const uint32_t K = lat_run(n), G = lat_blocks(n, K);
+  lat_dispatch<LatRuns>(K, dim3(G + 1), s, a, n, x, 0, ws, G, lat_steps(G));
+  hipLaunchKernelGGL(lat_combine_kernel, dim3(G), dim3(kScanThreads), 0, s, ws, 1);
   hipLaunchKernelGGL(eval_lat_sum_kernel, dim3(1), dim3(kScanThreads), 0, s, (const fr*)ws, G, y);
+  return hipGetLastError();
+}
+
+// in place: inout <- quotient by (X + c), rem <- remainder (if not null)
+static hipError_t div_lat(fr* inout, uint64_t n, const fr* c, fr* rem, fr* ws, hipStream_t s) {
+  const uint32_t K = lat_run(n), G = lat_blocks(n, K);
+  lat_dispatch<LatRuns>(K, dim3(G + 1), s, (const fr*)inout, n, c, 1, ws, G, lat_steps(G));
+  hipLaunchKernelGGL(lat_combine_kernel, dim3(G), dim3(kScanThreads), 0, s, ws, 0);
+  lat_dispatch<LatDown>(K, dim3(G), s, (const fr*)inout, inout, n, c, ws, G, rem);
   return hipGetLastError();
 }
 
@@ -666,7 +779,7 @@ static hipError_t evaluate_lat(const fr* a, uint64_t n, const fr* x, fr* y, fr* 
 
 uint64_t scan_scratch_elems(uint64_t n) {
   uint64_t a = pfx_scratch_elems(n), b = sfx_scratch_elems(n);
-  if (n >= kLatMin && n <= kLatMax && b < kLatWsElems) b = kLatWsElems;
+  if (n >= kLatMin && n <= kLatMax && b < lat_ws_elems(n)) b = lat_ws_elems(n);
   return a > b ? a : b;
 }
 
@@ -677,6 +790,7 @@ hipError_t grand_product(fr* inout, fr* incl_out, uint64_t n, fr* scratch, hipSt
 }
 
 hipError_t div_linear(fr* inout, uint64_t n, const fr* c, fr* rem, fr* scratch, hipStream_t s) {
+  if (n >= kLatMin && n <= kLatMax) return div_lat(inout, n, c, rem, scratch, s);
   fr* pw = scratch;
   const int np = sfx_pw_count(n);
   hipLaunchKernelGGL(sfx_powers_kernel, dim3(1), dim3(64), 0, s, c, 1, pw, std::min(np, kPw));

@@ -258,7 +258,8 @@ def test_div_golden(pl, golden):
         np.testing.assert_array_equal(pl.to_host(rem), k["div_rem_%d" % n])
 
 
-@pytest.mark.parametrize("n", [4096, 4097, 77777, 1 << 20, (1 << 24) + 3])
+@pytest.mark.parametrize("n", [4095, 4096, 4097, 65281, 77777, 255 * 256 * 17 + 1, 1 << 20, (1 << 20) + 5,
+                               255 * 256 * 28, 255 * 256 * 28 + 1, (1 << 24) + 3])
 def test_div_vs_oracle(pl, n):
     import torch
     print("div", n, flush=True)
@@ -272,6 +273,21 @@ def test_div_vs_oracle(pl, n):
     np.testing.assert_array_equal(pl.to_host(ad), q)
     np.testing.assert_array_equal(pl.to_host(rem), r)
     np.testing.assert_array_equal(pl.to_host(cd), c)  # divisor untouched
+
+
+def test_div_reused_work_and_special_divisors(pl):
+    # one work buffer across calls; c = 0 (division by X), c = -1 (X - 1)
+    import torch
+    n = (1 << 20) + 3
+    a = po.uniform(n, 41)
+    work = torch.empty((pl.lib().pl_scan_work_bytes(n) // 32 + 1, 4), dtype=torch.int64, device="cuda")
+    for c in (po.uniform(1, 1), po.from_u64(0), po.neg(po.one()), po.uniform(1, 2)):
+        ad = pl.to_dev(a)
+        rem = torch.empty((1, 4), dtype=torch.int64, device="cuda")
+        pl.div_linear(ad, pl.to_dev(c), rem=rem, work=work)
+        q, r = po.div(a, c)
+        np.testing.assert_array_equal(pl.to_host(ad), q)
+        np.testing.assert_array_equal(pl.to_host(rem), r)
 
 
 def test_miss_ops_literals(pl, golden):
