@@ -73,6 +73,8 @@ def parse():
     ap.add_argument("--no-tune", action="store_true", help="default pass splits")
     ap.add_argument("--settle-ms", type=float, default=300.0,
                     help="untimed device time each section runs before timing (0: none, e.g. under counters)")
+    ap.add_argument("--pcie-probe", action="store_true",
+                    help="only time pinned host <-> device copies of 2^lg * 32 bytes (1-D, 2-D, both ways at once)")
     ap.add_argument("--copy-probe", action="store_true",
                     help="only run a plain device copy of 2^lg * 32 bytes (HBM counter calibration)")
     return ap.parse_args()
@@ -300,6 +302,84 @@ def bench_batch(pl, synth, steps, stream):
         "graph_kernel_nodes": nk, "plan_bytes": nbytes,
         "bit_exact_vs_plain": d_b == d_p,
     }
+
+
+def pcie_probe(lg):
+    """Pinned host <-> device copy rates for 2^lg elements (the ceiling of the
+    host-to-host NTT): plain copies each way, both ways at once on two
+    streams, and 2-D copies of 2^6-row column groups as the host-to-host
+    NTT issues them (and the same groups as one 1-D copy per row)."""
+    import ctypes
+    import torch
+    n = 1 << lg
+    nbytes = n * 32
+    h_in = torch.empty((n, 4), dtype=torch.int64).pin_memory()
+    h_out = torch.empty((n, 4), dtype=torch.int64).pin_memory()
+    d_a = torch.empty((n, 4), dtype=torch.int64, device="cuda")
+    d_b = torch.empty((n, 4), dtype=torch.int64, device="cuda")
+    s1, s2 = torch.cuda.Stream(), torch.cuda.Stream()
+
+    def rate(fn, reps=5):
+        fn()
+        torch.cuda.synchronize()
+        t = time.perf_counter()
+        for _ in range(reps):
+            fn()
+        torch.cuda.synchronize()
+        return round(nbytes / ((time.perf_counter() - t) / reps) / 1e9, 2)
+
+    def both():
+        with torch.cuda.stream(s1):
+            d_a.copy_(h_in, non_blocking=True)
+        with torch.cuda.stream(s2):
+            h_out.copy_(d_b, non_blocking=True)
+
+    out = {"bytes": nbytes, "h2d_gbps": rate(lambda: d_a.copy_(h_in, non_blocking=True)),
+           "d2h_gbps": rate(lambda: h_out.copy_(d_a, non_blocking=True)), "both_ways_gbps_each": rate(both)}
+    hip = ctypes.CDLL("libamdhip64.so")
+    hip.hipMemcpy2DAsync.argtypes = [ctypes.c_void_p, ctypes.c_size_t, ctypes.c_void_p, ctypes.c_size_t,
+                                     ctypes.c_size_t, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p]
+    hip.hipMemcpyAsync.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_size_t, ctypes.c_int, ctypes.c_void_p]
+    cols = n >> 6  # 2^6 rows of 2^(lg-6) elements
+    for groups in (1, 4, 64):
+        w = cols // groups * 32
+
+        def g2d():
+            st = torch.cuda.current_stream().cuda_stream
+            for g in range(groups):
+                r = hip.hipMemcpy2DAsync(ctypes.c_void_p(d_a.data_ptr() + g * w), cols * 32,
+                                         ctypes.c_void_p(h_in.data_ptr() + g * w), cols * 32, w, 64, 1,
+                                         ctypes.c_void_p(st))
+                if r != 0:
+                    raise RuntimeError("hipMemcpy2DAsync: %d" % r)
+        out["h2d_2d_groups_%d_gbps" % groups] = rate(g2d)
+
+        def d2d():
+            st = torch.cuda.current_stream().cuda_stream
+            for g in range(groups):
+                r = hip.hipMemcpy2DAsync(ctypes.c_void_p(h_out.data_ptr() + g * w), cols * 32,
+                                         ctypes.c_void_p(d_a.data_ptr() + g * w), cols * 32, w, 64, 2,
+                                         ctypes.c_void_p(st))
+                if r != 0:
+                    raise RuntimeError("hipMemcpy2DAsync: %d" % r)
+        out["d2h_2d_groups_%d_gbps" % groups] = rate(d2d)
+
+        def d2d_side():
+            with torch.cuda.stream(s2):
+                d2d()
+        out["d2h_2d_groups_%d_side_stream_gbps" % groups] = rate(d2d_side)
+
+        def d2h_rows():
+            st = torch.cuda.current_stream().cuda_stream
+            for g in range(groups):
+                for row in range(64):
+                    off = row * cols * 32 + g * w
+                    r = hip.hipMemcpyAsync(ctypes.c_void_p(h_out.data_ptr() + off),
+                                           ctypes.c_void_p(d_a.data_ptr() + off), w, 2, ctypes.c_void_p(st))
+                    if r != 0:
+                        raise RuntimeError("hipMemcpyAsync: %d" % r)
+        out["d2h_rows_groups_%d_gbps" % groups] = rate(d2h_rows)
+    return out
 
 
 def bench_miss_ops(pl, synth, steps, stream, peak):
@@ -577,6 +657,9 @@ def main():
             b.copy_(a)
         torch.cuda.synchronize()
         print(json.dumps({"copy_probe_bytes_read": n * 32, "copy_probe_bytes_written": n * 32, "launches": 3}))
+        return
+    if args.pcie_probe:
+        print(json.dumps(pcie_probe(lg)))
         return
     peak = pl.probe_mad_rate() * 1e12
     progress("measured v_mad_u64_u32 peak %.2f T/s" % (peak / 1e12))

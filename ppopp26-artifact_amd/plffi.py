@@ -645,14 +645,16 @@ def ntt_segmented(x, lg_i, lg_k, params_buf, stream=None):
     return x
 
 
-def ntt_host(h, dom, lg_k=4, npoly=1, inverse=False, out=None, stream=None):
+def ntt_host(h, dom, lg_k=4, npoly=1, inverse=False, out=None, work=None, stream=None):
     """Host-to-host NTT of npoly polynomials held in a host tensor (pinned for
-    overlap), shape (npoly * 2^lg, 4) int64; returns the output host tensor."""
+    overlap), shape (npoly * 2^lg, 4) int64; returns the output host tensor.
+    work: None (the call allocates its device slots) or a device tensor of at
+    least pl_ntt_host_work_bytes(lg, npoly) bytes."""
     import torch
     if out is None:
         out = torch.empty_like(h)
     check("pl_ntt_host", lib().pl_ntt_host(ctypes.c_void_p(h.data_ptr()), ctypes.c_void_p(out.data_ptr()), dom.lg,
-                                           npoly, 1 if inverse else 0, dom.h, lg_k, None, _stream(stream)))
+                                           npoly, 1 if inverse else 0, dom.h, lg_k, _ptr(work), _stream(stream)))
     return out
 
 
