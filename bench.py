@@ -230,12 +230,14 @@ def bench_host_to_host(pl, synth, lg, stream, reps=5):
 
 
 def bench_lde(pl, synth, lg, lb, steps, warmup, stream, peak):
+    """lb >= 1: blowup 2^lb; lb = -1: blowup 3 (cosets 7 w_3N^t)."""
     import torch
-    n, m = 1 << lg, 1 << (lg + lb)
-    dom = pl.LdeDomain(lg, lb)
-    c = dev_uniform(pl, n, synth.seed(synth.OP_LDE, lg, lb))
+    n = 1 << lg
+    m = 3 * n if lb < 0 else 1 << (lg + lb)
+    dom = pl.LdeDomain(lg, 0, blowup3=True) if lb < 0 else pl.LdeDomain(lg, lb)
+    c = dev_uniform(pl, n, synth.seed(synth.OP_LDE, lg, max(lb, 0)))
     out = torch.empty((m, 4), dtype=torch.int64, device="cuda")
-    work = torch.empty((m, 4), dtype=torch.int64, device="cuda")
+    work = torch.empty((max(dom.work_bytes // 32, 1), 4), dtype=torch.int64, device="cuda")
     step = lambda: pl.lde(c, dom, out=out, work=work)  # noqa: E731
     for _ in range(warmup):
         step()
@@ -243,8 +245,12 @@ def bench_lde(pl, synth, lg, lb, steps, warmup, stream, peak):
     torch.cuda.synchronize()
     ms = time_events(step, steps, stream)
     mulmod = (m // 2) * lg + m
+    if lb < 0:
+        what = "coset LDE 2^%d -> 3 x 2^%d (blowup 3, cosets 7 w_3N^t), device-resident" % (lg, lg)
+    else:
+        what = "coset LDE 2^%d -> 2^%d (lambda=%d, shift 7^%d), device-resident" % (lg, lg + lb, lb, 1 << lb)
     res = {
-        "workload": "coset LDE 2^%d -> 2^%d (lambda=%d, shift 7^%d), device-resident" % (lg, lg + lb, lb, 1 << lb),
+        "workload": what,
         "ms": round(ms, 4),
         "value": round(n / (ms * 1e-3), 1),
         "unit": "Fr elems/s (input N)",
@@ -702,6 +708,9 @@ def main():
         torch.cuda.empty_cache()
         result["lde_l3"] = bench_lde(pl, synth, lg, 3, max(1, args.steps // 2), args.warmup, stream, peak)
         progress("lde lambda 3: %.3f ms" % result["lde_l3"]["ms"])
+        torch.cuda.empty_cache()
+        result["lde_b3"] = bench_lde(pl, synth, lg, -1, max(1, args.steps // 2), args.warmup, stream, peak)
+        progress("lde blowup 3: %.3f ms" % result["lde_b3"]["ms"])
         torch.cuda.empty_cache()
     if not args.no_miss:
         result["miss_ops"] = bench_miss_ops(pl, synth, args.steps, stream, peak)

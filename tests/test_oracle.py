@@ -356,3 +356,22 @@ def test_sequence_by_python_pow():
         got = to_int(po.sequence(x, 300))
         assert got == [mont_pow_int(s, i) for i in range(300)]
     assert po.sequence(po.uniform(1, 6), 0).shape == (0, 4)
+
+
+def test_threads_do_not_change_results():
+    # the oracle's transforms split work over host threads; the results do
+    # not depend on how many
+    lg = 15
+    x = po.uniform(1 << lg, 99)
+    was = po.get_threads()
+    try:
+        po.set_threads(1)
+        a = (po.ntt(x, lg), po.ntt(x, lg, inverse=True), po.lde(x[: 1 << 13], 13, 2, po.from_u64(7)),
+             po.coset_intt(x, lg, po.from_u64(5)))
+        po.set_threads(16)
+        b = (po.ntt(x, lg), po.ntt(x, lg, inverse=True), po.lde(x[: 1 << 13], 13, 2, po.from_u64(7)),
+             po.coset_intt(x, lg, po.from_u64(5)))
+    finally:
+        po.set_threads(was)
+    for u, v in zip(a, b):
+        np.testing.assert_array_equal(u, v)
