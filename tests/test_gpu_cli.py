@@ -171,3 +171,29 @@ def test_input_loader_missing_file(tmp_path):
     assert "Error opening file" in p.stderr and "seeded synthetic data" in p.stderr
     x = load(os.path.join(tmp_path, "input.bin"))
     np.testing.assert_array_equal(x, po.uniform(1 << 10, po.seed(po.OP_GP, 10)))
+
+
+def test_ntt_cli_bench_sweep():
+    # the reference's segment-count sweep: lg_k = 1..8 at one size
+    p = run(["NTT", "--bench-sweep", "22", "1"])
+    assert p.returncode == 0
+    assert p.stdout.count("=== lg_N = 22") == 8
+    assert p.stdout.count("our-ntt execution time:") == 8
+
+
+def test_lde_cli_naive_and_sweep():
+    p = run(["LDE", "--bench-naive", "20", "2"])
+    assert p.returncode == 0 and "[naive-lde] lg_N=20, lambda=2" in p.stdout
+    p = run(["LDE", "--bench-sweep", "20", "2", "1"])
+    assert p.returncode == 0 and p.stdout.count("=== lde lg_N = 20, lambda = 2") == 8
+
+
+def test_miss_ops_cli_no_arguments(tmp_path):
+    # the reference's default: 2^22 from ../../input.bin (absent here: the
+    # seeded fallback), every timing line three times
+    d = tmp_path / "a" / "b"
+    d.mkdir(parents=True)
+    p = run(["miss_ops"], cwd=str(d), timeout=600)
+    assert p.returncode == 0
+    for tag in ("[naive_gp]", "[ours_gp]", "[ours_eval]", "[naive_div]", "[ours_div]"):
+        assert p.stdout.count(tag + " execution time:") == 3, tag

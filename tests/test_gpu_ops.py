@@ -78,7 +78,7 @@ def test_lde_naive_vs_oracle(pl, lg_n, lb):
     np.testing.assert_array_equal(out, po.naive_lde(c, lg_n, lb))
 
 
-@pytest.mark.parametrize("lg_n,lb", [(12, 1), (20, 1), (20, 2), (22, 1), (22, 2)])
+@pytest.mark.parametrize("lg_n,lb", [(12, 1), (20, 1), (20, 2), (22, 1), (22, 2), (24, 1)])
 def test_evals_to_coset_lde_pipeline(pl, lg_n, lb):
     """The prover pipeline on evaluations: iNTT -> multiply coefficient i by
     7^i -> zero-pad to 2^lb N -> NTT of size 2^lb N, i.e. pl_ntt(inverse)
@@ -92,6 +92,23 @@ def test_evals_to_coset_lde_pipeline(pl, lg_n, lb):
     torch.cuda.empty_cache()
     want = po.lde(po.ntt(e, lg_n, inverse=True), lg_n, lb, po.from_u64(7))
     assert_fr_equal(out, want, "evals->lde %d %d" % (lg_n, lb))
+
+
+@pytest.mark.parametrize("lg_n", [12, 20, 22, 24])
+def test_evals_to_blowup3_pipeline(pl, lg_n):
+    """The same pipeline with blowup 3: iNTT -> the three cosets 7 w_3N^t of
+    size N (pl_lde3_domain_create, shift 7), against the oracle's composition."""
+    from conftest import assert_fr_equal
+    import torch
+    e = po.uniform(1 << lg_n, po.seed(po.OP_LDE, lg_n, 300))
+    print("evals -> blowup-3 lde %d" % lg_n, flush=True)
+    coeffs = pl.ntt(pl.to_dev(e), pl.Domain(lg_n), inverse=True)
+    dom = pl.LdeDomain(lg_n, 0, shift=po.from_u64(7), blowup3=True)
+    out = pl.to_host(pl.lde(coeffs, dom))
+    dom.close()
+    torch.cuda.empty_cache()
+    want = po.lde3(po.ntt(e, lg_n, inverse=True), lg_n, po.from_u64(7))
+    assert_fr_equal(out, want, "evals->lde3 %d" % lg_n)
 
 
 @pytest.mark.parametrize("lg_n,lb", [(12, 2), (16, 3)])
