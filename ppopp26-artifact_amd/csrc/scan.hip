@@ -497,11 +497,14 @@ static hipError_t suffix_scan(const fr* a, fr* out, uint64_t n, const fr* pw, fr
 //   evaluation (m = x)
 //   launch 2  block b: P_b = B_b sum_t L_(256 b + t) A_t;
 //   launch 3  y = sum_b P_b;
-//   division by X + c (m = -c; s_i = a_i + m s_(i+1), q_i = s_(i+1))
-//   launch 2  block b: P_b = sum_t L_(256 b + t) A_t (its suffix value);
-//   launch 3  block b forms its carry C_b = sum_j P_(b+1+j) B_j, folds it
-//             into its last thread, scans its 256 run values with mu (the
-//             carry at the end of each run) and rescans each run in place.
+//   division by X + c (m = -c; s_i = a_i + m s_(i+1), q_i = s_(i+1));
+//             launch 1 also keeps each run's upper-half Horner value H
+//   launch 2  block b scans its 256 run values with mu: the run suffixes S_t
+//             and the block value P_b = S_0;
+//   launch 3  two threads per run: each block forms its carry
+//             C_b = sum_j P_(b+1+j) B_j; the carry at the end of run t is
+//             S_(t+1) + mu^(255 - t) C_b, the lower half starts from
+//             H_t + m^(K/2) times it, and both halves rescan in place.
 //
 // The launch boundaries carry the tables and values between blocks; a
 // release/acquire hand-off inside one launch costs more than a launch here
@@ -509,8 +512,10 @@ static hipError_t suffix_scan(const fr* a, fr* out, uint64_t n, const fr* pw, fr
 constexpr uint64_t kLatMin = 1ull << 12;
 constexpr uint64_t kLatMax = 255ull * 256 * 28;  // runs of at most 28 coefficients (in registers)
 constexpr uint32_t kLatMaxG = 255;
-// workspace (Fr elements): A, B, P (256 each), mu^(2^s) (16), L (256 G)
-constexpr uint64_t kWsA = 0, kWsB = 256, kWsP = 512, kWsPw = 768, kWsL = 784;
+// workspace (Fr elements): A, B, P (256 each), mu^(2^s) (16), m^(K/2) (1),
+// L (256 G; the division replaces it by the run suffixes S), and for the
+// division the upper-half Horner values H (256 G)
+constexpr uint64_t kWsA = 0, kWsB = 256, kWsP = 512, kWsPw = 768, kWsHalf = 784, kWsL = 800;
 
 __device__ __forceinline__ fr rp_one_fr() {
   fr o;
@@ -533,8 +538,9 @@ __device__ __forceinline__ fr load_m(const fr* x, int neg) {
 // after the previous step (an empty asm that reads the running value), so
 // the loads issued before it are waited for one by one as the chain reaches
 // them instead of all at once
+// *upper (if not null): the value after the upper half, sum_(k >= K/2) a_k m^(k - K/2)
 template <int K>
-__device__ __forceinline__ fr horner29_inorder(fr (&v)[K], const uint32_t (&w9)[9]) {
+__device__ __forceinline__ fr horner29_inorder(fr (&v)[K], const uint32_t (&w9)[9], fr* upper) {
   v9 h;
   unpack_ext(v[K - 1].l, h.l);
   static_for<0, K - 1>([&](auto k_) {
@@ -545,6 +551,9 @@ __device__ __forceinline__ fr horner29_inorder(fr (&v)[K], const uint32_t (&w9)[
     v9 xv;
     unpack_ext(e.l, xv.l);
     h = v9_add(v9_mul<false, false, false>(h, w9), xv);
+    if constexpr (k == K / 2) {
+      if (upper) *upper = v9_canonical(h);
+    }
   });
   return v9_canonical(h);
 }
@@ -566,10 +575,12 @@ __device__ __forceinline__ void load_run(const fr* a, uint64_t n, uint64_t base,
   }
 }
 
-// launch 1; steps: doubling steps of the table block (8 for A, then B's)
+// launch 1; steps: doubling steps of the table block (8 for A, then B's);
+// halves (division): also the upper-half Horner value of every run and m^(K/2)
 template <int K>
 __global__ void __launch_bounds__(kScanThreads) lat_runs_kernel(const fr* a, uint64_t n, const fr* x, int neg,
-                                                                fr* ws, uint32_t G, int steps) {
+                                                                fr* ws, uint32_t G, int steps, int halves) {
+  static_assert(K % 2 == 0, "runs split in two halves");
   const int t = threadIdx.x;
   if (blockIdx.x == 0) {
     // ---- table block: wave 0 squares mu, waves 1..3 extend A, then B ----
@@ -578,8 +589,9 @@ __global__ void __launch_bounds__(kScanThreads) lat_runs_kernel(const fr* a, uin
     const int w = t >> 6;
     if (w == 0) {
       const fr xp = load_m(x, neg);
-      fr mu = xp;  // m^K R', left to right
+      fr mu = xp;  // m^K R', left to right; K is even, so m^(K/2) precedes the last squaring
       for (int bit = 30 - __builtin_clz(K); bit >= 0; bit--) {
+        if (bit == 0 && t == 0) fr_store(ws + kWsHalf, mu);
         mu = mulp(mu, mu);
         if ((K >> bit) & 1) mu = mulp(mu, xp);
       }
@@ -617,20 +629,18 @@ __global__ void __launch_bounds__(kScanThreads) lat_runs_kernel(const fr* a, uin
   const bool full = (uint64_t)(b + 1) * kScanThreads * K <= n;
   fr v[K];
   load_run<K>(a, n, base, full, v);
-  fr L;
-  if (full) {
-    // the Horner chain starts after every load is issued (w9 passes through)
-    asm volatile("" : "+v"(w9[0]), "+v"(w9[1]), "+v"(w9[2]), "+v"(w9[3]), "+v"(w9[4]), "+v"(w9[5]), "+v"(w9[6]),
-                 "+v"(w9[7]), "+v"(w9[8])::"memory");
-    L = horner29_inorder<K>(v, w9);
-  } else {
-    L = horner29<K>(v, w9);
-  }
-  fr_store(ws + kWsL + (uint64_t)b * kScanThreads + t, L);
+  // the Horner chain starts after every load is issued (w9 passes through)
+  asm volatile("" : "+v"(w9[0]), "+v"(w9[1]), "+v"(w9[2]), "+v"(w9[3]), "+v"(w9[4]), "+v"(w9[5]), "+v"(w9[6]),
+               "+v"(w9[7]), "+v"(w9[8])::"memory");
+  fr upper;
+  const fr L = horner29_inorder<K>(v, w9, halves ? &upper : nullptr);
+  const uint64_t r = (uint64_t)b * kScanThreads + t;
+  fr_store(ws + kWsL + r, L);
+  if (halves) fr_store(ws + kWsL + (uint64_t)kScanThreads * G + r, upper);
 }
 
-// launch 2: P_b = sum_t L_(256 b + t) A_t, times B_b for the evaluation
-__global__ void __launch_bounds__(kScanThreads) lat_combine_kernel(fr* ws, int with_b) {
+// evaluation, launch 2: P_b = B_b sum_t L_(256 b + t) A_t
+__global__ void __launch_bounds__(kScanThreads) lat_combine_kernel(fr* ws) {
   __shared__ uint4 l[2][kScanThreads];
   const int t = threadIdx.x;
   const uint32_t b = blockIdx.x;
@@ -644,7 +654,7 @@ __global__ void __launch_bounds__(kScanThreads) lat_combine_kernel(fr* ws, int w
     }
     __syncthreads();
   }
-  if (t == 0) fr_store(ws + kWsP + b, with_b ? mulp(c, fr_load(ws + kWsB + b)) : c);
+  if (t == 0) fr_store(ws + kWsP + b, mulp(c, fr_load(ws + kWsB + b)));
 }
 
 // evaluation, launch 3: y = sum of the G block values
@@ -664,51 +674,77 @@ __global__ void __launch_bounds__(kScanThreads) eval_lat_sum_kernel(const fr* ws
   if (t == 0) fr_store(y, v);
 }
 
-// division, launch 3: block b forms its carry C_b = s at the end of block b
-// = sum_(j >= 0) P_(b+1+j) B_j (B_j = M^j, M = mu^256: one product per thread
-// and a sum), folds it into thread 255 (s at the start of that run =
-// L + mu C_b), scans the run values with mu (s at the start of each run; the
-// next thread's is the carry at its end) and rescans its run:
-// q_i = s_(i+1), s_i = a_i + m s_(i+1).  Block 0 also writes the remainder
-// s_0 = P_0 + M C_0.  Every thread reads its whole run before writing it, so
-// q may be a.
+// division, launch 2: the run suffixes of block b, S_t = sum_(t' >= t) L_t' mu^(t' - t)
+// (S_t replaces L_t), and the block value P_b = S_0
+__global__ void __launch_bounds__(kScanThreads) div_lat_suffix_kernel(fr* ws) {
+  __shared__ uint4 l[2][kScanThreads];
+  const int t = threadIdx.x;
+  const uint64_t r = (uint64_t)blockIdx.x * kScanThreads + t;
+  fr v = fr_load(ws + kWsL + r);
+  fr pw[8];  // mu^(2^s), loaded once before the levels
+#pragma unroll
+  for (int s = 0; s < 8; s++) pw[s] = fr_load(ws + kWsPw + s);
+  lds_put(l, t, v);
+  __syncthreads();
+#pragma unroll
+  for (int s = 0; s < 8; s++) {
+    const int o = 1 << s;
+    const fr other = t + o < kScanThreads ? lds_get(l, t + o) : fr_zero();
+    __syncthreads();
+    if (t + o < kScanThreads) v = fr_add(v, mulp(other, pw[s]));
+    lds_put(l, t, v);
+    __syncthreads();
+  }
+  fr_store(ws + kWsL + r, v);
+  if (t == 0) fr_store(ws + kWsP + blockIdx.x, v);
+}
+
+// division, launch 3: two threads per run, each rescanning half of it in
+// place.  Launch block d covers runs [(d mod 2) 128, +128) of launch-1 block
+// b = d / 2.  C_b = s at the end of block b = sum_(j >= 0) P_(b+1+j) B_j (one
+// product per thread and a sum); the carry at the end of run t is
+// S_(t+1) + mu^(255 - t) C_b; the lower half starts from
+// s at the middle = H_t + m^(K/2) (that carry).  q_i = s_(i+1),
+// s_i = a_i + m s_(i+1).  Block 0 also writes the remainder s_0 = P_0 + M C_0.
+// Every thread reads its half-run before writing it, so q may be a.
 template <int K>
 __global__ void __launch_bounds__(kScanThreads) div_lat_down_kernel(const fr* a, fr* q, uint64_t n, const fr* c,
                                                                     fr* ws, uint32_t G, fr* rem) {
+  constexpr int H = K / 2;
   __shared__ uint4 l[2][kScanThreads];
-  const int t = threadIdx.x;
-  const uint32_t b = blockIdx.x;
-  const uint64_t base = ((uint64_t)b * kScanThreads + t) * K;
-  const bool full = (uint64_t)(b + 1) * kScanThreads * K <= n;
-  fr v[K];
-  load_run<K>(a, n, base, full, v);
-  fr cb = b + 1 + t < G ? mulp(fr_load(ws + kWsP + b + 1 + t), fr_load(ws + kWsB + t)) : fr_zero();
-  lds_put(l, t, cb);
+  const int tid = threadIdx.x;
+  const uint32_t b = blockIdx.x >> 1;
+  const int t = (int)((blockIdx.x & 1) * 128 + (tid >> 1));
+  const int upper = tid & 1;
+  const uint64_t r = (uint64_t)b * kScanThreads + t;
+  const uint64_t base = r * K + (upper ? H : 0);
+  fr v[H];
+  load_run<H>(a, n, base, (uint64_t)(b + 1) * kScanThreads * K <= n, v);
+  fr cb = b + 1 + tid < G ? mulp(fr_load(ws + kWsP + b + 1 + tid), fr_load(ws + kWsB + tid)) : fr_zero();
+  lds_put(l, tid, cb);
   __syncthreads();
   for (int s = kScanThreads / 2; s > 0; s >>= 1) {
-    if (t < s) {
-      cb = fr_add(cb, lds_get(l, t + s));
-      lds_put(l, t, cb);
+    if (tid < s) {
+      cb = fr_add(cb, lds_get(l, tid + s));
+      lds_put(l, tid, cb);
     }
     __syncthreads();
   }
   cb = lds_get(l, 0);
-  __syncthreads();
-  if (b == 0 && t == 0 && rem) {
+  if (blockIdx.x == 0 && tid == 0 && rem) {
     const fr p0 = fr_load(ws + kWsP);
     fr_store(rem, G > 1 ? fr_add(p0, mulp(cb, fr_load(ws + kWsB + 1))) : p0);
   }
-  fr h = fr_load(ws + kWsL + (uint64_t)b * kScanThreads + t);
-  if (t == kScanThreads - 1) h = fr_add(h, mulp(cb, fr_load(ws + kWsPw)));
-  block_suffix<0>(h, l, ws + kWsPw);
-  const fr cin = t + 1 < kScanThreads ? lds_get(l, t + 1) : cb;
+  fr cin = mulp(cb, fr_load(ws + kWsA + (kScanThreads - 1 - t)));
+  if (t + 1 < kScanThreads) cin = fr_add(cin, fr_load(ws + kWsL + r + 1));
+  if (!upper) cin = fr_add(fr_load(ws + kWsL + (uint64_t)kScanThreads * G + r), mulp(cin, fr_load(ws + kWsHalf)));
   uint32_t w9[9];
   unpack_ext(load_m(c, 1).l, w9);
   v9 sn;
   unpack_ext(cin.l, sn.l);
-  static_for<0, K>([&](auto j_) {
-    constexpr int k = K - 1 - decltype(j_)::value;
-    if (base + k < n) fr_store(q + base + k, k == K - 1 ? cin : v9_canonical(sn));
+  static_for<0, H>([&](auto j_) {
+    constexpr int k = H - 1 - decltype(j_)::value;
+    if (base + k < n) fr_store(q + base + k, k == H - 1 ? cin : v9_canonical(sn));
     if constexpr (k > 0) {
       v9 xv;
       unpack_ext(v[k].l, xv.l);
@@ -725,7 +761,7 @@ static uint32_t lat_run(uint64_t n) {
 static uint32_t lat_blocks(uint64_t n, uint32_t K) {
   return (uint32_t)((n + (uint64_t)kScanThreads * K - 1) / ((uint64_t)kScanThreads * K));
 }
-static uint64_t lat_ws_elems(uint64_t n) { return kWsL + (uint64_t)kScanThreads * lat_blocks(n, lat_run(n)); }
+static uint64_t lat_ws_elems(uint64_t n) { return kWsL + 2ull * kScanThreads * lat_blocks(n, lat_run(n)); }
 
 template <template <int> class F, typename... Args>
 static void lat_dispatch(uint32_t K, Args... args) {
@@ -742,8 +778,8 @@ static void lat_dispatch(uint32_t K, Args... args) {
 template <int K>
 struct LatRuns {
   static void run(dim3 g, hipStream_t s, const fr* a, uint64_t n, const fr* x, int neg, fr* ws, uint32_t G,
-                  int steps) {
-    hipLaunchKernelGGL(lat_runs_kernel<K>, g, dim3(kScanThreads), 0, s, a, n, x, neg, ws, G, steps);
+                  int steps, int halves) {
+    hipLaunchKernelGGL(lat_runs_kernel<K>, g, dim3(kScanThreads), 0, s, a, n, x, neg, ws, G, steps, halves);
   }
 };
 template <int K>
@@ -758,8 +794,8 @@ static int lat_steps(uint32_t G) { return 8 + (G > 1 ? 32 - __builtin_clz(G - 1)
 
 static hipError_t evaluate_lat(const fr* a, uint64_t n, const fr* x, fr* y, fr* ws, hipStream_t s) {
   const uint32_t K = lat_run(n), G = lat_blocks(n, K);
-  lat_dispatch<LatRuns>(K, dim3(G + 1), s, a, n, x, 0, ws, G, lat_steps(G));
-  hipLaunchKernelGGL(lat_combine_kernel, dim3(G), dim3(kScanThreads), 0, s, ws, 1);
+  lat_dispatch<LatRuns>(K, dim3(G + 1), s, a, n, x, 0, ws, G, lat_steps(G), 0);
+  hipLaunchKernelGGL(lat_combine_kernel, dim3(G), dim3(kScanThreads), 0, s, ws);
   hipLaunchKernelGGL(eval_lat_sum_kernel, dim3(1), dim3(kScanThreads), 0, s, (const fr*)ws, G, y);
   return hipGetLastError();
 }
@@ -767,9 +803,9 @@ static hipError_t evaluate_lat(const fr* a, uint64_t n, const fr* x, fr* y, fr* 
 // in place: inout <- quotient by (X + c), rem <- remainder (if not null)
 static hipError_t div_lat(fr* inout, uint64_t n, const fr* c, fr* rem, fr* ws, hipStream_t s) {
   const uint32_t K = lat_run(n), G = lat_blocks(n, K);
-  lat_dispatch<LatRuns>(K, dim3(G + 1), s, (const fr*)inout, n, c, 1, ws, G, lat_steps(G));
-  hipLaunchKernelGGL(lat_combine_kernel, dim3(G), dim3(kScanThreads), 0, s, ws, 0);
-  lat_dispatch<LatDown>(K, dim3(G), s, (const fr*)inout, inout, n, c, ws, G, rem);
+  lat_dispatch<LatRuns>(K, dim3(G + 1), s, (const fr*)inout, n, c, 1, ws, G, lat_steps(G), 1);
+  hipLaunchKernelGGL(div_lat_suffix_kernel, dim3(G), dim3(kScanThreads), 0, s, ws);
+  lat_dispatch<LatDown>(K, dim3(2 * G), s, (const fr*)inout, inout, n, c, ws, G, rem);
   return hipGetLastError();
 }
 
