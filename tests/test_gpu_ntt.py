@@ -166,3 +166,42 @@ def test_ntt_roundtrip_2p28(pl):
         np.testing.assert_array_equal(pl.to_host(pl.evaluate(xd, pt)), pl.to_host(y[k:k + 1]))
     del xd, y
     torch.cuda.empty_cache()
+
+
+def test_concurrent_calls_on_distinct_streams(pl):
+    # the C-ABI is thread-safe for distinct streams: four host threads, each
+    # with its own stream, domain and buffers, run transforms and scans at
+    # once (ctypes releases the interpreter lock during the calls)
+    import threading
+    import torch
+    lgs = [12, 14, 16, 18]
+    xs = [po.uniform(1 << lg, 700 + lg) for lg in lgs]
+    pts = [po.uniform(1, 800 + lg) for lg in lgs]
+    got, err = [None] * 4, []
+
+    def work(i):
+        try:
+            s = torch.cuda.Stream()
+            lg = lgs[i]
+            with torch.cuda.stream(s):
+                dom = pl.Domain(lg)
+                d = pl.to_dev(xs[i])
+                outs = []
+                for _ in range(5):
+                    y = pl.ntt(d, dom, stream=s)
+                    e = pl.evaluate(d, pl.to_dev(pts[i]), stream=s)
+                    outs = [y, e]
+                s.synchronize()
+                got[i] = [pl.to_host(t) for t in outs]
+        except Exception as ex:  # reported below
+            err.append(repr(ex))
+
+    ts = [threading.Thread(target=work, args=(i,)) for i in range(4)]
+    for t in ts:
+        t.start()
+    for t in ts:
+        t.join(timeout=120)
+    assert not err, err
+    for i, lg in enumerate(lgs):
+        np.testing.assert_array_equal(got[i][0], po.ntt(xs[i], lg))
+        np.testing.assert_array_equal(got[i][1], po.eval_(xs[i], pts[i]))
