@@ -66,7 +66,7 @@ def parse():
     ap.add_argument("--no-serve", action="store_true")
     ap.add_argument("--no-miss", action="store_true", help="skip grand product / evaluation / division")
     ap.add_argument("--serve-jobs", type=int, default=200, help="jobs in the served mix (every run uses the same mix)")
-    ap.add_argument("--serve-slots", type=int, default=3, help="job groups running at once (batched form)")
+    ap.add_argument("--serve-slots", type=int, default=2, help="job groups running at once (batched form)")
     ap.add_argument("--serve-large", type=int, default=25, help="log2 of the work that makes a job large")
     ap.add_argument("--plans", default=os.path.join(ROOT, "ppopp26-artifact_amd", "tune", "mi355x_plans.txt"),
                     help="autotuned pass-split cache (measured and written when absent)")

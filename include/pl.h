@@ -104,7 +104,7 @@ int pl_ntt_stage_no_rotate(void* d_chunk, uint64_t chunk_elems, int lg_domain, i
                            uint64_t chunk_id, const void* d_params, pl_stream_t stream);
 
 /* The chunked flow's index algebra [ct_stage_indices, ntt.cu:168-178;
- * rotate_stage_output_index, ntt.cu:190-195] (host functions): thread tid
+ * rotate_stage_output_index, ntt.cu:189-195] (host functions): thread tid
  * of a step (stage, iterations) owns the global pair (*idx0, *idx1 =
  * *idx0 + 2^stage); a step's output element g belongs at g with bits
  * [stage, stage + iterations) rotated right by one. */
@@ -397,7 +397,7 @@ void pl_batch_destroy(pl_batch* b);
 enum pl_serve_mode { PL_SERVE_BATCHED = 0, PL_SERVE_PLAIN = 1 };
 typedef struct pl_server_config {
   size_t hbm_bytes;      /* arena size; 0 = free device memory less 32 GiB for domain tables */
-  int slots;             /* groups running at once, 1..8; 0 = 3 (plain: 1) */
+  int slots;             /* groups running at once, 1..8; 0 = 2 (plain: 1) */
   int mode;              /* pl_serve_mode */
   int max_group_jobs;    /* jobs merged into one group; 0 = 16 */
   uint64_t group_elems;  /* merge while a group's 8 N sum stays <= this; 0 = 2^22 */

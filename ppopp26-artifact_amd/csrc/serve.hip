@@ -502,7 +502,7 @@ int pl_server_create(const pl_server_config* cfg, pl_server** out) {
   if (cfg) c = *cfg;
   if (c.mode != PL_SERVE_BATCHED && c.mode != PL_SERVE_PLAIN) return PL_ERR_ARG;
   if (c.slots < 0 || c.slots > 8 || c.max_group_jobs < 0) return PL_ERR_ARG;
-  if (c.slots == 0) c.slots = 3;
+  if (c.slots == 0) c.slots = 2;  // two stream pairs: the process's four hardware queues
   if (c.mode == PL_SERVE_PLAIN) c.slots = 1;
   if (c.max_group_jobs == 0) c.max_group_jobs = 16;
   if (c.group_elems == 0) c.group_elems = 1ull << 22;
