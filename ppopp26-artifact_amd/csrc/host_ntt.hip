@@ -154,12 +154,17 @@ int pl_ntt_host(const void* h_in, void* h_out, int lg, uint32_t npoly, int inver
   if (((uintptr_t)d_work & 15) != 0) return PL_ERR_ARG;
   hipStream_t s = (hipStream_t)stream;
   void* w = d_work;
+  // without a caller workspace the slots come from the stream-ordered pool
+  // (no device-wide synchronization, reused across calls)
   if (!w) {
-    hipError_t e = hipMalloc(&w, pl_ntt_host_work_bytes(lg, npoly));
+    hipError_t e = hipMallocAsync(&w, pl_ntt_host_work_bytes(lg, npoly), s);
     if (e != hipSuccess) return pl::hip_status(e);
   }
   hipError_t e = pl::ntt_host(d->d, (const pl::fr*)h_in, (pl::fr*)h_out, npoly, inverse != 0, lg_k, (pl::fr*)w, s);
-  if (!d_work) (void)hipFree(w);
+  if (!d_work) {
+    hipError_t e2 = hipFreeAsync(w, s);
+    if (e == hipSuccess) e = e2;
+  }
   return e == hipSuccess ? PL_OK : pl::hip_status(e);
 }
 
