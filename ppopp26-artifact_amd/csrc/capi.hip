@@ -325,6 +325,11 @@ int pl_div_linear(void* d_inout, uint64_t n, const void* d_c, void* d_rem, void*
   return e == hipSuccess ? PL_OK : hip_fail(e);
 }
 
+// The element-wise kernels run one thread per element in blocks of 256;
+// n <= 2^36 (2 TiB of elements, beyond any one GPU's memory) keeps the
+// block count inside 32 bits.
+constexpr uint64_t kMaxElemwise = 1ull << 36;
+
 size_t pl_vec_work_bytes(uint64_t n) {
   uint64_t a = pl::ops_scratch_elems(n), b = pl::scan_scratch_elems(n);
   return (size_t)(a > b ? a : b) * sizeof(pl::fr);
@@ -333,7 +338,7 @@ size_t pl_vec_work_bytes(uint64_t n) {
 int pl_vec_op(int op, const void* d_a, const void* d_b, void* d_out, uint64_t n, void* d_work, pl_stream_t stream) {
   if (op < PL_VEC_ADD || op > PL_VEC_FROM_MONT) return PL_ERR_ARG;
   const bool needs_b = !(op == PL_VEC_NEG || op == PL_VEC_INV || op == PL_VEC_TO_MONT || op == PL_VEC_FROM_MONT);
-  if (n && (!d_a || !d_out || (needs_b && !d_b))) return PL_ERR_ARG;
+  if (n > kMaxElemwise || (n && (!d_a || !d_out || (needs_b && !d_b)))) return PL_ERR_ARG;
   if (!aligned16(d_a) || !aligned16(d_b) || !aligned16(d_out) || !aligned16(d_work)) return PL_ERR_ARG;
   if (n == 0) return PL_OK;
   Scratch sc;
@@ -347,7 +352,8 @@ int pl_vec_op(int op, const void* d_a, const void* d_b, void* d_out, uint64_t n,
 }
 
 int pl_exp_mod(const void* d_in, void* d_out, uint64_t n, int exp, pl_stream_t stream) {
-  if (exp < 0 || (n && (!d_in || !d_out)) || !aligned16(d_in) || !aligned16(d_out)) return PL_ERR_ARG;
+  if (exp < 0 || n > kMaxElemwise || (n && (!d_in || !d_out)) || !aligned16(d_in) || !aligned16(d_out))
+    return PL_ERR_ARG;
   hipError_t e = pl::exp_mod((const pl::fr*)d_in, (pl::fr*)d_out, n, (uint32_t)exp, (hipStream_t)stream);
   return e == hipSuccess ? PL_OK : hip_fail(e);
 }
@@ -360,7 +366,7 @@ int pl_gen_sequence(uint64_t n, const void* d_x, void* d_out, pl_stream_t stream
 }
 
 int pl_repeat_to_poly(const void* d_x, void* d_out, uint64_t n, pl_stream_t stream) {
-  if ((n && (!d_x || !d_out)) || !aligned16(d_x) || !aligned16(d_out)) return PL_ERR_ARG;
+  if (n > kMaxElemwise || (n && (!d_x || !d_out)) || !aligned16(d_x) || !aligned16(d_out)) return PL_ERR_ARG;
   if (n && ranges_overlap(d_x, 32, d_out, n * 32)) return PL_ERR_ARG;
   hipError_t e = pl::repeat_to_poly((const pl::fr*)d_x, (pl::fr*)d_out, n, (hipStream_t)stream);
   return e == hipSuccess ? PL_OK : hip_fail(e);
