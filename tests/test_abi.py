@@ -83,6 +83,17 @@ def test_argument_errors_without_device():
     assert L.pl_ntt(None, None, 12, 0, None, None, None) == -1
     assert L.pl_grand_product(None, 10, None, None, None) == -1
     assert L.pl_div_linear(None, 0, None, None, None, None) == -1
+    # element-wise entry points: the checks run before any device call
+    big = (1 << 36) + 1
+    assert L.pl_exp_mod(4096, 8192, 4, -1, None) == -1  # negative exponent
+    assert L.pl_exp_mod(4096, 8192, big, 3, None) == -1  # beyond the grid bound
+    assert L.pl_exp_mod(4100, 8192, 4, 3, None) == -1  # misaligned
+    assert L.pl_vec_op(0, 4096, 8192, 12288, big, None, None) == -1
+    assert L.pl_vec_op(99, 4096, 8192, 12288, 4, None, None) == -1  # no such op
+    assert L.pl_gen_sequence((1 << 28) + 1, 4096, 8192, None) == -1
+    assert L.pl_gen_sequence(4, 4096 + 3 * 32, 4096, None) == -1  # x inside the output
+    assert L.pl_repeat_to_poly(4096 + 32, 4096, 4, None) == -1  # x inside the output
+    assert L.pl_repeat_to_poly(4096, 8192, big, None) == -1
 
 
 def test_batch_argument_errors_without_device():
