@@ -191,22 +191,27 @@ static hipError_t perm_gp(const Item& it, const ScanJob& sc, hipStream_t s) {
                             s);
 }
 
+// The transform launches go in before the scans.  Captured the other way
+// round, a kernel trace of the launched graph showed the scan chain (short
+// kernels, one after another) running almost to its end before the first
+// ragged pass started (0.46 ms of a 2.7 ms batch); in this order the scans
+// run beside the passes.
 hipError_t phase_enqueue(const PhasePlan& ph, hipStream_t main, hipStream_t side, hipEvent_t fork, hipEvent_t join) {
   hipError_t e;
   if (!ph.scans.empty()) {
     if ((e = hipEventRecord(fork, main)) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(side, fork, 0)) != hipSuccess) return e;
-    for (auto& sc : ph.scans) {
-      const Item& it = ph.items[sc.item];
-      e = sc.op == IT_EVAL_DIV ? eval_div(it.in, it.out, sc.n, it.z, it.y, sc.scratch, side) : perm_gp(it, sc, side);
-      if (e != hipSuccess) return e;
-    }
   }
   for (auto& ir : ph.irregular)
     if ((e = ntt_launch(*ir.first, ir.second, main)) != hipSuccess) return e;
   for (auto& g : ph.groups) launch_ragged(g, main);
   if ((e = hipGetLastError()) != hipSuccess) return e;
   if (!ph.scans.empty()) {
+    for (auto& sc : ph.scans) {
+      const Item& it = ph.items[sc.item];
+      e = sc.op == IT_EVAL_DIV ? eval_div(it.in, it.out, sc.n, it.z, it.y, sc.scratch, side) : perm_gp(it, sc, side);
+      if (e != hipSuccess) return e;
+    }
     if ((e = hipEventRecord(join, side)) != hipSuccess) return e;
     if ((e = hipStreamWaitEvent(main, join, 0)) != hipSuccess) return e;
   }
