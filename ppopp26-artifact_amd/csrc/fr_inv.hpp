@@ -11,8 +11,9 @@
 // divided by 2^31 modulo r (a Montgomery step: r = 1 mod 2^32).  After
 // 17 * 31 >= 2 * 255 - 1 steps b = gcd = 1 and v = y^-1 mod r.
 //
-// Plain integer code for host and device (tested on the host against the
-// oracle's inversion); one thread runs it.
+// Plain integer code for host and device (tested on the host against
+// Fermat's little theorem).  `inverse` runs on one thread; `inverse_wave`
+// runs the four updates of each round on four lanes of a wave at once.
 #pragma once
 #include <stdint.h>
 #if defined(__HIPCC__)
@@ -140,6 +141,76 @@ PL_HD void mod_update(uint32_t* out, const uint32_t* u, int64_t f, const uint32_
   for (int i = 0; i < 8; i++) out[i] = x[i];
 }
 
+// x <- -x mod r for x in [0, r)
+PL_HD void negate_mod_r(uint32_t* x) {
+  uint32_t nz = 0;
+  for (int i = 0; i < 8; i++) nz |= x[i];
+  int64_t bw = 0;
+  for (int i = 0; i < 8; i++) {
+    int64_t s = (int64_t)(nz ? kR[i] : 0) - x[i] + bw;
+    x[i] = (uint32_t)s;
+    bw = s >> 32;  // 0 or -1
+  }
+}
+
+// the 31 binary-GCD steps of one outer round on the 64-bit approximations of
+// a and b; m = (f0, g0, f1, g1)
+PL_HD void inner_steps(const uint32_t* a, const uint32_t* b, int64_t* m) {
+  uint32_t ab[8];
+  for (int i = 0; i < 8; i++) ab[i] = a[i] | b[i];
+  int n = bitlen8(ab);
+  if (n < 64) n = 64;
+  uint64_t xa = (uint64_t)(a[0] & 0x7fffffffu) | (bits33(a, n - 33) << 31);
+  uint64_t xb = (uint64_t)(b[0] & 0x7fffffffu) | (bits33(b, n - 33) << 31);
+  int64_t f0 = 1, g0 = 0, f1 = 0, g1 = 1;
+  for (int j = 0; j < 31; j++) {
+    if (xa & 1) {
+      if (xa < xb) {
+        uint64_t tx = xa; xa = xb; xb = tx;
+        int64_t tf = f0; f0 = f1; f1 = tf;
+        int64_t tg = g0; g0 = g1; g1 = tg;
+      }
+      xa = (xa - xb) >> 1;
+      f0 -= f1;
+      g0 -= g1;
+    } else {
+      xa >>= 1;
+    }
+    f1 <<= 1;
+    g1 <<= 1;
+  }
+  m[0] = f0;
+  m[1] = g0;
+  m[2] = f1;
+  m[3] = g1;
+}
+
+// One of the four full-width updates of an outer round, by role:
+//   0: |f0 a + g0 b| / 2^31 -> res, *neg = its sign was negative
+//   1: |f1 a + g1 b| / 2^31 -> res, *neg likewise
+//   2: (f0 u + g0 v) / 2^31 mod r -> res
+//   3: (f1 u + g1 v) / 2^31 mod r -> res
+// Role 2 (3) then takes role 0's (1's) sign: res <- -res mod r if that row
+// was negated (fix by negate_mod_r).  The four are independent, so the wave
+// form runs them on four lanes at once.
+PL_HD void round_role(int role, const uint32_t* a, const uint32_t* b, const uint32_t* u, const uint32_t* v,
+                      const int64_t* m, uint32_t* res, bool* neg) {
+  const int64_t f = m[(role & 1) * 2], g = m[(role & 1) * 2 + 1];
+  *neg = false;
+  if (role < 2) {
+    S10 t = {};
+    add_mul(t, a, f);
+    add_mul(t, b, g);
+    if (s10_neg(t)) {
+      s10_negate(t);
+      *neg = true;
+    }
+    s10_shr31_to8(t, res);
+  } else {
+    mod_update(res, u, f, v, g);
+  }
+}
+
 // y^-1 mod r for 0 < y < r (y as 8 little-endian limbs); 0 for y = 0
 PL_HD void inverse(const uint32_t* y, uint32_t* out) {
   uint32_t a[8], b[8], u[8], v[8];
@@ -157,56 +228,62 @@ PL_HD void inverse(const uint32_t* y, uint32_t* out) {
   }
   u[0] = 1;
   for (int outer = 0; outer < 17; outer++) {
-    uint32_t ab[8];
-    for (int i = 0; i < 8; i++) ab[i] = a[i] | b[i];
-    int n = bitlen8(ab);
-    if (n < 64) n = 64;
-    uint64_t xa = (uint64_t)(a[0] & 0x7fffffffu) | (bits33(a, n - 33) << 31);
-    uint64_t xb = (uint64_t)(b[0] & 0x7fffffffu) | (bits33(b, n - 33) << 31);
-    int64_t f0 = 1, g0 = 0, f1 = 0, g1 = 1;
-    for (int j = 0; j < 31; j++) {
-      if (xa & 1) {
-        if (xa < xb) {
-          uint64_t tx = xa; xa = xb; xb = tx;
-          int64_t tf = f0; f0 = f1; f1 = tf;
-          int64_t tg = g0; g0 = g1; g1 = tg;
-        }
-        xa = (xa - xb) >> 1;
-        f0 -= f1;
-        g0 -= g1;
-      } else {
-        xa >>= 1;
-      }
-      f1 <<= 1;
-      g1 <<= 1;
-    }
-    S10 na = {}, nb = {};
-    add_mul(na, a, f0);
-    add_mul(na, b, g0);
-    add_mul(nb, a, f1);
-    add_mul(nb, b, g1);
-    if (s10_neg(na)) {
-      s10_negate(na);
-      f0 = -f0;
-      g0 = -g0;
-    }
-    if (s10_neg(nb)) {
-      s10_negate(nb);
-      f1 = -f1;
-      g1 = -g1;
-    }
-    s10_shr31_to8(na, a);
-    s10_shr31_to8(nb, b);
-    uint32_t nu[8], nv[8];
-    mod_update(nu, u, f0, v, g0);
-    mod_update(nv, u, f1, v, g1);
+    int64_t m[4];
+    inner_steps(a, b, m);
+    uint32_t res[4][8];
+    bool neg[4];
+    for (int role = 0; role < 4; role++) round_role(role, a, b, u, v, m, res[role], &neg[role]);
+    if (neg[0]) negate_mod_r(res[2]);
+    if (neg[1]) negate_mod_r(res[3]);
     for (int i = 0; i < 8; i++) {
-      u[i] = nu[i];
-      v[i] = nv[i];
+      a[i] = res[0][i];
+      b[i] = res[1][i];
+      u[i] = res[2][i];
+      v[i] = res[3][i];
     }
   }
   for (int i = 0; i < 8; i++) out[i] = v[i];
 }
+
+#if defined(__HIPCC__)
+// The same inverse by one wave: every lane runs the (uniform) inner steps,
+// lane l runs role l & 3 of each round's update, and the four results are
+// shared by readlane.  Call from all lanes of a wave, converged; every lane
+// returns y^-1.
+__device__ __forceinline__ void inverse_wave(const uint32_t* y, uint32_t* out) {
+  const int role = (int)(__lane_id() & 3);
+  uint32_t a[8], b[8], u[8], v[8];
+  uint32_t nz = 0;
+  for (int i = 0; i < 8; i++) {
+    a[i] = (uint32_t)__builtin_amdgcn_readfirstlane((int)y[i]);
+    b[i] = kR[i];
+    u[i] = 0;
+    v[i] = 0;
+    nz |= a[i];
+  }
+  if (!nz) {
+    for (int i = 0; i < 8; i++) out[i] = 0;
+    return;
+  }
+  u[0] = 1;
+  for (int outer = 0; outer < 17; outer++) {
+    int64_t m[4];
+    inner_steps(a, b, m);
+    uint32_t res[8];
+    bool neg;
+    round_role(role, a, b, u, v, m, res, &neg);
+    const int n0 = __builtin_amdgcn_readlane((int)neg, 0), n1 = __builtin_amdgcn_readlane((int)neg, 1);
+    if ((role == 2 && n0) || (role == 3 && n1)) negate_mod_r(res);
+    for (int i = 0; i < 8; i++) {
+      a[i] = (uint32_t)__builtin_amdgcn_readlane((int)res[i], 0);
+      b[i] = (uint32_t)__builtin_amdgcn_readlane((int)res[i], 1);
+      u[i] = (uint32_t)__builtin_amdgcn_readlane((int)res[i], 2);
+      v[i] = (uint32_t)__builtin_amdgcn_readlane((int)res[i], 3);
+    }
+  }
+  for (int i = 0; i < 8; i++) out[i] = v[i];
+}
+#endif
 
 #undef PL_HD
 

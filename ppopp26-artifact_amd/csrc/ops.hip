@@ -46,12 +46,15 @@ __device__ __forceinline__ fr const_fr(uint32_t (*f)(int)) {
 // ---- single inversion ----------------------------------------------------
 // Montgomery-form inverse of a (a = x R -> x^-1 R); zero maps to zero.  The
 // integer inverse of the canonical value x R comes from the optimized binary
-// GCD in fr_inv.hpp (17 rounds of 31 steps, no data-dependent trip count).
-__device__ fr fr_inv_single(const fr& a_in) {
+// GCD in fr_inv.hpp (17 rounds of 31 steps, no data-dependent trip count),
+// run by a whole wave: the four full-width updates of each round go to four
+// lanes at once (70 -> ~35 us against one lane).  Call from all lanes of a
+// wave, converged, with a the same on every lane (lane 0's is used); every
+// lane returns the inverse.
+__device__ fr fr_inv_wave(const fr& a_in) {
   const fr a = fr_reduce_once(a_in);
-  if (fr_is_zero(a)) return fr_zero();
   fr r;
-  inv::inverse(a.l, r.l);
+  inv::inverse_wave(a.l, r.l);  // zero maps to zero
   // r = (xR)^-1 = x^-1 R^-1 ; times R^3 under Montgomery: x^-1 R
   return fr_mul(r, const_fr(rrr_limb));
 }
@@ -356,15 +359,16 @@ __global__ void __launch_bounds__(kOpThreads) inv_down_kernel(const fr* in, cons
 
 // single inversion at the top of the batch: *p <- the R'-view inverse of *p
 // times 2^-5 (level-0 outputs with num) or 2^-10 (without), i.e.
-// fr_inv_single(W) times 32 or 1 (see the batch-inversion notes above)
+// fr_inv_wave(W) times 32 or 1 (see the batch-inversion notes above); one
+// block of one wave
 __global__ void inv_single_kernel(fr* p, int with_num) {
-  if (threadIdx.x == 0 && blockIdx.x == 0) {
-    fr z = fr_inv_single(fr_load(p));
+  if (threadIdx.x < 64 && blockIdx.x == 0) {
+    fr z = fr_inv_wave(fr_load(p));
     if (with_num) {
 #pragma unroll
       for (int d = 0; d < 5; d++) z = fr_add(z, z);
     }
-    fr_store(p, z);
+    if (threadIdx.x == 0) fr_store(p, z);
   }
 }
 
@@ -578,7 +582,7 @@ __global__ void __launch_bounds__(256) perm_terms_kernel(RatioArgs a, fr* rn, fr
 // Words are read in the R'-view (the batch-inversion notes above): num_j and
 // den_j are stored words S(x), i.e. R'-view words of x / 32; the 32s cancel
 // between the two products and the total but for one, which the inverse
-// absorbs (32 fr_inv_single(W) is the R'-view word of 1 / (32 W/R')), so level
+// absorbs (32 fr_inv_wave(W) is the R'-view word of 1 / (32 W/R')), so level
 // 0 yields the stored words S(z_i).
 
 // Up-sweep over one level: thread products of num and den over each
@@ -623,9 +627,10 @@ __global__ void __launch_bounds__(kOpThreads) gp_up_kernel(const fr* nv, const f
   }
 }
 
-// 32 fr_inv_single(total): the R'-view word of 1 / (32 total / R')
+// 32 fr_inv_wave(total): the R'-view word of 1 / (32 total / R'); by a whole
+// wave, as fr_inv_wave
 __device__ __forceinline__ fr gp_inverse(const fr& total) {
-  fr z = fr_inv_single(total);
+  fr z = fr_inv_wave(total);
 #pragma unroll
   for (int d = 0; d < 5; d++) z = fr_add(z, z);
   return z;
@@ -703,8 +708,9 @@ __device__ __forceinline__ void gp_down_body(const fr* nv, const fr* dv, const f
   block_scan2_short(pre, suf, l, l2, one);
   fr cw;
   if constexpr (TOP) {
-    if (t == 0) {
-      // total den product = suf_0 x (thread 0's product)
+    if (t < 64) {
+      // total den product = suf_0 x (thread 0's product): lane 0's value
+      // (wave 0 runs it whole for the wave-wide inversion)
       v9 a;
       uint32_t b[9];
       pbd_unpack(suf.l, a.l);
@@ -712,13 +718,15 @@ __device__ __forceinline__ void gp_down_body(const fr* nv, const fr* dv, const f
       const fr total = v9_canonical(v9_mul<false, false, false>(a, b));
       fr z;
       if (tot) {
-        fr_store(tot, total);  // inverted by gp_inverse beside the next sweep
+        if (t == 0) fr_store(tot, total);  // inverted by gp_inverse beside the next sweep
         z = rp_one();
       } else {
         z = gp_inverse(total);
       }
+      if (t == 0) {
 #pragma unroll
-      for (int i = 0; i < 8; i++) cinv[i] = z.l[i];
+        for (int i = 0; i < 8; i++) cinv[i] = z.l[i];
+      }
     }
     __syncthreads();
 #pragma unroll
@@ -754,7 +762,7 @@ __device__ __forceinline__ void gp_down_body(const fr* nv, const fr* dv, const f
 
 // bulk levels: three waves per SIMD (the local den suffixes stay in registers)
 // inv != nullptr: one more block than the level needs; block 0 (dispatched
-// first) has thread 0 write gp_inverse(*tot) to inv while the other blocks
+// first) has its wave 0 write gp_inverse(*tot) to inv while the other blocks
 // sweep (the inverse then multiplies the run carries, gp_run_down_kernel)
 template <bool INCL>
 __global__ void __launch_bounds__(kOpThreads) __attribute__((amdgpu_waves_per_eu(3)))
@@ -763,7 +771,10 @@ gp_down_kernel(const fr* nv, const fr* dv, const fr* tn, const fr* td, uint64_t 
   uint32_t blk = blockIdx.x;
   if (inv) {
     if (blk == 0) {
-      if (threadIdx.x == 0) fr_store(inv, gp_inverse(fr_load(tot)));
+      if (threadIdx.x < 64) {
+        const fr z = gp_inverse(fr_load(tot));
+        if (threadIdx.x == 0) fr_store(inv, z);
+      }
       return;
     }
     blk--;
