@@ -98,6 +98,37 @@ PL_HD void s10_shr31_to8(const S10& a, uint32_t* x) {
   for (int i = 0; i < 8; i++) x[i] = (a.w[i] >> 31) | (a.w[i + 1] << 1);
 }
 
+// x (nine limbs, x < 5 r) <- x mod r.  qe = floor(X / (r_hi + 1)) from the
+// top 64 bits X = x >> 224 (r_hi = r >> 224) is floor(x / r) or one less:
+// X / (r_hi + 1) < x / r < (X + 1) / r_hi, and the two ends differ by
+// (X + r_hi + 1) / (r_hi (r_hi + 1)) < 6 / r_hi.  Subtract qe r, then r once
+// more if x is still >= r.
+PL_HD void reduce_below_5r(uint32_t* x) {
+  const uint64_t X = ((uint64_t)x[8] << 32) | x[7], rh1 = (uint64_t)kR[7] + 1;
+  const uint32_t qe = (uint32_t)(X >= rh1) + (uint32_t)(X >= 2 * rh1) + (uint32_t)(X >= 3 * rh1) +
+                      (uint32_t)(X >= 4 * rh1);
+  {
+    int64_t bw = 0;
+    uint64_t c = 0;
+    for (int i = 0; i < 9; i++) {
+      const uint64_t qr = (i < 8 ? (uint64_t)qe * kR[i] : 0) + c;  // qe r, limb i, with its carry
+      c = qr >> 32;
+      int64_t s = (int64_t)x[i] - (int64_t)(uint32_t)qr + bw;
+      x[i] = (uint32_t)s;
+      bw = s >> 32;  // 0 or -1
+    }
+  }
+  uint32_t d[9];
+  int64_t bw = 0;
+  for (int i = 0; i < 9; i++) {
+    int64_t s = (int64_t)x[i] - (i < 8 ? kR[i] : 0) + bw;
+    d[i] = (uint32_t)s;
+    bw = s >> 32;  // 0 or -1
+  }
+  if (bw == 0)
+    for (int i = 0; i < 9; i++) x[i] = d[i];
+}
+
 // u' = (u f0 + v g0) / 2^31 mod r, with u, v < r
 PL_HD void mod_update(uint32_t* out, const uint32_t* u, int64_t f, const uint32_t* v, int64_t g) {
   S10 t = {};
@@ -126,18 +157,7 @@ PL_HD void mod_update(uint32_t* out, const uint32_t* u, int64_t f, const uint32_
   uint32_t x[9];
   for (int i = 0; i < 8; i++) x[i] = (t.w[i] >> 31) | (t.w[i + 1] << 1);
   x[8] = (t.w[8] >> 31) | (t.w[9] << 1);
-  // x < 5 r: subtract r while x >= r (at most four times)
-  for (int it = 0; it < 4; it++) {
-    uint32_t d[9];
-    int64_t bw = 0;
-    for (int i = 0; i < 9; i++) {
-      int64_t s = (int64_t)x[i] - (i < 8 ? kR[i] : 0) + bw;
-      d[i] = (uint32_t)s;
-      bw = s >> 32;  // 0 or -1
-    }
-    if (bw == 0)
-      for (int i = 0; i < 9; i++) x[i] = d[i];
-  }
+  reduce_below_5r(x);  // x < 5 r
   for (int i = 0; i < 8; i++) out[i] = x[i];
 }
 

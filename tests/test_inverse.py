@@ -62,3 +62,79 @@ def test_inverse_matches_fermat(inv_host):
         if v != pow(y, R - 2, R) or (v * y) % R != 1:
             bad.append(y)
     assert not bad, "first failing inputs: %s" % [hex(b) for b in bad[:3]]
+
+
+@pytest.fixture(scope="module")
+def mod_update_host(tmp_path_factory):
+    d = tmp_path_factory.mktemp("modupd")
+    src = d / "mod_update_host.cpp"
+    src.write_text(
+        '#include "%s"\n'
+        'extern "C" void mod_update_host(const uint32_t* u, int64_t f, const uint32_t* v, int64_t g,\n'
+        '                                uint32_t* out) { pl::inv::mod_update(out, u, f, v, g); }\n'
+        % os.path.abspath(HDR)
+    )
+    lib = d / "libmodupd.so"
+    subprocess.check_call(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", str(src), "-o", str(lib)])
+    so = ctypes.CDLL(str(lib))
+    so.mod_update_host.argtypes = [ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p, ctypes.c_int64, ctypes.c_void_p]
+    so.mod_update_host.restype = None
+
+    def call(u, f, v, g):
+        a = (ctypes.c_uint32 * 8)(*[(u >> (32 * i)) & 0xFFFFFFFF for i in range(8)])
+        b = (ctypes.c_uint32 * 8)(*[(v >> (32 * i)) & 0xFFFFFFFF for i in range(8)])
+        o = (ctypes.c_uint32 * 8)()
+        so.mod_update_host(a, f, b, g, o)
+        return sum(int(o[i]) << (32 * i) for i in range(8))
+
+    return call
+
+
+def test_mod_update_is_the_exact_reduced_combination(mod_update_host):
+    # (u f + v g) / 2^31 mod r, fully reduced, over u, v < r and |f| + |g| <= 2^31
+    # (the bound the 31-step rounds keep); the extremes drive the quotient
+    # estimate of the final reduction to both of its outcomes
+    inv31 = pow(1 << 31, R - 2, R)
+    rng = random.Random(4711)
+    vals = [0, 1, 2, R - 1, R - 2, (R - 1) // 2, 1 << 254, R - (1 << 200)]
+    coef = [(1 << 31, 0), (-(1 << 31), 0), (0, 1 << 31), (0, -(1 << 31)), ((1 << 30), (1 << 30)),
+            (-(1 << 30), -(1 << 30)), ((1 << 30), -(1 << 30)), (1, -1), (0, 0), ((1 << 31) - 1, 1)]
+    cases = [(u, f, v, g) for u in vals for v in vals for f, g in coef]
+    for _ in range(3000):
+        t = rng.randrange(0, (1 << 31) + 1)
+        f = rng.randrange(-t, t + 1)
+        g = (t - abs(f)) * rng.choice([-1, 1])
+        cases.append((rng.randrange(R), f, rng.randrange(R), g))
+    bad = []
+    for u, f, v, g in cases:
+        got = mod_update_host(u, f, v, g)
+        if got != (u * f + v * g) * inv31 % R:
+            bad.append((hex(u), f, hex(v), g))
+    assert not bad, "first failing cases: %s" % bad[:3]
+
+
+def test_reduce_below_5r_at_the_boundaries(tmp_path):
+    # x = k r + e for k < 5: e = 0, 1, r - 1 and e just below the width where
+    # the top-bits estimate of k falls one short (its one correction step)
+    src = tmp_path / "red_host.cpp"
+    src.write_text(
+        '#include "%s"\n'
+        'extern "C" void red_host(uint32_t* x) { pl::inv::reduce_below_5r(x); }\n' % os.path.abspath(HDR)
+    )
+    lib = tmp_path / "libred.so"
+    subprocess.check_call(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", str(src), "-o", str(lib)])
+    so = ctypes.CDLL(str(lib))
+    so.red_host.argtypes = [ctypes.c_void_p]
+    so.red_host.restype = None
+    rng = random.Random(99)
+    es = [0, 1, 2, R - 1, R - 2, 1 << 223, (1 << 224) - 1, 1 << 224, 3 << 223, 1 << 230]
+    es += [rng.randrange(1 << 226) for _ in range(50)] + [rng.randrange(R) for _ in range(50)]
+    for k in range(5):
+        for e in es:
+            x = k * R + e
+            if x >= 5 * R:
+                continue
+            a = (ctypes.c_uint32 * 9)(*[(x >> (32 * i)) & 0xFFFFFFFF for i in range(9)])
+            so.red_host(a)
+            got = sum(int(a[i]) << (32 * i) for i in range(8))
+            assert got == x % R, (k, hex(e))
