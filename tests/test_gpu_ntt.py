@@ -145,6 +145,12 @@ def test_ntt_roundtrip_2p26(pl):
     w = po.root_of_unity(lg)
     for k in (1, 12345, n - 1):
         np.testing.assert_array_equal(pl.to_host(y[k:k + 1]), po.eval_(x, po.pow_(w, k)))
+    # every output weighs in: sum_k X[k] w^(-jk) = N x_j (the inverse DFT at
+    # one point, by the device evaluation, which the oracle pins up to 2^24)
+    nn = po.from_u64(n)
+    for j in (0, 1, 777, n - 5):
+        pt = pl.to_dev(po.inv(po.pow_(w, j)))
+        np.testing.assert_array_equal(pl.to_host(pl.evaluate(y, pt)), po.mul(nn, x[j:j + 1]))
 
 
 def test_ntt_roundtrip_2p28(pl):
@@ -164,6 +170,10 @@ def test_ntt_roundtrip_2p28(pl):
     for k in (1, n // 2 + 3):
         pt = pl.to_dev(po.pow_(w, k))
         np.testing.assert_array_equal(pl.to_host(pl.evaluate(xd, pt)), pl.to_host(y[k:k + 1]))
+    # the inverse DFT at one point weighs every output: sum_k X[k] w^(-jk) = N x_j
+    for j in (0, 5):
+        pt = pl.to_dev(po.inv(po.pow_(w, j)))
+        np.testing.assert_array_equal(pl.to_host(pl.evaluate(y, pt)), po.mul(po.from_u64(n), pl.to_host(xd[j:j + 1])))
     del xd, y
     torch.cuda.empty_cache()
 
