@@ -154,11 +154,18 @@ int pl_ntt_segmented(void* d_inout, int lg_n, int lg_i, int lg_k, const void* d_
  * arrive (2^lg_k groups, each a strided 2-D copy), its last pass on line
  * groups whose natural-order outputs leave at once, and the next
  * polynomial's copy-in overlaps the current one's transform and copy-out.
- * Blocks until h_out holds every result.  d_work: NULL or
+ * lg_k < 0: the count in use for this size (pl_ntt_host_groups).  Blocks
+ * until h_out holds every result.  d_work: NULL or
  * pl_ntt_host_work_bytes(lg, npoly) bytes. */
 size_t pl_ntt_host_work_bytes(int lg, uint32_t npoly);
 int pl_ntt_host(const void* h_in, void* h_out, int lg, uint32_t npoly, int inverse, const pl_domain* d, int lg_k,
                 void* d_work, pl_stream_t stream);
+/* The copy-group count (lg_k) per size for pl_ntt_host's lg_k < 0 [the
+ * reference's segment count k of our_ntt, swept by hand with NTT
+ * --bench-sweep, ntt.cu:1494-1500]: the autotuner's choice (PL_TUNE_HOST) or
+ * a cache file's, else 2.  _set: lg_k < 0 restores the default; at most 20. */
+int pl_ntt_host_groups(int lg);
+int pl_ntt_host_groups_set(int lg, int lg_k);
 
 /* ---- coset low-degree extension ---------------------------------------
  * [our_lde, lde.cu:1267-1365 (lde_spread_powers_kernel lde.cu:623-648,
@@ -260,9 +267,14 @@ int pl_div_linear(void* d_inout, uint64_t n, const void* d_c, void* d_rem, void*
  * chosen splits apply to domains created afterwards (process-wide).
  * pl_plan_get reports the split in use (P = 0 below lg 4: one tiny kernel);
  * pl_plan_set sets one (P = 0 restores the default); pl_plan_load applies a
- * cache file and returns the number of sizes read; pl_plan_candidates writes
- * up to max_cand splits (8 ints each, zero-padded) and returns their count. */
-enum pl_tune_flags { PL_TUNE_REMEASURE = 1 };
+ * cache file and returns the number of entries it applied (splits and
+ * copy-group counts); pl_plan_candidates writes
+ * up to max_cand splits (8 ints each, zero-padded) and returns their count.
+ * PL_TUNE_HOST: also time pl_ntt_host (one polynomial, pinned host buffers)
+ * for each lg in [max(lg_lo, 12), min(lg_hi, 26)] over lg_k = 0..6 and keep
+ * the fastest as that size's copy-group count ("h <lg> <lg_k> <ms>" lines in
+ * the cache; pl_plan_load applies them too). */
+enum pl_tune_flags { PL_TUNE_REMEASURE = 1, PL_TUNE_HOST = 2 };
 int pl_autotune(int lg_lo, int lg_hi, const char* cache_path, int reps, int flags);
 int pl_plan_get(int lg, int* m, int* P);
 int pl_plan_set(int lg, const int* m, int P);

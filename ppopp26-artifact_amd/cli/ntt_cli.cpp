@@ -185,7 +185,7 @@ static void usage(const char* exe) {
             << "  " << exe << " --check-split <lg_N>\n"
             << "  " << exe << " --check-baseline <lg_N>\n"
             << "  " << exe << " --bench-sweep <lg_N> [reps]\n"
-            << "  " << exe << " --autotune <lg_lo> <lg_hi> [cache_file]\n";
+            << "  " << exe << " --autotune <lg_lo> <lg_hi> [cache_file [host]]\n";
 }
 
 int main(int argc, char** argv) {
@@ -207,19 +207,23 @@ int main(int argc, char** argv) {
       return run_bench(c, c.ours_lg_k, c.ours_lg_k, c.reps, false);
     }
     if (cmd == "--autotune") {
-      // pass-split autotuner (replaces the manual segment sweep)
+      // pass-split autotuner, and with "host" the copy groups (replaces the
+      // manual segment sweep)
       if (argc < 4) {
         usage(argv[0]);
         return 1;
       }
       const int lo = std::atoi(argv[2]), hi = std::atoi(argv[3]);
       const char* cache = argc >= 5 ? argv[4] : nullptr;
-      cli::check(pl_autotune(lo, hi, cache, 5, 0), "pl_autotune");
+      // "host": also the copy groups of our_ntt's host-to-host scope
+      const bool host = argc >= 6 && std::string(argv[5]) == "host";
+      cli::check(pl_autotune(lo, hi, cache, 5, host ? PL_TUNE_HOST : 0), "pl_autotune");
       for (int lg = lo; lg <= hi; lg++) {
         int m[8], P = 0;
         cli::check(pl_plan_get(lg, m, &P), "pl_plan_get");
         std::cout << "[autotune] lg_N = " << lg << ": " << P << " passes of";
         for (int i = 0; i < P; i++) std::cout << " 2^" << m[i];
+        if (host && lg >= 12 && lg <= 26) std::cout << "; host-to-host: 2^" << pl_ntt_host_groups(lg) << " copy groups";
         std::cout << std::endl;
       }
       return 0;

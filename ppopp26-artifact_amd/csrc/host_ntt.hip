@@ -18,11 +18,31 @@
 // ntt.cu:801-859; run_second_ntt_pass_async_scatter, ntt.cu:861-977) has
 // the same overlap structure over chunks of a host-resident vector.
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "ntt_pass.hpp"
 
 namespace pl {
+
+// copy-group count (lg_k) per size for callers that pass lg_k < 0: set by
+// the autotuner or a cache file, else kHostLgKDefault (process-wide)
+static std::mutex g_hg_mu;
+static int g_host_lg_k[29] = {-1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1,
+                              -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1, -1};
+
+int host_groups_get(int lg) {
+  if (lg < 1 || lg > 28) return kHostLgKDefault;
+  std::lock_guard<std::mutex> g(g_hg_mu);
+  return g_host_lg_k[lg] >= 0 ? g_host_lg_k[lg] : kHostLgKDefault;
+}
+
+bool host_groups_set(int lg, int lg_k) {
+  if (lg < 1 || lg > 28 || lg_k > kHostLgKMax) return false;
+  std::lock_guard<std::mutex> g(g_hg_mu);
+  g_host_lg_k[lg] = lg_k < 0 ? -1 : lg_k;
+  return true;
+}
 
 namespace {
 
@@ -150,8 +170,9 @@ size_t pl_ntt_host_work_bytes(int lg, uint32_t npoly) {
 // [our_ntt, ntt.cu:979-1037]
 int pl_ntt_host(const void* h_in, void* h_out, int lg, uint32_t npoly, int inverse, const pl_domain* d, int lg_k,
                 void* d_work, pl_stream_t stream) {
-  if (!h_in || !h_out || !d || npoly == 0 || d->d.lg != lg || lg_k < 0) return PL_ERR_ARG;
+  if (!h_in || !h_out || !d || npoly == 0 || d->d.lg != lg) return PL_ERR_ARG;
   if (((uintptr_t)d_work & 15) != 0) return PL_ERR_ARG;
+  if (lg_k < 0) lg_k = pl::host_groups_get(lg);  // tuned or default
   hipStream_t s = (hipStream_t)stream;
   void* w = d_work;
   // without a caller workspace the slots come from the stream-ordered pool
@@ -167,5 +188,12 @@ int pl_ntt_host(const void* h_in, void* h_out, int lg, uint32_t npoly, int inver
   }
   return e == hipSuccess ? PL_OK : pl::hip_status(e);
 }
+
+int pl_ntt_host_groups(int lg) {
+  if (lg < 1 || lg > 28) return PL_ERR_ARG;
+  return pl::host_groups_get(lg);
+}
+
+int pl_ntt_host_groups_set(int lg, int lg_k) { return pl::host_groups_set(lg, lg_k) ? PL_OK : PL_ERR_ARG; }
 
 }  // extern "C"

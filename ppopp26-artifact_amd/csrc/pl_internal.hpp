@@ -95,6 +95,11 @@ hipError_t ntt_launch_one(const Domain& d, const NttCall& c, int p, const fr* in
 hipError_t ntt_launch_part(const Domain& d, const NttCall& c, int p, const fr* in, fr* out, uint64_t in_off,
                            uint64_t out_off, uint64_t nlines, hipStream_t s);
 uint64_t last_pass_row(const NttPlan& P, uint64_t lp);
+// host-to-host NTT copy groups (2^lg_k) per size: tuned, else the default
+constexpr int kHostLgKDefault = 2;
+constexpr int kHostLgKMax = 20;
+int host_groups_get(int lg);
+bool host_groups_set(int lg, int lg_k);  // lg_k < 0: back to the default
 hipError_t domain_build(Domain& d, int lg, hipStream_t s);
 // maps a HIP error to a pl_status and records its text for pl_last_hip_error
 int hip_status(hipError_t e);

@@ -58,6 +58,8 @@ SIGNATURES = {
     "pl_ntt_segmented": (_i, [_vp, _i, _i, _i, _vp, _vp, _vp]),
     "pl_ntt_host_work_bytes": (ctypes.c_size_t, [_i, _u32]),
     "pl_ntt_host": (_i, [_vp, _vp, _i, _u32, _i, _vp, _i, _vp, _vp]),
+    "pl_ntt_host_groups": (_i, [_i]),
+    "pl_ntt_host_groups_set": (_i, [_i, _i]),
     "pl_lde_domain_create": (_i, [_i, _i, _vp, ctypes.POINTER(_vp)]),
     "pl_lde_domain_destroy": (_i, [_vp]),
     "pl_lde_domain_bytes": (ctypes.c_size_t, [_vp]),
@@ -516,9 +518,17 @@ def probe_mad_rate(stream=None):
     return v.value
 
 
-def autotune(lg_lo, lg_hi, cache_path=None, reps=5, remeasure=False):
+def autotune(lg_lo, lg_hi, cache_path=None, reps=5, remeasure=False, host=False):
+    """pass splits of each size (and, host=True, the host-to-host copy groups)"""
     check("pl_autotune", lib().pl_autotune(lg_lo, lg_hi, cache_path.encode() if cache_path else None, reps,
-                                           1 if remeasure else 0))
+                                           (1 if remeasure else 0) | (2 if host else 0)))
+
+
+def host_groups(lg):
+    """lg_k that pl_ntt_host uses for lg_k < 0 (tuned, else the default)"""
+    v = lib().pl_ntt_host_groups(lg)
+    check("pl_ntt_host_groups", min(v, 0))
+    return v
 
 
 def plan_get(lg):
@@ -645,9 +655,10 @@ def ntt_segmented(x, lg_i, lg_k, params_buf, stream=None):
     return x
 
 
-def ntt_host(h, dom, lg_k=4, npoly=1, inverse=False, out=None, work=None, stream=None):
+def ntt_host(h, dom, lg_k=-1, npoly=1, inverse=False, out=None, work=None, stream=None):
     """Host-to-host NTT of npoly polynomials held in a host tensor (pinned for
     overlap), shape (npoly * 2^lg, 4) int64; returns the output host tensor.
+    lg_k: 2^lg_k copy groups, or -1 for the size's tuned (else default) count.
     work: None (the call allocates its device slots) or a device tensor of at
     least pl_ntt_host_work_bytes(lg, npoly) bytes."""
     import torch

@@ -178,3 +178,27 @@ def test_host_to_host_ntt_vs_oracle(pl, lg, npoly, lg_k, inverse):
     # in place in the host buffer, as our_ntt leaves its result in `input`
     pl.ntt_host(h, dom, lg_k=lg_k, npoly=npoly, inverse=inverse, out=h)
     assert_fr_equal(h.numpy().view(np.uint64), got, "host ntt in place")
+
+
+def test_host_copy_groups_tuned(pl, tmp_path):
+    """the autotuner's host pass (PL_TUNE_HOST) times pl_ntt_host over the
+    copy-group counts, caches "h" lines and applies them; lg_k = -1 then runs
+    with the tuned count, against the oracle"""
+    import torch
+    cache = tmp_path / "plans.txt"
+    # splits already cached, so only the host-to-host copy groups are timed
+    splits = ["%d %d %s 0.0" % (lg, len(pl.plan_get(lg)), " ".join(map(str, pl.plan_get(lg)))) for lg in (14, 15)]
+    cache.write_text("\n".join(splits) + "\n")
+    print("autotune host 14..15", flush=True)
+    pl.autotune(14, 15, str(cache), reps=2, host=True)
+    lines = [l.split() for l in cache.read_text().splitlines() if l.startswith("h ")]
+    got = {int(l[1]): int(l[2]) for l in lines}
+    assert sorted(got) == [14, 15] and all(0 <= k <= 6 for k in got.values())
+    assert pl.host_groups(14) == got[14] and pl.host_groups(15) == got[15]
+    lg = 15
+    x = po.uniform(1 << lg, po.seed(po.OP_NTT, lg, 900))
+    h = torch.from_numpy(x.view(np.int64)).pin_memory()
+    out = pl.ntt_host(h, pl.Domain(lg))  # lg_k = -1: the tuned count
+    np.testing.assert_array_equal(out.numpy().view(np.uint64), po.ntt(x, lg))
+    for g in (14, 15):
+        pl.lib().pl_ntt_host_groups_set(g, -1)

@@ -77,3 +77,23 @@ def test_autotune_argument_errors(pl):
     except ImportError:
         pass
     assert L.pl_autotune(12, 13, None, 3, 0) == -5
+
+
+def test_host_groups_cache_and_set(pl, tmp_path):
+    # copy-group counts of the host-to-host NTT: default 2, set/restore,
+    # and "h <lg> <lg_k> <ms>" cache lines (malformed or out-of-range ones ignored)
+    L = pl.lib()
+    assert L.pl_ntt_host_groups(24) == 2
+    assert L.pl_ntt_host_groups(0) == -1 and L.pl_ntt_host_groups(29) == -1
+    assert L.pl_ntt_host_groups_set(24, 5) == 0 and L.pl_ntt_host_groups(24) == 5
+    assert L.pl_ntt_host_groups_set(24, -1) == 0 and L.pl_ntt_host_groups(24) == 2
+    assert L.pl_ntt_host_groups_set(24, 21) == -1 and L.pl_ntt_host_groups_set(30, 1) == -1
+    p = tmp_path / "plans.txt"
+    p.write_text("16 3 6 5 5 0.02\n# host\nh 20 3 4.1\nh 22 0 9.9\nh 23 99 1.0\nh x 1\nh 24\n")
+    assert L.pl_plan_load(str(p).encode()) == 3  # one split, two copy-group counts
+    assert pl.host_groups(20) == 3 and pl.host_groups(22) == 0
+    assert pl.host_groups(23) == 2 and pl.host_groups(24) == 2
+    for lg in (20, 22):
+        L.pl_ntt_host_groups_set(lg, -1)
+    L.pl_plan_set(16, None, 0)
+    assert L.pl_autotune(12, 13, None, 3, 8) == -1  # unknown flag
