@@ -86,13 +86,6 @@ __device__ __forceinline__ bool fr_is_zero(const fr& a) {
   return o == 0;
 }
 
-__device__ __forceinline__ bool fr_eq(const fr& a, const fr& b) {
-  uint32_t o = 0;
-#pragma unroll
-  for (int i = 0; i < 8; i++) o |= a.l[i] ^ b.l[i];
-  return o == 0;
-}
-
 // ---- carry-chain primitives ----------------------------------------------
 // Add/sub chains use the compiler's carry builtins (it picks VOP2/VOP3
 // encodings within the constant-bus limit and keeps r's limbs in VGPRs).

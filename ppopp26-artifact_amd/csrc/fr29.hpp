@@ -270,43 +270,6 @@ PL_HD29 void reduce_below_2r(const uint32_t* v, uint32_t* t) {
 
 namespace pl {
 
-// ---- lazy arithmetic on stored-form elements in [0, 2r) -------------------
-// 2r < 2^256, so a value below 2r still fits eight limbs; sums below 4r carry
-// into a 257th bit that the trial subtraction accounts for.
-__host__ __device__ constexpr uint32_t r2_limb(int i) {
-  return i == 0 ? 0x00000002u : i == 1 ? 0xfffffffeu : i == 2 ? 0xfffcb7fdu : i == 3 ? 0xa77b4805u
-       : i == 4 ? 0x1343b00au : i == 5 ? 0x6673b010u : i == 6 ? 0x533afa90u : 0xe7db4ea6u;
-}
-
-// a + b, reduced below 2r (a, b < 2r)
-__device__ __forceinline__ fr fr_add_lazy(const fr& a, const fr& b) {
-  fr s, d;
-  uint32_t c = 0, bo = 0;
-#pragma unroll
-  for (int i = 0; i < 8; i++) s.l[i] = __builtin_addc(a.l[i], b.l[i], c, &c);
-#pragma unroll
-  for (int i = 0; i < 8; i++) d.l[i] = __builtin_subc(s.l[i], r2_limb(i), bo, &bo);
-  const bool ge = (c != 0) | (bo == 0);  // s >= 2r
-  fr o;
-#pragma unroll
-  for (int i = 0; i < 8; i++) o.l[i] = ge ? d.l[i] : s.l[i];
-  return o;
-}
-
-// a - b, in [0, 2r) (a, b < 2r)
-__device__ __forceinline__ fr fr_sub_lazy(const fr& a, const fr& b) {
-  fr d;
-  uint32_t bo = 0;
-#pragma unroll
-  for (int i = 0; i < 8; i++) d.l[i] = __builtin_subc(a.l[i], b.l[i], bo, &bo);
-  const uint32_t mask = 0u - bo;
-  fr o;
-  uint32_t c = 0;
-#pragma unroll
-  for (int i = 0; i < 8; i++) o.l[i] = __builtin_addc(d.l[i], r2_limb(i) & mask, c, &c);
-  return o;
-}
-
 // ---- device product: one asm block per column -----------------------------
 #include "mont29_columns.inc"
 
