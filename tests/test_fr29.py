@@ -108,9 +108,6 @@ def test_modulus_constants():
     body = text[text.index("constexpr uint32_t r29(int i)"):]
     r29 = [int(v, 16) for v in re.findall(r"0x([0-9a-f]+)u", body[: body.index(";")])]
     assert r29 == limbs(R, 9, 29)
-    body = text[text.index("constexpr uint32_t r2_limb(int i)"):]
-    r2 = [int(v, 16) for v in re.findall(r"0x([0-9a-f]+)u", body[: body.index(";")])]
-    assert r2 == limbs(2 * R, 8, 32)
 
 
 def test_generated_columns_current_and_complete():
