@@ -213,6 +213,29 @@ def test_eval_vs_oracle(pl, n):
     np.testing.assert_array_equal(pl.to_host(y), po.eval_(c, x))
 
 
+def test_eval_div_random_mid_sizes_one_workspace(pl):
+    """seeded random sizes over the whole mid-size range (every run length K =
+    4..28, ragged last blocks), evaluation and division with remainder, one
+    workspace sized for the largest reused by every call, against the oracle"""
+    import random
+    import torch
+    rng = random.Random(31337)
+    sizes = [rng.randrange(4096, 255 * 256 * 28 + 1) for _ in range(14)]
+    work = torch.empty((pl.lib().pl_scan_work_bytes(max(sizes)) // 32 + 1, 4), dtype=torch.int64, device="cuda")
+    for j, n in enumerate(sizes):
+        print("mid-size eval/div n=%d" % n, flush=True)
+        a = po.uniform(n, po.seed(po.OP_DIV, 1, j))
+        x = po.uniform(1, 50 + j)
+        ad = pl.to_dev(a)
+        y = pl.evaluate(ad, pl.to_dev(x), work=work)
+        np.testing.assert_array_equal(pl.to_host(y), po.eval_(a, x), "eval n=%d" % n)
+        rem = torch.empty((1, 4), dtype=torch.int64, device="cuda")
+        pl.div_linear(ad, pl.to_dev(x), rem=rem, work=work)
+        q, r = po.div(a, x)
+        np.testing.assert_array_equal(pl.to_host(ad), q, "div n=%d" % n)
+        np.testing.assert_array_equal(pl.to_host(rem), r, "rem n=%d" % n)
+
+
 def test_eval_reused_work_and_zero_point(pl):
     # one work buffer across calls with different points (the mid-size form
     # keeps a flag there that each call must clear), x = 0 and x = 1 included
