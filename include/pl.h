@@ -184,8 +184,8 @@ size_t pl_lde_domain_bytes(const pl_lde_domain* d);
 size_t pl_lde_work_bytes(int lg_n, int lg_blowup);
 int pl_lde(const void* d_coeffs, void* d_out, const pl_lde_domain* d, void* d_work, pl_stream_t stream);
 /* Blowup 3 (no counterpart in the reference, whose our_lde takes
- * power-of-two blowups; the BASELINE form "3 coset NTTs of size N with
- * shifts g w_3N^t"): M = 3N outputs out[k] = sum_i c_i (g w_3N^k)^i, natural
+ * power-of-two blowups; run as three coset NTTs of size N with
+ * shifts g w_3N^t): M = 3N outputs out[k] = sum_i c_i (g w_3N^k)^i, natural
  * order, w_3N = 7^((r-1)/(3N)) (its cube is w_N), g = 7 when h_shift is
  * NULL; lg_n <= 26.  Run by pl_lde on such a domain: three coset NTTs of
  * size N with shifts g w_3N^t side by side, then the interleave

@@ -73,7 +73,7 @@ void ref_eval(const uint64_t* c, uint64_t n, const uint64_t* x, uint64_t* y) {
   st(y, acc);
 }
 
-// The PLONK permutation grand product of BASELINE's miss_ops workload over
+// The PLONK permutation grand product of the miss_ops permutation workload over
 // the reference's field type, element by element as the reference's
 // element-wise operators compose it (function.cu:41-152; the quotient term
 // shape of zksnark_compute_permutation_constraint.cu:13-14, div_mod as a

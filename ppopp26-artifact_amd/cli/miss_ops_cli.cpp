@@ -135,7 +135,7 @@ int main(int argc, char** argv) {
     }
 
     if (perm) {
-      // BASELINE workload: ratio over a,b,c / s1,s2,s3 with k = (1, 7, 13), then z
+      // the permutation workload: ratio over a,b,c / s1,s2,s3 with k = (1, 7, 13), then z
       std::vector<void*> cols(6);
       for (int j = 0; j < 6; j++) {
         std::vector<uint64_t> h(4 * N);

@@ -193,7 +193,7 @@ size_t pl_lde_work_bytes(int lg_n, int lg_blowup) {
   return (size_t)sizeof(pl::fr) << (lg_n + lg_blowup);
 }
 
-// blowup 3 (BASELINE's "3 coset NTTs of size N with shifts g w_3N^t"; no
+// blowup 3 (three coset NTTs of size N with shifts g w_3N^t; no
 // counterpart in the reference, whose our_lde takes power-of-two blowups)
 int pl_lde3_domain_create(int lg_n, const void* h_shift, pl_lde_domain** out) {
   if (!out || lg_n < 1) return PL_ERR_ARG;

@@ -491,19 +491,51 @@ __global__ void __launch_bounds__(256) perm_table_kernel(RatioArgs a, uint32_t c
   }
 }
 
-// num and den of element i (before reduction)
+// v (nine signed limbs, |v_i| < 2^31) carried to balanced digits, the same
+// integer: digits 0..7 in [-2^28, 2^28), the carries into the top limb
+__device__ __forceinline__ void v9_carry_balanced(v9& v) {
+#pragma unroll
+  for (int i = 0; i < 8; i++) {
+    const uint32_t d = sext29(v.l[i]);
+    v.l[i + 1] += (uint32_t)(((int32_t)v.l[i] - (int32_t)d) >> 29);  // exact
+    v.l[i] = d;
+  }
+}
+
+// num and den of element i (before reduction).  SK: k = (1, 7, 13), the
+// reference's coset constants: t = S(beta w^i) is one product and the other
+// two terms are 7 t and 13 t by limb shifts.  t has balanced digits
+// (|t_i| <= 2^28), so 7 t_i fits a signed limb; carried back to balanced
+// digits, 7 t plus 6 t fits again (13 t = 7 t + 6 t); each carried value is
+// the same integer, so n_j = w_j + gamma + k_j t keeps the three-term bound.
+template <bool SK>
 __device__ __forceinline__ void perm_terms_at(const RatioArgs& a, const TermConsts& c, uint64_t i, v9& num,
                                               v9& den) {
   v9 w1;
   unpack_ext(fr_load(a.w1 + (i >> 14)).l, w1.l);
+  v9 t1, t7;  // SK: beta w^i, then 7 beta w^i (balanced digits)
 #pragma unroll
   for (int j = 0; j < 3; j++) {
     v9 wj, sj;
     unpack_ext(fr_load(a.w[j] + i).l, wj.l);
     const v9 wg = v9_add(wj, c.g);
-    uint32_t K[9];
-    pbd_unpack(fr_load(a.tab + j * a.tstride + (i & (kWin - 1))).l, K);
-    const v9 nj = v9_add(wg, v9_mul<false, false, true>(w1, K));
+    v9 kt;
+    if (!SK || j == 0) {
+      uint32_t K[9];
+      pbd_unpack(fr_load(a.tab + j * a.tstride + (i & (kWin - 1))).l, K);
+      kt = v9_mul<false, false, true>(w1, K);
+      if (SK) t1 = kt;
+    } else if (j == 1) {
+#pragma unroll
+      for (int l = 0; l < 9; l++) t7.l[l] = (t1.l[l] << 3) - t1.l[l];
+      v9_carry_balanced(t7);
+      kt = t7;
+    } else {
+#pragma unroll
+      for (int l = 0; l < 9; l++) kt.l[l] = t7.l[l] + (t1.l[l] << 2) + (t1.l[l] << 1);
+      v9_carry_balanced(kt);
+    }
+    const v9 nj = v9_add(wg, kt);
     unpack_ext(fr_load(a.s[j] + i).l, sj.l);
     const v9 dj = v9_add(wg, v9_mul<false, false, true>(sj, c.B));
     num = j == 0 ? nj : v9_mul<false, false, false>(num, nj.l);  // the first factor needs no product
@@ -517,14 +549,14 @@ __device__ __forceinline__ void perm_terms_at(const RatioArgs& a, const TermCons
 // per lane (step 1 pairs num on even lanes with den on odd lanes; after steps
 // 2 and 3, lane 8g holds the run's num product and lane 8g + 1 its den
 // product).  Lanes past n carry R' mod r.
-template <bool ZERO_DEN>
+template <bool ZERO_DEN, bool SK>
 __global__ void __launch_bounds__(256) perm_terms_kernel(RatioArgs a, fr* rn, fr* rd) {
   const uint64_t i = (uint64_t)blockIdx.x * blockDim.x + threadIdx.x;
   if constexpr (!ZERO_DEN) {
     if (i >= a.n) return;
     const TermConsts c = term_consts(a);
     v9 num, den;
-    perm_terms_at(a, c, i, num, den);
+    perm_terms_at<SK>(a, c, i, num, den);
     fr_store(a.num + i, v9_canonical(num));
     fr_store(a.den + i, v9_canonical(den));
   } else {
@@ -532,7 +564,7 @@ __global__ void __launch_bounds__(256) perm_terms_kernel(RatioArgs a, fr* rn, fr
     if (i < a.n) {
       const TermConsts c = term_consts(a);
       v9 num, den;
-      perm_terms_at(a, c, i, num, den);
+      perm_terms_at<SK>(a, c, i, num, den);
       nw = v9_canonical(num);
       dw = v9_canonical(den);
       if (fr_is_zero(dw)) {
@@ -985,6 +1017,19 @@ static RatioArgs ratio_args(const fr* const w[3], const fr* const sg[3], const f
   return a;
 }
 
+// k = (1, 7, 13) in Montgomery form, the coset constants of the miss_ops
+// permutation workload (terms of the shape of
+// zksnark_compute_permutation_constraint.cu:13-14): the terms kernel then
+// forms k_1 t and k_2 t by limb shifts
+static bool reference_ks(const host::fr64 k[3]) {
+  const uint64_t v[3] = {1, 7, 13};
+  for (int j = 0; j < 3; j++) {
+    const host::fr64 m = host::from_u64(v[j]);
+    if (memcmp(m.l, k[j].l, sizeof(m.l)) != 0) return false;
+  }
+  return true;
+}
+
 // the window tables at the front of scratch (perm_table_elems(n) elements)
 static void perm_tables(RatioArgs& a, fr*& scratch, hipStream_t s) {
   const uint32_t count = (uint32_t)std::min<uint64_t>(a.n, kWin);
@@ -1001,8 +1046,12 @@ hipError_t perm_ratio(const fr* const w[3], const fr* const sg[3], const fr* bet
   perm_tables(a, scratch, s);
   a.num = ratio;
   a.den = scratch;
-  hipLaunchKernelGGL(perm_terms_kernel<false>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, nullptr,
-                     nullptr);
+  if (reference_ks(k))
+    hipLaunchKernelGGL((perm_terms_kernel<false, true>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a,
+                       nullptr, nullptr);
+  else
+    hipLaunchKernelGGL((perm_terms_kernel<false, false>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a,
+                       nullptr, nullptr);
   // ratio = num / den in place over num
   return batch_inverse(scratch, ratio, ratio, n, scratch + n, s);
 }
@@ -1017,7 +1066,12 @@ hipError_t perm_grand_product(const fr* const w[3], const fr* const sg[3], const
   const uint64_t nr = (n + kOpPer - 1) / kOpPer;
   fr* rn = scratch + n;
   fr* rd = rn + nr;
-  hipLaunchKernelGGL(perm_terms_kernel<true>, dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, rn, rd);
+  if (reference_ks(k))
+    hipLaunchKernelGGL((perm_terms_kernel<true, true>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, rn,
+                       rd);
+  else
+    hipLaunchKernelGGL((perm_terms_kernel<true, false>), dim3((unsigned)((n + 255) / 256)), dim3(256), 0, s, a, rn,
+                       rd);
   // levels above the elements: the runs, then block products
   struct Level {
     fr *nv, *dv, *tn, *td;

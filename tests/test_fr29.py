@@ -7,7 +7,8 @@ its constants, on the host.
 * the DFT constants in ntt_pass.hpp (balanced digits) against the
   reference's own 16th roots of unity (tests/golden/ref_constants.json),
   times 2^261;
-* r and 2r as written in fr29.hpp;
+* r as written in fr29.hpp;
+* the permutation terms' limb shifts for k = (1, 7, 13) (7 t, 13 t);
 * mont29_columns.inc is what its generator writes, every column block adds
   each product term exactly once, and the balanced-digit blocks close the
   previous column as (acc + 2^28) >> 29.
@@ -142,3 +143,44 @@ def test_generated_columns_current_and_complete():
                     want.add(("m[%d]" % i, "nr29(%d)" % (k - i)))
         assert sorted(pairs) == sorted(want), k
 
+
+def test_reference_constant_terms_limb_shifts():
+    """The terms kernel's k = (1, 7, 13) path (ops.hip perm_terms_at<SK>): from
+    t with balanced digits (|t_i| <= 2^28, small signed top), 7 t by limb shifts,
+    carried to balanced digits (v9_carry_balanced), then 7 t + 6 t carried
+    again.  Every int32 step stays in range, the integers are exactly 7 T and
+    13 T, and the digits come out balanced.  Emulates the 32-bit limb arithmetic."""
+    import random
+    rng = random.Random(7)
+
+    def i32(v):
+        assert -(1 << 31) <= v < (1 << 31), v
+        return v
+
+    def sext29(v):
+        v &= (1 << 29) - 1
+        return v - (1 << 29) if v >= (1 << 28) else v
+
+    def carry(v):
+        v = list(v)
+        for i in range(8):
+            d = sext29(v[i])
+            c = i32(v[i] - d) >> 29
+            v[i + 1] = i32(v[i + 1] + c)
+            v[i] = d
+        return v
+
+    def value(v):
+        return sum(l << (29 * i) for i, l in enumerate(v))
+
+    cases = [[-(1 << 28)] * 8 + [-(1 << 23)], [(1 << 28) - 1] * 8 + [(1 << 23)],
+             [(1 << 28) - 1, -(1 << 28)] * 4 + [0]]
+    cases += [[rng.randrange(-(1 << 28), 1 << 28) for _ in range(8)] + [rng.randrange(-(1 << 23), 1 << 23)]
+              for _ in range(20000)]
+    for t in cases:
+        T = value(t)
+        t7 = carry([i32((x << 3) - x) for x in t])
+        assert value(t7) == 7 * T and all(-(1 << 28) <= d < (1 << 28) for d in t7[:8])
+        t13 = carry([i32(a + (x << 2) + (x << 1)) for a, x in zip(t7, t)])
+        assert value(t13) == 13 * T and all(-(1 << 28) <= d < (1 << 28) for d in t13[:8])
+        assert abs(t13[8]) < (1 << 28)

@@ -174,6 +174,48 @@ def test_perm_grand_product_zero_den(pl, lg, i0):
     np.testing.assert_array_equal(pl.to_host(z), exp)
 
 
+@pytest.mark.parametrize("lg", [12, 20])
+@pytest.mark.parametrize("kind", ["small", "permuted", "random"])
+def test_perm_general_constants(pl, lg, kind):
+    """coset constants other than (1, 7, 13) take the general terms path (one
+    product per k_j w^i term); (1, 7, 13) itself forms 7 t and 13 t by limb
+    shifts.  Both against the oracle, ratio and grand product."""
+    n = 1 << lg
+    print("perm general k", lg, kind, flush=True)
+    cols, beta, gamma = _perm_inputs(n, lg, 80)
+    if kind == "small":
+        ks = [po.from_u64(1), po.from_u64(5), po.from_u64(11)]
+    elif kind == "permuted":
+        ks = [po.from_u64(7), po.from_u64(1), po.from_u64(13)]
+    else:
+        ks = [po.uniform(1, 90 + j) for j in range(3)]
+    dom = pl.Domain(lg)
+    dev = [pl.to_dev(c) for c in cols]
+    exp_ratio = po.perm_ratio(cols, beta, gamma, ks, lg)
+    ratio = pl.perm_ratio(dev[:3], dev[3:], pl.to_dev(beta), pl.to_dev(gamma), np.concatenate(ks), dom)
+    np.testing.assert_array_equal(pl.to_host(ratio), exp_ratio)
+    z = pl.perm_ratio(dev[:3], dev[3:], pl.to_dev(beta), pl.to_dev(gamma), np.concatenate(ks), dom,
+                      grand_product=True)
+    np.testing.assert_array_equal(pl.to_host(z), po.grand_product(exp_ratio))
+
+
+def test_perm_reference_constants_extreme_terms(pl):
+    """k = (1, 7, 13) with beta = r - 1 and w^i near r: the limb-shift terms 7 t
+    and 13 t at their largest digits, against the oracle"""
+    lg = 14
+    n = 1 << lg
+    rm1 = [0xFFFFFFFF00000000, 0x53BDA402FFFE5BFE, 0x3339D80809A1D805, 0x73EDA753299D7D48]
+    cols, _, gamma = _perm_inputs(n, lg, 120)
+    beta = np.array([rm1], dtype=np.uint64)
+    ks = [po.from_u64(1), po.from_u64(7), po.from_u64(13)]
+    dom = pl.Domain(lg)
+    dev = [pl.to_dev(c) for c in cols]
+    exp_ratio = po.perm_ratio(cols, beta, gamma, ks, lg)
+    z = pl.perm_ratio(dev[:3], dev[3:], pl.to_dev(beta), pl.to_dev(gamma), np.concatenate(ks), dom,
+                      grand_product=True)
+    np.testing.assert_array_equal(pl.to_host(z), po.grand_product(exp_ratio))
+
+
 def test_perm_grand_product_deep(pl):
     """2^26 elements (runs of 8, then block products over two levels): the
     fused grand product equals the oracle's grand product of the ratio vector,

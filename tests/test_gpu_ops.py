@@ -52,7 +52,7 @@ def test_lde_blowup3_vs_oracle(pl, lg_n):
     """The blowup-3 LDE (3 coset NTTs of size N with shifts 7 w_3N^t, output
     interleaved to the natural order of the 3N-point domain) against the
     oracle, whose closed form is pinned by brute force; at 2^20 and 2^22 the
-    BASELINE pipeline on evaluations (iNTT first) as well."""
+    evaluation-domain pipeline (iNTT, times 7^i, coset NTTs) as well."""
     from conftest import assert_fr_equal
     import torch
     c = po.uniform(1 << lg_n, po.seed(po.OP_LDE, lg_n, 3))
