@@ -29,8 +29,8 @@ def test_lde_golden_small(pl, golden):
             np.testing.assert_array_equal(pl.to_host(pl.lde(c, g)), k["lde_g_%d_%d" % (lg_n, lb)])
 
 
-@pytest.mark.parametrize("lg_n,lb", [(8, 1), (9, 2), (12, 2), (12, 3), (16, 2), (17, 1), (20, 2), (20, 3),
-                                     (22, 2), (22, 3), (24, 2), (24, 3)])
+@pytest.mark.parametrize("lg_n,lb", [(8, 1), (9, 2), (12, 2), (12, 3), (12, 4), (16, 2), (17, 1), (18, 4),
+                                     (20, 2), (20, 3), (22, 2), (22, 3), (24, 2), (24, 3)])
 def test_lde_vs_oracle(pl, lg_n, lb):
     """our_lde bit-exact over the whole output, up to the benchmark sizes
     (2^24 -> 2^26 and 2^27; the oracle's coset transforms run on the host's
