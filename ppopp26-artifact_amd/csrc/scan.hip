@@ -253,7 +253,7 @@ __device__ void pw_extend(const PwExt& e) {
 // the block finishes it.
 template <int LG>
 __global__ void __launch_bounds__(kScanThreads) sfx_reduce_kernel(const fr* a, uint64_t n, const fr* pw, fr* partial,
-                                                                   PwExt ext) {
+                                                                   PwExt ext, const fr* mtab) {
   constexpr int kPer = ScanGeom<LG>::kPer, kLgSeg = ScanGeom<LG>::kLgSeg;
   __shared__ uint4 l[2][kScanThreads];
   const int t = threadIdx.x;
@@ -275,14 +275,19 @@ __global__ void __launch_bounds__(kScanThreads) sfx_reduce_kernel(const fr* a, u
     v[k] = i < n ? fr_load(a + i) : fr_zero();
   });
   fr h = horner29<kPer>(v, M);
-  // m^t R' from the powers m^(2^bit) R'
+  // m^t R': from the table when the caller built one (sfx_powers_kernel),
+  // else from the powers m^(2^bit) R'
   fr mt;
+  if (mtab) {
+    mt = fr_load(mtab + t);
+  } else {
 #pragma unroll
-  for (int i = 0; i < 8; i++) mt.l[i] = rp_one_limb(i);
-  static_for<0, 8>([&](auto b_) {
-    constexpr int b = decltype(b_)::value;
-    if ((t >> b) & 1) mt = mulp(mt, fr_load(pw + b));
-  });
+    for (int i = 0; i < 8; i++) mt.l[i] = rp_one_limb(i);
+    static_for<0, 8>([&](auto b_) {
+      constexpr int b = decltype(b_)::value;
+      if ((t >> b) & 1) mt = mulp(mt, fr_load(pw + b));
+    });
+  }
   h = mulp(h, mt);
   lds_put(l, t, h);
   __syncthreads();
@@ -400,14 +405,32 @@ hipError_t fill_zero(fr* p, uint64_t n, hipStream_t s) {
 // doublings) is m R' in the stored form; squares stay in that form.  Level L
 // of a scan (values combined 4096^L at a time) uses pw + 12 L: its multiplier
 // is m^(4096^L), whose powers of two continue the same squaring chain.
-__global__ void sfx_powers_kernel(const fr* src, int neg, fr* pw, int count) {
-  if (threadIdx.x != 0) return;
-  fr m = fr_load(src);
-  if (neg) m = fr_neg(m);
-  for (int d = 0; d < 5; d++) m = fr_add(m, m);
+// tab (evaluation, 128 threads): also tab[t] = m^t R' for t < 256, which the
+// reduce multiplies each thread's value by; the second wave extends it by
+// doubling (step j fills [2^j, 2^(j+1)) from pw[j]) while thread 0 raises the
+// next square.
+__global__ void sfx_powers_kernel(const fr* src, int neg, fr* pw, int count, fr* tab) {
+  const int t = threadIdx.x;
+  fr m;
+  if (t == 0) {
+    m = fr_load(src);
+    if (neg) m = fr_neg(m);
+    for (int d = 0; d < 5; d++) m = fr_add(m, m);
+  }
+  if (tab && t == 64) {
+    fr one;
+#pragma unroll
+    for (int i = 0; i < 8; i++) one.l[i] = rp_one_limb(i);
+    fr_store(tab, one);
+  }
   for (int j = 0; j < count; j++) {
-    fr_store(pw + j, m);
-    m = mulp(m, m);
+    if (t == 0) fr_store(pw + j, m);
+    if (tab) __syncthreads();  // pw[j] and the table's entries below 2^j visible
+    if (t == 0 && j + 1 < count) m = mulp(m, m);
+    if (tab && j < 8 && t >= 64) {
+      for (int i = t - 64; i < (1 << j); i += 64)
+        fr_store(tab + (1 << j) + i, mulp(fr_load(tab + i), fr_load(pw + j)));
+    }
   }
 }
 
@@ -439,7 +462,7 @@ static uint64_t sfx_scratch_elems(uint64_t n) {
 // *total = s_0 (if total).  out == nullptr: reduction only.  A top level of at
 // most 256 values runs one element per thread.
 static hipError_t suffix_scan(const fr* a, fr* out, uint64_t n, const fr* pw, fr* total, fr* scratch, hipStream_t s,
-                              PwExt ext = PwExt{nullptr, 0, 0}) {
+                              PwExt ext = PwExt{nullptr, 0, 0}, const fr* mt = nullptr) {
   if (n <= (uint64_t)kSeg) {
     const bool tiny = n <= (uint64_t)kScanThreads;
     if (out) {
@@ -453,9 +476,9 @@ static hipError_t suffix_scan(const fr* a, fr* out, uint64_t n, const fr* pw, fr
       // one block: the reduce kernel's tree value is s_0
       const PwExt none{nullptr, 0, 0};
       if (tiny)
-        hipLaunchKernelGGL(sfx_reduce_kernel<0>, dim3(1), dim3(kScanThreads), 0, s, a, n, pw, total, none);
+        hipLaunchKernelGGL(sfx_reduce_kernel<0>, dim3(1), dim3(kScanThreads), 0, s, a, n, pw, total, none, mt);
       else
-        hipLaunchKernelGGL(sfx_reduce_kernel<kLgPer>, dim3(1), dim3(kScanThreads), 0, s, a, n, pw, total, none);
+        hipLaunchKernelGGL(sfx_reduce_kernel<kLgPer>, dim3(1), dim3(kScanThreads), 0, s, a, n, pw, total, none, mt);
     }
     return hipGetLastError();
   }
@@ -469,7 +492,7 @@ static hipError_t suffix_scan(const fr* a, fr* out, uint64_t n, const fr* pw, fr
   if (out)
     hipLaunchKernelGGL(sfx_run_reduce_kernel, dim3(grid), dim3(kScanThreads), 0, s, a, n, pw, part, lt, ext);
   else
-    hipLaunchKernelGGL(sfx_reduce_kernel<kLgPer>, dim3(grid), dim3(kScanThreads), 0, s, a, n, pw, part, ext);
+    hipLaunchKernelGGL(sfx_reduce_kernel<kLgPer>, dim3(grid), dim3(kScanThreads), 0, s, a, n, pw, part, ext, mt);
   // carry[b] = s at the end of block b = sum_{b' > b} part[b'] (m^4096)^(b'-b-1)
   hipError_t e = suffix_scan(part, out ? carry : nullptr, nb, pw + kLgSeg, total, lt + nb * kScanThreads, s);
   if (e != hipSuccess || !out) return e;
@@ -814,7 +837,8 @@ static hipError_t div_lat(fr* inout, uint64_t n, const fr* c, fr* rem, fr* ws, h
 // ---------------------------------------------------------------------------
 
 uint64_t scan_scratch_elems(uint64_t n) {
-  uint64_t a = pfx_scratch_elems(n), b = sfx_scratch_elems(n);
+  // the evaluation's m^t table sits between the powers and the scan
+  uint64_t a = pfx_scratch_elems(n), b = sfx_scratch_elems(n) + kScanThreads;
   if (n >= kLatMin && n <= kLatMax && b < lat_ws_elems(n)) b = lat_ws_elems(n);
   return a > b ? a : b;
 }
@@ -829,7 +853,7 @@ hipError_t div_linear(fr* inout, uint64_t n, const fr* c, fr* rem, fr* scratch, 
   if (n >= kLatMin && n <= kLatMax) return div_lat(inout, n, c, rem, scratch, s);
   fr* pw = scratch;
   const int np = sfx_pw_count(n);
-  hipLaunchKernelGGL(sfx_powers_kernel, dim3(1), dim3(64), 0, s, c, 1, pw, std::min(np, kPw));
+  hipLaunchKernelGGL(sfx_powers_kernel, dim3(1), dim3(64), 0, s, c, 1, pw, std::min(np, kPw), (fr*)nullptr);
   return suffix_scan(inout, inout, n, pw, rem, scratch + np, s, pw_ext(pw, np));
 }
 
@@ -837,8 +861,10 @@ hipError_t evaluate(const fr* coeffs, uint64_t n, const fr* x, fr* y, fr* scratc
   if (n >= kLatMin && n <= kLatMax) return evaluate_lat(coeffs, n, x, y, scratch, s);
   fr* pw = scratch;
   const int np = sfx_pw_count(n);
-  hipLaunchKernelGGL(sfx_powers_kernel, dim3(1), dim3(64), 0, s, x, 0, pw, std::min(np, kPw));
-  return suffix_scan(coeffs, nullptr, n, pw, y, scratch + np, s, pw_ext(pw, np));
+  // the reduce's factors m^t (t < 256) as a table beside the powers
+  fr* mt = scratch + np;
+  hipLaunchKernelGGL(sfx_powers_kernel, dim3(1), dim3(128), 0, s, x, 0, pw, std::min(np, kPw), mt);
+  return suffix_scan(coeffs, nullptr, n, pw, y, mt + kScanThreads, s, pw_ext(pw, np), mt);
 }
 
 // one scan for an evaluation + division pair: q(X) = (p(X) - p(z)) / (X - z)
@@ -846,7 +872,7 @@ hipError_t evaluate(const fr* coeffs, uint64_t n, const fr* x, fr* y, fr* scratc
 hipError_t eval_div(const fr* in, fr* q, uint64_t n, const fr* z, fr* y, fr* scratch, hipStream_t s) {
   fr* pw = scratch;
   const int np = sfx_pw_count(n);
-  hipLaunchKernelGGL(sfx_powers_kernel, dim3(1), dim3(64), 0, s, z, 0, pw, std::min(np, kPw));
+  hipLaunchKernelGGL(sfx_powers_kernel, dim3(1), dim3(64), 0, s, z, 0, pw, std::min(np, kPw), (fr*)nullptr);
   return suffix_scan(in, q, n, pw, y, scratch + np, s, pw_ext(pw, np));
 }
 
