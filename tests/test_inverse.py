@@ -113,6 +113,68 @@ def test_mod_update_is_the_exact_reduced_combination(mod_update_host):
     assert not bad, "first failing cases: %s" % bad[:3]
 
 
+def _steps_if_form(a, b):
+    """One round's 31 steps as the algorithm states them (branches, unpacked
+    factors), on the same 64-bit approximations: low 31 bits and the 33 bits
+    below the top of max(a, b) (at least 64 bits wide)."""
+    n = max((a | b).bit_length(), 64)
+    xa = (a & 0x7FFFFFFF) | (((a >> (n - 33)) & ((1 << 33) - 1)) << 31)
+    xb = (b & 0x7FFFFFFF) | (((b >> (n - 33)) & ((1 << 33) - 1)) << 31)
+    f0, g0, f1, g1 = 1, 0, 0, 1
+    for _ in range(31):
+        if xa & 1:
+            if xa < xb:
+                xa, xb, f0, f1, g0, g1 = xb, xa, f1, f0, g1, g0
+            xa, f0, g0 = xa - xb, f0 - f1, g0 - g1
+        xa >>= 1
+        f1, g1 = 2 * f1, 2 * g1
+    return f0, g0, f1, g1
+
+
+def test_inner_steps_match_the_stated_steps(tmp_path):
+    # inner_steps (its 64-bit approximations taken limb by limb) against the
+    # steps on Python integers, on inputs that drive the factors to their
+    # extremes: row 1 reaches +-2^31, the bound add_mul and mod_update accept,
+    # and row 0 passes 2^30 in magnitude; and on random pairs
+    src = tmp_path / "steps_host.cpp"
+    src.write_text(
+        '#include "%s"\n'
+        'extern "C" void steps_host(const uint32_t* a, const uint32_t* b, int64_t* m) '
+        '{ pl::inv::inner_steps(a, b, m); }\n' % os.path.abspath(HDR)
+    )
+    lib = tmp_path / "libsteps.so"
+    subprocess.check_call(["g++", "-O2", "-std=c++17", "-shared", "-fPIC", str(src), "-o", str(lib)])
+    so = ctypes.CDLL(str(lib))
+    so.steps_host.argtypes = [ctypes.c_void_p, ctypes.c_void_p, ctypes.c_void_p]
+    so.steps_host.restype = None
+
+    def call(a, b):
+        A = (ctypes.c_uint32 * 8)(*[(a >> (32 * i)) & 0xFFFFFFFF for i in range(8)])
+        B = (ctypes.c_uint32 * 8)(*[(b >> (32 * i)) & 0xFFFFFFFF for i in range(8)])
+        m = (ctypes.c_int64 * 4)()
+        so.steps_host(A, B, m)
+        return tuple(m)
+
+    rng = random.Random(3131)
+    cases = [(1, 1), (1, 2), (2, 1), (1, R), (R, 1), (R - 2, R), (3, (1 << 64) - 1), ((1 << 64) - 1, 3),
+             (1, (1 << 255) - 1), ((1 << 31) + 1, 1 << 62), (1 << 40, 1)]
+    cases += [(rng.randrange(1, 1 << 256), rng.randrange(1, 1 << 256)) for _ in range(3000)]
+    cases += [(rng.randrange(1, 1 << 64) | 1, rng.randrange(1, 1 << 64)) for _ in range(3000)]
+    cases += [(rng.randrange(1, 1 << k) | 1, rng.randrange(1, R)) for k in range(1, 256, 5) for _ in range(10)]
+    seen = [0, 0, 0, 0]
+    bad = []
+    for a, b in cases:
+        want = _steps_if_form(a, b)
+        got = call(a, b)
+        if got != want:
+            bad.append((hex(a), hex(b), got, want))
+        seen = [max(s, abs(v)) for s, v in zip(seen, want)]
+    assert not bad, "first failing cases: %s" % bad[:3]
+    # the extremes were reached: row 1 at 2^31, row 0 past 2^30
+    assert seen[2] == 1 << 31 and seen[3] == 1 << 31
+    assert seen[0] > 1 << 30 and seen[1] > 1 << 30
+
+
 def test_reduce_below_5r_at_the_boundaries(tmp_path):
     # x = k r + e for k < 5: e = 0, 1, r - 1 and e just below the width where
     # the top-bits estimate of k falls one short (its one correction step)
