@@ -138,11 +138,12 @@ def _perm_inputs(n, lg, salt):
     return cols, beta, gamma
 
 
-@pytest.mark.parametrize("lg", [1, 3, 8, 11, 12, 13, 14, 15, 21, 22])
+@pytest.mark.parametrize("lg", [1, 3, 8, 11, 12, 13, 14, 15, 18, 20, 21, 22])
 def test_perm_grand_product_levels(pl, lg):
     """every top-level geometry (a single block of 256 x 1, 2, 4, 8 run
-    products, directly above the runs of 8 elements or above block products)
-    against the oracle's ratio and grand product"""
+    products, directly above the runs of 8 elements or above block products,
+    whose count ranges from 2 to 256 blocks here) against the oracle's ratio
+    and grand product"""
     n = 1 << lg
     print("perm gp levels", lg, flush=True)
     cols, beta, gamma = _perm_inputs(n, lg, 40)
