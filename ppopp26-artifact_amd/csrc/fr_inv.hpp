@@ -46,7 +46,14 @@ PL_HD int bitlen8(const uint32_t* x) {
 // bits [s, s + 33) of x (s >= 31), as an integer
 PL_HD uint64_t bits33(const uint32_t* x, int s) {
   const int w = s >> 5, sh = s & 31;
-  const uint64_t x0 = w < 8 ? x[w] : 0, x1 = w + 1 < 8 ? x[w + 1] : 0, x2 = w + 2 < 8 ? x[w + 2] : 0;
+  // limbs w, w + 1, w + 2 (0 past the top) by selects, not an index, so that
+  // on a GPU the limbs stay in (scalar) registers
+  uint64_t x0 = 0, x1 = 0, x2 = 0;
+  for (int i = 0; i < 8; i++) {
+    x0 = i == w ? x[i] : x0;
+    x1 = i == w + 1 ? x[i] : x1;
+    x2 = i == w + 2 ? x[i] : x2;
+  }
   uint64_t lo = x0 | (x1 << 32);
   uint64_t v = sh ? ((lo >> sh) | (x2 << (64 - sh))) : lo;
   return v & ((1ull << 33) - 1);
@@ -84,20 +91,6 @@ PL_HD void add_mul(S10& acc, const uint32_t* x, int64_t f) {
 
 PL_HD bool s10_neg(const S10& a) { return (a.w[9] >> 31) != 0; }
 
-PL_HD void s10_negate(S10& a) {
-  uint64_t b = 1;
-  for (int i = 0; i < 10; i++) {
-    uint64_t t = (uint64_t)(uint32_t)~a.w[i] + b;
-    a.w[i] = (uint32_t)t;
-    b = t >> 32;
-  }
-}
-
-// x = a >> 31 (a non-negative, result < 2^256)
-PL_HD void s10_shr31_to8(const S10& a, uint32_t* x) {
-  for (int i = 0; i < 8; i++) x[i] = (a.w[i] >> 31) | (a.w[i + 1] << 1);
-}
-
 // x (nine limbs, x < 5 r) <- x mod r.  qe = floor(X / (r_hi + 1)) from the
 // top 64 bits X = x >> 224 (r_hi = r >> 224) is floor(x / r) or one less:
 // X / (r_hi + 1) < x / r < (X + 1) / r_hi, and the two ends differ by
@@ -127,38 +120,6 @@ PL_HD void reduce_below_5r(uint32_t* x) {
   }
   if (bw == 0)
     for (int i = 0; i < 9; i++) x[i] = d[i];
-}
-
-// u' = (u f0 + v g0) / 2^31 mod r, with u, v < r
-PL_HD void mod_update(uint32_t* out, const uint32_t* u, int64_t f, const uint32_t* v, int64_t g) {
-  S10 t = {};
-  add_mul(t, u, f);
-  add_mul(t, v, g);
-  // |t| < 2^32 r: make it non-negative by adding r * 2^32
-  if (s10_neg(t)) {
-    uint64_t c = 0;
-    for (int i = 0; i < 10; i++) {
-      uint64_t rr = (i >= 1 && i <= 8) ? kR[i - 1] : 0;
-      uint64_t s = (uint64_t)t.w[i] + rr + c;
-      t.w[i] = (uint32_t)s;
-      c = s >> 32;
-    }
-  }
-  // Montgomery step: q = -t mod 2^31 (r^-1 = 1 mod 2^31); t += q r; t /= 2^31
-  const uint32_t q = (0u - t.w[0]) & 0x7fffffffu;
-  {
-    uint64_t c = 0;
-    for (int i = 0; i < 10; i++) {
-      uint64_t s = (uint64_t)t.w[i] + c + (i < 8 ? (uint64_t)q * kR[i] : 0);
-      t.w[i] = (uint32_t)s;
-      c = s >> 32;
-    }
-  }
-  uint32_t x[9];
-  for (int i = 0; i < 8; i++) x[i] = (t.w[i] >> 31) | (t.w[i + 1] << 1);
-  x[8] = (t.w[8] >> 31) | (t.w[9] << 1);
-  reduce_below_5r(x);  // x < 5 r
-  for (int i = 0; i < 8; i++) out[i] = x[i];
 }
 
 // x <- -x mod r for x in [0, r)
@@ -213,22 +174,60 @@ PL_HD void inner_steps(const uint32_t* a, const uint32_t* b, int64_t* m) {
 // Role 2 (3) then takes role 0's (1's) sign: res <- -res mod r if that row
 // was negated (fix by negate_mod_r).  The four are independent, so the wave
 // form runs them on four lanes at once.
+// All four roles run one code path (in the wave form, no divergence but the
+// final reduction): t = f x + g y with (x, y) = (a, b) or (u, v); when t < 0
+// the integer rows negate it (~t + 1) and the modular rows add 2^32 r, in one
+// 10-limb pass; the modular rows then add q r, q = -t mod 2^31 (a Montgomery
+// step, q = 0 for the integer rows); both shift right by 31, and the modular
+// rows reduce below r.
 PL_HD void round_role(int role, const uint32_t* a, const uint32_t* b, const uint32_t* u, const uint32_t* v,
                       const int64_t* m, uint32_t* res, bool* neg) {
-  const int64_t f = m[(role & 1) * 2], g = m[(role & 1) * 2 + 1];
-  *neg = false;
-  if (role < 2) {
-    S10 t = {};
-    add_mul(t, a, f);
-    add_mul(t, b, g);
-    if (s10_neg(t)) {
-      s10_negate(t);
-      *neg = true;
-    }
-    s10_shr31_to8(t, res);
-  } else {
-    mod_update(res, u, f, v, g);
+  const bool modr = role >= 2;
+  // selects, not an index: the role differs per lane in the wave form
+  const int64_t f = (role & 1) ? m[2] : m[0], g = (role & 1) ? m[3] : m[1];
+  uint32_t x[8], y[8];
+  for (int i = 0; i < 8; i++) {
+    x[i] = modr ? u[i] : a[i];
+    y[i] = modr ? v[i] : b[i];
   }
+  S10 t = {};
+  add_mul(t, x, f);
+  add_mul(t, y, g);
+  const bool tneg = s10_neg(t);
+  *neg = !modr && tneg;
+  {
+    const uint32_t flip = *neg ? 0xffffffffu : 0u;
+    const bool addr = modr && tneg;  // |t| < 2^32 r: t + 2^32 r >= 0
+    uint64_t c = *neg ? 1 : 0;
+    for (int i = 0; i < 10; i++) {
+      const uint64_t rr = (addr && i >= 1 && i <= 8) ? kR[i - 1] : 0;
+      const uint64_t s = (uint64_t)(t.w[i] ^ flip) + rr + c;
+      t.w[i] = (uint32_t)s;
+      c = s >> 32;
+    }
+  }
+  // q = -t mod 2^31 makes t + q r divisible by 2^31 (r^-1 = 1 mod 2^31)
+  const uint32_t q = modr ? (0u - t.w[0]) & 0x7fffffffu : 0u;
+  {
+    uint64_t c = 0;
+    for (int i = 0; i < 10; i++) {
+      const uint64_t s = (uint64_t)t.w[i] + c + (i < 8 ? (uint64_t)q * kR[i] : 0);
+      t.w[i] = (uint32_t)s;
+      c = s >> 32;
+    }
+  }
+  uint32_t z[9];
+  for (int i = 0; i < 8; i++) z[i] = (t.w[i] >> 31) | (t.w[i + 1] << 1);
+  z[8] = (t.w[8] >> 31) | (t.w[9] << 1);
+  if (modr) reduce_below_5r(z);  // z < 5 r
+  for (int i = 0; i < 8; i++) res[i] = z[i];
+}
+
+// u' = (u f + v g) / 2^31 mod r, with u, v < r: the modular row of a round
+PL_HD void mod_update(uint32_t* out, const uint32_t* u, int64_t f, const uint32_t* v, int64_t g) {
+  const int64_t m[4] = {f, g, 0, 0};
+  bool neg;
+  round_role(2, u, v, u, v, m, out, &neg);
 }
 
 // y^-1 mod r for 0 < y < r (y as 8 little-endian limbs); 0 for y = 0
