@@ -73,17 +73,11 @@ PL_HD void add_mul(S10& acc, const uint32_t* x, int64_t f) {
   }
   p[8] = (uint32_t)c;
   p[9] = 0;
-  if (neg) {  // two's complement of p over 10 limbs
-    uint64_t b = 1;
-    for (int i = 0; i < 10; i++) {
-      uint64_t t = (uint64_t)(uint32_t)~p[i] + b;
-      p[i] = (uint32_t)t;
-      b = t >> 32;
-    }
-  }
-  uint64_t cc = 0;
+  // acc + (neg ? -p : p) in one carry chain: -p = ~p + 1 over 10 limbs
+  const uint32_t mk = neg ? 0xffffffffu : 0u;
+  uint64_t cc = neg ? 1 : 0;
   for (int i = 0; i < 10; i++) {
-    uint64_t t = (uint64_t)acc.w[i] + p[i] + cc;
+    uint64_t t = (uint64_t)acc.w[i] + (p[i] ^ mk) + cc;
     acc.w[i] = (uint32_t)t;
     cc = t >> 32;
   }
@@ -143,23 +137,43 @@ PL_HD void inner_steps(const uint32_t* a, const uint32_t* b, int64_t* m) {
   if (n < 64) n = 64;
   uint64_t xa = (uint64_t)(a[0] & 0x7fffffffu) | (bits33(a, n - 33) << 31);
   uint64_t xb = (uint64_t)(b[0] & 0x7fffffffu) | (bits33(b, n - 33) << 31);
-  int64_t f0 = 1, g0 = 0, f1 = 0, g1 = 1;
-  for (int j = 0; j < 31; j++) {
+  // One step: if a is odd, swap the rows when a < b, then row 0 -= row 1;
+  // a halves, row 1 doubles.  The rows (f0, g0) and (f1, g1) run packed as
+  // f + 2^32 g (mod 2^64): every step is linear in the rows, so the packed
+  // words follow them exactly, and after j steps |f0|, |g0| <= 2^j - 1 and
+  // |f1|, |g1| <= 2^j, so after 30 steps every factor is the sign-extended
+  // half it sits in.  The 31st step, whose row 1 can reach 2^31, runs on the
+  // unpacked factors.  a < b is the borrow of a - b from bit 63: logic on
+  // scalar registers on a GPU, where a 64-bit compare would take a vector one.
+  uint64_t p0 = 1, p1 = (uint64_t)1 << 32;
+  for (int j = 0; j < 30; j++) {
     if (xa & 1) {
-      if (xa < xb) {
-        uint64_t tx = xa; xa = xb; xb = tx;
-        int64_t tf = f0; f0 = f1; f1 = tf;
-        int64_t tg = g0; g0 = g1; g1 = tg;
-      }
-      xa = (xa - xb) >> 1;
-      f0 -= f1;
-      g0 -= g1;
-    } else {
-      xa >>= 1;
+      const uint64_t d = xa - xb;
+      // -1 when xa < xb, else 0, applied as a mask (a branch on it would
+      // become a 64-bit compare)
+      const uint64_t lt = (uint64_t)((int64_t)((~xa & xb) | (~(xa ^ xb) & d)) >> 63);
+      xb ^= lt & (xa ^ xb);  // b <- a on a swap
+      xa = (d ^ lt) - lt;    // a - b after the swap: -d on a swap, else d
+      const uint64_t tp = lt & (p0 ^ p1);
+      p0 ^= tp;
+      p1 ^= tp;
+      p0 -= p1;
     }
-    f1 <<= 1;
-    g1 <<= 1;
+    xa >>= 1;
+    p1 <<= 1;
   }
+  int64_t f0 = (int32_t)(uint32_t)p0, f1 = (int32_t)(uint32_t)p1;
+  int64_t g0 = (int64_t)(p0 - (uint64_t)f0) >> 32, g1 = (int64_t)(p1 - (uint64_t)f1) >> 32;
+  if (xa & 1) {
+    if (xa < xb) {
+      int64_t tf = f0; f0 = f1; f1 = tf;
+      int64_t tg = g0; g0 = g1; g1 = tg;
+    }
+    f0 -= f1;
+    g0 -= g1;
+  }
+  f1 *= 2;
+  g1 *= 2;
   m[0] = f0;
   m[1] = g0;
   m[2] = f1;
