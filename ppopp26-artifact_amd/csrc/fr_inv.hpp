@@ -59,28 +59,16 @@ PL_HD uint64_t bits33(const uint32_t* x, int s) {
   return v & ((1ull << 33) - 1);
 }
 
-// acc += x * f  (x: 8 limbs unsigned, f signed, |f| <= 2^31)
-PL_HD void add_mul(S10& acc, const uint32_t* x, int64_t f) {
-  const bool neg = f < 0;
-  const uint64_t af = neg ? (uint64_t)(-f) : (uint64_t)f;
-  // p = x * |f| as 9 limbs
-  uint32_t p[10];
+// p = x * a as 10 limbs (x: 8 limbs unsigned, a <= 2^31; limb 9 zero)
+PL_HD void mul_u32(const uint32_t* x, uint64_t a, uint32_t* p) {
   uint64_t c = 0;
   for (int i = 0; i < 8; i++) {
-    uint64_t t = (uint64_t)x[i] * af + c;  // af < 2^32: no overflow
+    const uint64_t t = (uint64_t)x[i] * a + c;  // a < 2^32: no overflow
     p[i] = (uint32_t)t;
     c = t >> 32;
   }
   p[8] = (uint32_t)c;
   p[9] = 0;
-  // acc + (neg ? -p : p) in one carry chain: -p = ~p + 1 over 10 limbs
-  const uint32_t mk = neg ? 0xffffffffu : 0u;
-  uint64_t cc = neg ? 1 : 0;
-  for (int i = 0; i < 10; i++) {
-    uint64_t t = (uint64_t)acc.w[i] + (p[i] ^ mk) + cc;
-    acc.w[i] = (uint32_t)t;
-    cc = t >> 32;
-  }
 }
 
 PL_HD bool s10_neg(const S10& a) { return (a.w[9] >> 31) != 0; }
@@ -189,11 +177,13 @@ PL_HD void inner_steps(const uint32_t* a, const uint32_t* b, int64_t* m) {
 // was negated (fix by negate_mod_r).  The four are independent, so the wave
 // form runs them on four lanes at once.
 // All four roles run one code path (in the wave form, no divergence but the
-// final reduction): t = f x + g y with (x, y) = (a, b) or (u, v); when t < 0
-// the integer rows negate it (~t + 1) and the modular rows add 2^32 r, in one
-// 10-limb pass; the modular rows then add q r, q = -t mod 2^31 (a Montgomery
-// step, q = 0 for the integer rows); both shift right by 31, and the modular
-// rows reduce below r.
+// final reduction), in two 10-limb carry passes: t = f x + g y with (x, y) =
+// (a, b) or (u, v), as (x |f| ^ m_f) + (y |g| ^ m_g) + [f < 0] + [g < 0]
+// (two's complement); then, when t < 0, the integer rows negate it
+// (~t + 1) and the modular rows add 2^32 r, in the same pass as the
+// modular rows' q r, q = -t mod 2^31 (a Montgomery step; adding 2^32 r
+// leaves the low word, and q, as they are; q = 0 for the integer rows);
+// both shift right by 31, and the modular rows reduce below r.
 PL_HD void round_role(int role, const uint32_t* a, const uint32_t* b, const uint32_t* u, const uint32_t* v,
                       const int64_t* m, uint32_t* res, bool* neg) {
   const bool modr = role >= 2;
@@ -204,28 +194,32 @@ PL_HD void round_role(int role, const uint32_t* a, const uint32_t* b, const uint
     x[i] = modr ? u[i] : a[i];
     y[i] = modr ? v[i] : b[i];
   }
-  S10 t = {};
-  add_mul(t, x, f);
-  add_mul(t, y, g);
+  const bool nf = f < 0, ng = g < 0;
+  uint32_t px[10], py[10];
+  mul_u32(x, nf ? (uint64_t)(-f) : (uint64_t)f, px);
+  mul_u32(y, ng ? (uint64_t)(-g) : (uint64_t)g, py);
+  S10 t;
+  {
+    const uint32_t mf = nf ? 0xffffffffu : 0u, mg = ng ? 0xffffffffu : 0u;
+    uint64_t c = (uint64_t)nf + (uint64_t)ng;
+    for (int i = 0; i < 10; i++) {
+      const uint64_t s = (uint64_t)(px[i] ^ mf) + (py[i] ^ mg) + c;
+      t.w[i] = (uint32_t)s;
+      c = s >> 32;
+    }
+  }
   const bool tneg = s10_neg(t);
   *neg = !modr && tneg;
   {
     const uint32_t flip = *neg ? 0xffffffffu : 0u;
     const bool addr = modr && tneg;  // |t| < 2^32 r: t + 2^32 r >= 0
+    // q = -t mod 2^31 makes t + q r divisible by 2^31 (r^-1 = 1 mod 2^31)
+    const uint32_t q = modr ? (0u - t.w[0]) & 0x7fffffffu : 0u;
     uint64_t c = *neg ? 1 : 0;
     for (int i = 0; i < 10; i++) {
+      // below 2^32 + 2^63 + 2^32 + 2^32: no overflow
       const uint64_t rr = (addr && i >= 1 && i <= 8) ? kR[i - 1] : 0;
-      const uint64_t s = (uint64_t)(t.w[i] ^ flip) + rr + c;
-      t.w[i] = (uint32_t)s;
-      c = s >> 32;
-    }
-  }
-  // q = -t mod 2^31 makes t + q r divisible by 2^31 (r^-1 = 1 mod 2^31)
-  const uint32_t q = modr ? (0u - t.w[0]) & 0x7fffffffu : 0u;
-  {
-    uint64_t c = 0;
-    for (int i = 0; i < 10; i++) {
-      const uint64_t s = (uint64_t)t.w[i] + c + (i < 8 ? (uint64_t)q * kR[i] : 0);
+      const uint64_t s = (uint64_t)(t.w[i] ^ flip) + (i < 8 ? (uint64_t)q * kR[i] : 0) + rr + c;
       t.w[i] = (uint32_t)s;
       c = s >> 32;
     }

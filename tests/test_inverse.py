@@ -134,7 +134,7 @@ def _steps_if_form(a, b):
 def test_inner_steps_match_the_stated_steps(tmp_path):
     # inner_steps (its 64-bit approximations taken limb by limb) against the
     # steps on Python integers, on inputs that drive the factors to their
-    # extremes: row 1 reaches +-2^31, the bound add_mul and mod_update accept,
+    # extremes: row 1 reaches +-2^31, the factor bound the round updates accept,
     # and row 0 passes 2^30 in magnitude; and on random pairs
     src = tmp_path / "steps_host.cpp"
     src.write_text(
