@@ -9,7 +9,10 @@
 // full values: a <- (f0 a + g0 b) / 2^31, b <- (f1 a + g1 b) / 2^31 (exact;
 // negated with its row when negative), and u, v by the same combination
 // divided by 2^31 modulo r (a Montgomery step: r = 1 mod 2^32).  After
-// 17 * 31 >= 2 * 255 - 1 steps b = gcd = 1 and v = y^-1 mod r.
+// 17 * 31 >= 2 * 255 - 1 steps b = gcd = 1 and v = y^-1 mod r.  The rounds
+// stop as soon as a = 0: b (odd throughout) is then gcd(y, r) = 1, and a
+// further round only halves a = 0 and doubles row 1, so b' = b and
+// v' = 2^31 v / 2^31 = v; random inputs converge in about 12 or 13 rounds.
 //
 // Plain integer code for host and device (tested on the host against
 // Fermat's little theorem).  `inverse` runs on one thread; `inverse_wave`
@@ -262,12 +265,15 @@ PL_HD void inverse(const uint32_t* y, uint32_t* out) {
     for (int role = 0; role < 4; role++) round_role(role, a, b, u, v, m, res[role], &neg[role]);
     if (neg[0]) negate_mod_r(res[2]);
     if (neg[1]) negate_mod_r(res[3]);
+    uint32_t az = 0;
     for (int i = 0; i < 8; i++) {
       a[i] = res[0][i];
       b[i] = res[1][i];
       u[i] = res[2][i];
       v[i] = res[3][i];
+      az |= a[i];
     }
+    if (!az) break;  // converged: later rounds leave b = 1 and v as they are
   }
   for (int i = 0; i < 8; i++) out[i] = v[i];
 }
@@ -301,12 +307,15 @@ __device__ __forceinline__ void inverse_wave(const uint32_t* y, uint32_t* out) {
     round_role(role, a, b, u, v, m, res, &neg);
     const int n0 = __builtin_amdgcn_readlane((int)neg, 0), n1 = __builtin_amdgcn_readlane((int)neg, 1);
     if ((role == 2 && n0) || (role == 3 && n1)) negate_mod_r(res);
+    uint32_t az = 0;
     for (int i = 0; i < 8; i++) {
       a[i] = (uint32_t)__builtin_amdgcn_readlane((int)res[i], 0);
       b[i] = (uint32_t)__builtin_amdgcn_readlane((int)res[i], 1);
       u[i] = (uint32_t)__builtin_amdgcn_readlane((int)res[i], 2);
       v[i] = (uint32_t)__builtin_amdgcn_readlane((int)res[i], 3);
+      az |= a[i];
     }
+    if (!az) break;  // converged (a uniform branch: a is the same on every lane)
   }
   for (int i = 0; i < 8; i++) out[i] = v[i];
 }
